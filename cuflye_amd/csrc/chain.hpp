@@ -1,0 +1,84 @@
+// chain.hpp - launch interface of the group-chaining kernels (chain.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/flyehip.h"
+#include "exsort_dev.hpp"
+
+namespace fh
+{
+
+// One query record of a batch.
+struct QueryDesc
+{
+	int64_t curId;       // signed id
+	int64_t filtStart;   // into the batch's filtered-position array
+	int32_t curLen;
+	int32_t filtCount;
+};
+
+// One (query, target) group: hits [hitStart, hitStart + n) of the batch's
+// sorted hit arrays, all with the same target strand.
+struct GroupDesc
+{
+	int64_t hitStart;
+	int64_t extId;       // signed
+	int32_t n;
+	int32_t query;       // index into the batch's QueryDesc array
+	int32_t extLen;
+	int32_t reserved;
+};
+
+// A chain proposal (an OverlapRange before and after the filters).
+struct Proposal
+{
+	int32_t curBegin;
+	int32_t curEnd;
+	int32_t extBegin;
+	int32_t extEnd;
+	int32_t score;
+	int32_t chainLength;
+	float div;
+};
+
+struct ChainArgs
+{
+	flyehip_overlap_params_t params;
+	const GroupDesc* groups;
+	const QueryDesc* queries;
+	const int32_t* order;      // processing order (may be null)
+	int32_t nGroups;
+	int32_t logN;
+	const int32_t* cur;        // hits in (extId, curPos) order
+	const int32_t* ext;
+	const int32_t* filtered;   // per-query ascending filtered positions
+	// per-hit scratch (indexed like the hits)
+	uint64_t* sortKey;
+	uint32_t* sortVal;
+	int32_t* posL;
+	int32_t* posR;
+	Seg* small;
+	int32_t* hcur;
+	int32_t* hext;
+	int32_t* score;
+	int32_t* bt;
+	Proposal* props;
+	Proposal* outProps;
+	// outputs
+	int32_t* groupCount;
+	int32_t* queue;
+	// glibc logf exception table (sorted by argument bits)
+	const uint32_t* logX;
+	const uint32_t* logY;
+};
+
+// Per-hit scratch bytes the chaining kernel needs.
+constexpr size_t kChainScratchPerHit = 8 + 4 + 4 + 4 + sizeof(Seg) + 4 + 4 + 4 + 4 + 2 * sizeof(Proposal);
+
+hipError_t launchChainGroups(const ChainArgs& a, int blocks, hipStream_t stream);
+hipError_t launchWriteRows(const ChainArgs& a, const int64_t* rowStart, flyehip_overlap_t* rows,
+						   hipStream_t stream);
+
+} // namespace fh

@@ -1,0 +1,200 @@
+// dev_common.hpp - shared device helpers for the gfx950 kernels: wave64
+// scans and reductions, k-mer arithmetic on the 2-bit packed read store, the
+// hash used by the open-addressed k-mer table, and the id mapping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/flyehip.h"
+
+#define FH_WAVE 64
+
+namespace fh
+{
+
+// ---- ids (tools/replay_read_to_graph_source_pack.py:85-88) -------------
+__host__ __device__ inline uint32_t toInternal(int64_t sid)
+{
+	return sid > 0 ? static_cast<uint32_t>(2 * (sid - 1)) : static_cast<uint32_t>(2 * (-sid) - 1);
+}
+__host__ __device__ inline int64_t toSigned(uint32_t iid)
+{
+	return (iid & 1U) ? -static_cast<int64_t>((iid + 1) / 2) : static_cast<int64_t>(iid / 2 + 1);
+}
+
+// ---- wave64 helpers ------------------------------------------------------
+__device__ inline int laneId() { return static_cast<int>(__lane_id()); }
+
+__device__ inline int waveSum(int v)
+{
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, FH_WAVE);
+	return v;
+}
+__device__ inline int waveMax(int v)
+{
+	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, FH_WAVE));
+	return v;
+}
+__device__ inline int waveMin(int v)
+{
+	for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, FH_WAVE));
+	return v;
+}
+// inclusive prefix sum over lanes 0..63
+__device__ inline int waveInclusiveSum(int v)
+{
+	const int l = laneId();
+	for (int o = 1; o < FH_WAVE; o <<= 1)
+	{
+		int t = __shfl_up(v, o, FH_WAVE);
+		if (l >= o) v += t;
+	}
+	return v;
+}
+// inclusive prefix max over lanes 0..63
+__device__ inline int waveInclusiveMax(int v)
+{
+	const int l = laneId();
+	for (int o = 1; o < FH_WAVE; o <<= 1)
+	{
+		int t = __shfl_up(v, o, FH_WAVE);
+		if (l >= o) v = max(v, t);
+	}
+	return v;
+}
+__device__ inline long long waveInclusiveSum64(long long v)
+{
+	const int l = laneId();
+	for (int o = 1; o < FH_WAVE; o <<= 1)
+	{
+		long long t = __shfl_up(v, o, FH_WAVE);
+		if (l >= o) v += t;
+	}
+	return v;
+}
+
+// Exclusive scan of two counters over a block of NT threads (NT % 64 == 0).
+// `sh` must hold 2 * (NT / 64) ints of LDS.  Returns the exclusive prefix of
+// (a, b) and the block totals.
+template <int NT>
+__device__ inline void blockScan2(int a, int b, int& exA, int& exB, int& totA, int& totB, int* sh)
+{
+	const int w = static_cast<int>(threadIdx.x) / FH_WAVE;
+	int ia = waveInclusiveSum(a);
+	int ib = waveInclusiveSum(b);
+	if (laneId() == FH_WAVE - 1)
+	{
+		sh[2 * w] = ia;
+		sh[2 * w + 1] = ib;
+	}
+	__syncthreads();
+	int baseA = 0, baseB = 0;
+	totA = 0;
+	totB = 0;
+	for (int i = 0; i < NT / FH_WAVE; ++i)
+	{
+		int sa = sh[2 * i], sb = sh[2 * i + 1];
+		if (i < w)
+		{
+			baseA += sa;
+			baseB += sb;
+		}
+		totA += sa;
+		totB += sb;
+	}
+	exA = baseA + ia - a;
+	exB = baseB + ib - b;
+	__syncthreads();
+}
+
+template <int NT>
+__device__ inline int blockSum(int a, int* sh)
+{
+	const int w = static_cast<int>(threadIdx.x) / FH_WAVE;
+	int s = waveSum(a);
+	if (laneId() == 0) sh[w] = s;
+	__syncthreads();
+	int t = 0;
+	for (int i = 0; i < NT / FH_WAVE; ++i) t += sh[i];
+	__syncthreads();
+	return t;
+}
+
+// ---- k-mers on the packed store -------------------------------------------
+// Read n's forward strand is stored 16 bases per uint32 word, MSB first
+// (base 0 in bits 31..30), starting at word wordOff[n]; two spare words
+// follow every read so a window never reads past the allocation.
+__device__ inline uint64_t kmerAt(const uint32_t* words, int64_t wordOff, int32_t pos, int k)
+{
+	const uint32_t* w = words + wordOff + (pos >> 4);
+	const int off = pos & 15;  // bases into the first word
+	uint64_t hi = (static_cast<uint64_t>(w[0]) << 32) | w[1];
+	if (off + k <= 32)
+	{
+		uint64_t v = hi >> (2 * (32 - off - k));
+		return k == 32 ? v : (v & ((1ULL << (2 * k)) - 1));
+	}
+	// k up to 31 with off up to 15: take a third word
+	const int need = off + k - 32;  // bases from w[2]
+	uint64_t a = (hi << (2 * need)) | (static_cast<uint64_t>(w[2]) >> (2 * (16 - need)));
+	return a & ((1ULL << (2 * k)) - 1);
+}
+
+__host__ __device__ inline uint64_t rcKmer(uint64_t x, int k)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	uint64_t v = ~x;
+	uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+	uint64_t r = (static_cast<uint64_t>(__builtin_bitreverse32(lo)) << 32) | __builtin_bitreverse32(hi);
+	r = ((r >> 1) & 0x5555555555555555ULL) | ((r & 0x5555555555555555ULL) << 1);
+	return r >> (64 - 2 * k);
+#else
+	uint64_t r = 0;
+	for (int i = 0; i < k; ++i)
+	{
+		r = (r << 2) | (~x & 3ULL);
+		x >>= 2;
+	}
+	return r;
+#endif
+}
+
+// Kmer::standardForm: canonical = min(x, rc); flipped unless x < rc.
+__host__ __device__ inline uint64_t canonical(uint64_t x, int k, bool& flip)
+{
+	uint64_t r = rcKmer(x, k);
+	flip = !(x < r);
+	return flip ? r : x;
+}
+
+// 64-bit mixer (splitmix64 finaliser) for the open-addressed table.
+__host__ __device__ inline uint64_t hashKmer(uint64_t x)
+{
+	x ^= x >> 30;
+	x *= 0xbf58476d1ce4e5b9ULL;
+	x ^= x >> 27;
+	x *= 0x94d049bb133111ebULL;
+	x ^= x >> 31;
+	return x;
+}
+
+constexpr uint64_t kEmptyKey = ~0ULL;
+constexpr uint64_t kRepMark = ~0ULL;
+constexpr int kCountBits = 24;
+constexpr uint64_t kCountMask = (1ULL << kCountBits) - 1;
+
+// Probe for `key`; returns the slot or -1 when absent.
+__device__ inline int64_t findSlot(const uint64_t* keys, uint64_t mask, uint64_t key)
+{
+	uint64_t h = hashKmer(key) & mask;
+	for (;;)
+	{
+		uint64_t k = keys[h];
+		if (k == key) return static_cast<int64_t>(h);
+		if (k == kEmptyKey) return -1;
+		h = (h + 1) & mask;
+	}
+}
+
+} // namespace fh

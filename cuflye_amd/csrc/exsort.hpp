@@ -1,0 +1,301 @@
+// exsort.hpp - exact emulation of libstdc++'s std::sort (introsort) ordering.
+//
+// Flye sorts with std::sort at every order-sensitive point of the overlap
+// path (k-mer matches by (extId, curPos), a target group by extPos, DP rows by
+// score, proposals by score, read-to-graph alignments by curBegin, chains by
+// score).  std::sort is not stable, and on equal keys its output order is
+// whatever introsort's partition swaps leave behind; the chaining DP, the
+// backtrack and onlyMaxExt all read that order.  Bit-exact results therefore
+// need the same permutation, not just a sorted array.
+//
+// The steps reproduced are those of libstdc++'s introsort as shipped since
+// GCC 4.x (bits/stl_algo.h, unchanged through GCC 13): threshold 16; pivot by
+// median of (first+1, mid, last-1) moved to *first; unguarded Hoare partition
+// of [first+1, last) around *first; recursion on the right part, loop on the
+// left; depth limit 2*floor(log2 n), after which the range is heap-sorted
+// (make_heap + sort_heap); a final insertion sort over the whole range.  Each
+// step is rewritten here from that published description; the tests compare
+// every variant against the real std::sort of the host libstdc++.
+//
+// Two consequences used by the GPU code:
+//  * The final insertion sort is stable and every leaf range left by the
+//    introsort loop is bounded by its pivots, so the final pass equals a
+//    stable sort of each leaf (equivalently of the whole range).
+//  * One unguarded partition is a pure function of two flag sequences:
+//    ge[i] = !(a[i] < p) and le[i] = !(p < a[i]) over [first+1, last).  The
+//    k-th ge position from the left (L[k]) swaps with the k-th le position from
+//    the right (R[k]) while L[k] < R[k]; with m such swaps the cut is L[0]
+//    when m == 0, else min(L[m], R[m-1]) (L[m] = +inf when it does not exist).
+//    A ge element at i with left rank kl swaps iff kl < (number of le elements
+//    right of i); an le element at j with right rank kr swaps iff
+//    kr < (number of ge elements left of j).  Both need only prefix counts,
+//    so a partition runs as scans over all lanes.
+//
+// Keys are uint64 (every sort of the path maps its comparator onto an
+// ascending uint64 key); values are uint32 payloads moved with their keys.
+
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define EXS_HD __host__ __device__ inline
+#else
+#define EXS_HD inline
+#endif
+
+namespace exsort
+{
+
+constexpr int kThreshold = 16;
+
+EXS_HD int floorLog2(uint32_t n)
+{
+	int r = 0;
+	while (n > 1)
+	{
+		n >>= 1;
+		++r;
+	}
+	return r;
+}
+
+EXS_HD int depthLimit(uint32_t n) { return 2 * floorLog2(n); }
+
+// Array view over separate key/value arrays (global or LDS memory).
+struct Span
+{
+	uint64_t* k;
+	uint32_t* v;
+	EXS_HD void swap(int a, int b) const
+	{
+		uint64_t tk = k[a];
+		k[a] = k[b];
+		k[b] = tk;
+		uint32_t tv = v[a];
+		v[a] = v[b];
+		v[b] = tv;
+	}
+};
+
+// move_median_to_first(result, a, b, c)
+EXS_HD void medianToFirst(const Span& s, int result, int a, int b, int c)
+{
+	const uint64_t ka = s.k[a], kb = s.k[b], kc = s.k[c];
+	if (ka < kb)
+	{
+		if (kb < kc) s.swap(result, b);
+		else if (ka < kc) s.swap(result, c);
+		else s.swap(result, a);
+	}
+	else if (ka < kc) s.swap(result, a);
+	else if (kb < kc) s.swap(result, c);
+	else s.swap(result, b);
+}
+
+// Hoare partition of [first, last) around the value at `pivot` (unguarded).
+EXS_HD int unguardedPartition(const Span& s, int first, int last, int pivot)
+{
+	const uint64_t p = s.k[pivot];
+	for (;;)
+	{
+		while (s.k[first] < p) ++first;
+		--last;
+		while (p < s.k[last]) --last;
+		if (!(first < last)) return first;
+		s.swap(first, last);
+		++first;
+	}
+}
+
+EXS_HD int partitionPivot(const Span& s, int first, int last)
+{
+	int mid = first + (last - first) / 2;
+	medianToFirst(s, first, first + 1, mid, last - 1);
+	return unguardedPartition(s, first + 1, last, first);
+}
+
+// __push_heap: sift `value` up from holeIndex toward topIndex.
+EXS_HD void pushHeap(const Span& s, int base, int hole, int top, uint64_t vk, uint32_t vv)
+{
+	int parent = (hole - 1) / 2;
+	while (hole > top && s.k[base + parent] < vk)
+	{
+		s.k[base + hole] = s.k[base + parent];
+		s.v[base + hole] = s.v[base + parent];
+		hole = parent;
+		parent = (hole - 1) / 2;
+	}
+	s.k[base + hole] = vk;
+	s.v[base + hole] = vv;
+}
+
+// __adjust_heap over heap [base, base + len).
+EXS_HD void adjustHeap(const Span& s, int base, int hole, int len, uint64_t vk, uint32_t vv)
+{
+	const int top = hole;
+	int child = hole;
+	while (child < (len - 1) / 2)
+	{
+		child = 2 * (child + 1);
+		if (s.k[base + child] < s.k[base + child - 1]) --child;
+		s.k[base + hole] = s.k[base + child];
+		s.v[base + hole] = s.v[base + child];
+		hole = child;
+	}
+	if ((len & 1) == 0 && child == (len - 2) / 2)
+	{
+		child = 2 * (child + 1);
+		s.k[base + hole] = s.k[base + child - 1];
+		s.v[base + hole] = s.v[base + child - 1];
+		hole = child - 1;
+	}
+	pushHeap(s, base, hole, top, vk, vv);
+}
+
+// partial_sort(first, last, last): make_heap then sort_heap.
+EXS_HD void heapSort(const Span& s, int first, int last)
+{
+	const int len = last - first;
+	if (len < 2) return;
+	for (int parent = (len - 2) / 2;; --parent)
+	{
+		uint64_t vk = s.k[first + parent];
+		uint32_t vv = s.v[first + parent];
+		adjustHeap(s, first, parent, len, vk, vv);
+		if (parent == 0) break;
+	}
+	for (int end = last; end - first > 1;)
+	{
+		--end;
+		uint64_t vk = s.k[end];
+		uint32_t vv = s.v[end];
+		s.k[end] = s.k[first];
+		s.v[end] = s.v[first];
+		adjustHeap(s, first, 0, end - first, vk, vv);
+	}
+}
+
+// Stable insertion sort of [first, last): the final pass of std::sort.
+EXS_HD void insertionSort(const Span& s, int first, int last)
+{
+	for (int i = first + 1; i < last; ++i)
+	{
+		uint64_t vk = s.k[i];
+		uint32_t vv = s.v[i];
+		int j = i;
+		while (j > first && vk < s.k[j - 1])
+		{
+			s.k[j] = s.k[j - 1];
+			s.v[j] = s.v[j - 1];
+			--j;
+		}
+		s.k[j] = vk;
+		s.v[j] = vv;
+	}
+}
+
+// The introsort loop on [first, last) with a given remaining depth budget,
+// without the final insertion pass.  Uses an explicit stack (the recursion
+// goes to the right part, the loop continues on the left part; the order in
+// which disjoint ranges are processed does not change the result).
+template <int kStack = 64>
+EXS_HD void introLoop(const Span& s, int first, int last, int depth)
+{
+	int stF[kStack];
+	int stL[kStack];
+	int stD[kStack];
+	int top = 0;
+	stF[top] = first;
+	stL[top] = last;
+	stD[top] = depth;
+	++top;
+	while (top > 0)
+	{
+		--top;
+		int f = stF[top];
+		int l = stL[top];
+		int d = stD[top];
+		while (l - f > kThreshold)
+		{
+			if (d == 0)
+			{
+				heapSort(s, f, l);
+				break;
+			}
+			--d;
+			int cut = partitionPivot(s, f, l);
+			// right part [cut, l) with depth d is independent of the left part
+			stF[top] = cut;
+			stL[top] = l;
+			stD[top] = d;
+			++top;
+			l = cut;
+		}
+	}
+}
+
+// Whole std::sort on [first, last): introsort loop + final insertion sort.
+EXS_HD void stdSort(const Span& s, int first, int last)
+{
+	if (last - first < 2) return;
+	introLoop(s, first, last, depthLimit(static_cast<uint32_t>(last - first)));
+	insertionSort(s, first, last);
+}
+
+// The same partition computed from rank arithmetic instead of the scanning
+// loop; the GPU runs this formulation with scans over many lanes.  Host
+// reference implementation (sequential), used by the tests to check the
+// formulation against unguardedPartition.
+inline int partitionByRanks(const Span& s, int first, int last, int* posL, int* posR)
+{
+	// caller has already moved the median to `first`
+	const uint64_t p = s.k[first];
+	const int lo = first + 1;
+	int geTotal = 0, leTotal = 0;
+	for (int i = lo; i < last; ++i)
+	{
+		geTotal += !(s.k[i] < p);
+		leTotal += !(p < s.k[i]);
+	}
+	int geLeft = 0, leLeft = 0, m = 0, lm = -1;
+	for (int i = lo; i < last; ++i)
+	{
+		const bool ge = !(s.k[i] < p);
+		const bool le = !(p < s.k[i]);
+		const int leRightOfI = leTotal - leLeft - (le ? 1 : 0);
+		if (ge)
+		{
+			if (geLeft < leRightOfI)
+			{
+				posL[geLeft] = i;
+				++m;
+			}
+			else if (geLeft == m && lm < 0)
+			{
+				lm = i;
+			}
+		}
+		if (le)
+		{
+			const int kr = leRightOfI;
+			if (kr < geLeft) posR[kr] = i;
+		}
+		geLeft += ge;
+		leLeft += le;
+	}
+	int cut;
+	if (m == 0)
+	{
+		cut = lm;
+	}
+	else
+	{
+		cut = posR[m - 1];
+		if (lm >= 0 && lm < cut) cut = lm;
+	}
+	for (int t = 0; t < m; ++t) s.swap(posL[t], posR[t]);
+	return cut;
+}
+
+} // namespace exsort

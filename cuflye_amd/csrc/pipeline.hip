@@ -1,0 +1,625 @@
+// pipeline.hip - gfx950 kernels of the k-mer stage of getSeqOverlaps:
+// the VertexIndex build (count in an open-addressed HBM hash table, histogram,
+// thresholds applied, bucket fill in (read, position) order), the query k-mer
+// join in Flye's emission order (window count, scan, fill), the per-query exact
+// std::sort of the matches, the group split, divStats, and the device scan.
+//
+// Data layout in HBM (all sizes per element):
+//   packed reads   2 bits/base, 16 bases per uint32 word, each read word aligned
+//   hash table     keys u64 + vals u64 + fill cursor u32 per slot (power-of-two
+//                  slots, linear probing, splitmix64 finaliser as the hash)
+//   bucket entries u64 (internal id << 32 | position), bucket = contiguous run
+//   join windows   count i32 + filtered i32 + slot/flip u32 per query window
+//   matches        key u64 (internal extId << 32 | curPos) + extPos i32
+#include "pipeline.hpp"
+
+namespace fh
+{
+
+namespace
+{
+
+constexpr int kScanNT = 256;
+constexpr int kScanIT = 16;
+constexpr int kScanTile = kScanNT * kScanIT;
+
+__device__ inline int64_t blockExclusiveScan64(int64_t v, int64_t& total, int64_t* sh)
+{
+	const int w = static_cast<int>(threadIdx.x) / FH_WAVE;
+	const int nw = static_cast<int>(blockDim.x) / FH_WAVE;
+	int64_t incl = waveInclusiveSum64(v);
+	if (laneId() == FH_WAVE - 1) sh[w] = incl;
+	__syncthreads();
+	int64_t base = 0;
+	total = 0;
+	for (int i = 0; i < nw; ++i)
+	{
+		if (i < w) base += sh[i];
+		total += sh[i];
+	}
+	__syncthreads();
+	return base + incl - v;
+}
+
+__global__ void __launch_bounds__(kScanNT) scanReduceKernel(const int32_t* in, int64_t n, int64_t* blockSums)
+{
+	__shared__ int64_t sh[kScanNT / FH_WAVE];
+	const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kScanTile;
+	int64_t s = 0;
+	for (int i = 0; i < kScanIT; ++i)
+	{
+		const int64_t idx = b0 + static_cast<int64_t>(i) * kScanNT + threadIdx.x;
+		if (idx < n) s += in[idx];
+	}
+	int64_t total;
+	blockExclusiveScan64(s, total, sh);
+	if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) scanBlockSumsKernel(int64_t* sums, int64_t nb)
+{
+	__shared__ int64_t sh[1024 / FH_WAVE];
+	const int64_t per = (nb + 1023) / 1024;
+	const int64_t b = static_cast<int64_t>(threadIdx.x) * per;
+	const int64_t e = min(nb, b + per);
+	int64_t s = 0;
+	for (int64_t i = b; i < e; ++i) s += sums[i];
+	int64_t total;
+	int64_t ex = blockExclusiveScan64(s, total, sh);
+	for (int64_t i = b; i < e; ++i)
+	{
+		const int64_t v = sums[i];
+		sums[i] = ex;
+		ex += v;
+	}
+	if (threadIdx.x == 0) sums[nb] = total;
+}
+
+__global__ void __launch_bounds__(kScanNT) scanDownKernel(const int32_t* in, int64_t* out, int64_t n,
+														  const int64_t* blockSums, int64_t nb)
+{
+	__shared__ int64_t sh[kScanNT / FH_WAVE];
+	const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kScanIT;
+	int32_t v[kScanIT];
+	int64_t s = 0;
+	for (int i = 0; i < kScanIT; ++i)
+	{
+		const int64_t idx = b0 + i;
+		v[i] = idx < n ? in[idx] : 0;
+		s += v[i];
+	}
+	int64_t total;
+	int64_t ex = blockExclusiveScan64(s, total, sh) + blockSums[blockIdx.x];
+	for (int i = 0; i < kScanIT; ++i)
+	{
+		const int64_t idx = b0 + i;
+		if (idx < n) out[idx] = ex;
+		ex += v[i];
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = blockSums[nb];
+}
+
+__device__ inline int findSeg(const int64_t* off, int n, int64_t w)
+{
+	// largest r in [0, n) with off[r] <= w
+	int lo = 0, hi = n - 1;
+	while (lo < hi)
+	{
+		const int mid = (lo + hi + 1) >> 1;
+		if (off[mid] <= w) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+
+__global__ void countKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin, int k, HashTable t)
+{
+	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (w >= nWin) return;
+	const int r = findSeg(winOff, reads.n, w);
+	const int32_t pos = static_cast<int32_t>(w - winOff[r]);
+	bool flip;
+	const uint64_t c = canonical(kmerAt(reads.words, reads.wordOff[r], pos, k), k, flip);
+	uint64_t h = hashKmer(c) & t.mask;
+	for (;;)
+	{
+		const uint64_t key = t.keys[h];
+		if (key == c)
+		{
+			atomicAdd(reinterpret_cast<unsigned long long*>(&t.vals[h]), 1ULL);
+			return;
+		}
+		if (key == kEmptyKey)
+		{
+			const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&t.keys[h]),
+											static_cast<unsigned long long>(kEmptyKey),
+											static_cast<unsigned long long>(c));
+			if (prev == kEmptyKey || prev == c)
+			{
+				atomicAdd(reinterpret_cast<unsigned long long*>(&t.vals[h]), 1ULL);
+				return;
+			}
+		}
+		h = (h + 1) & t.mask;
+	}
+}
+
+constexpr int kHistLds = 1024;
+
+__global__ void __launch_bounds__(256) histKernel(HashTable t, uint64_t* hist, int hmax, uint64_t* acc)
+{
+	__shared__ unsigned int lh[kHistLds];
+	__shared__ unsigned int distinct;
+	for (int i = threadIdx.x; i < kHistLds; i += blockDim.x) lh[i] = 0;
+	if (threadIdx.x == 0) distinct = 0;
+	__syncthreads();
+	const uint64_t slots = t.mask + 1;
+	unsigned int myDistinct = 0;
+	for (uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; h < slots;
+		 h += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+	{
+		if (t.keys[h] == kEmptyKey) continue;
+		++myDistinct;
+		const uint64_t c = t.vals[h];
+		if (c < static_cast<uint64_t>(kHistLds)) atomicAdd(&lh[c], 1U);
+		else if (c < static_cast<uint64_t>(hmax)) atomicAdd(reinterpret_cast<unsigned long long*>(&hist[c]), 1ULL);
+		else
+		{
+			atomicAdd(reinterpret_cast<unsigned long long*>(&acc[0]), static_cast<unsigned long long>(c));
+			atomicAdd(reinterpret_cast<unsigned long long*>(&acc[1]), 1ULL);
+		}
+	}
+	atomicAdd(&distinct, myDistinct);
+	__syncthreads();
+	for (int i = threadIdx.x; i < kHistLds && i < hmax; i += blockDim.x)
+		if (lh[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&hist[i]), static_cast<unsigned long long>(lh[i]));
+	if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&acc[2]), static_cast<unsigned long long>(distinct));
+}
+
+__global__ void classifyKernel(HashTable t, int solidMin, int repMax, int32_t* sizes, uint64_t* counters)
+{
+	const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (h > t.mask) return;
+	int32_t size = 0;
+	if (t.keys[h] != kEmptyKey)
+	{
+		const uint64_t c = t.vals[h];
+		if (c >= static_cast<uint64_t>(solidMin) && c <= static_cast<uint64_t>(repMax))
+		{
+			size = static_cast<int32_t>(c);
+			atomicAdd(reinterpret_cast<unsigned long long*>(&counters[0]), 1ULL);
+		}
+		else if (c >= static_cast<uint64_t>(solidMin))
+		{
+			t.vals[h] = kRepMark;
+			atomicAdd(reinterpret_cast<unsigned long long*>(&counters[1]), 1ULL);
+		}
+		else
+		{
+			t.vals[h] = 0;
+		}
+	}
+	sizes[h] = size;
+	t.cursor[h] = 0;
+}
+
+__global__ void assignKernel(HashTable t, const int32_t* sizes, const int64_t* starts)
+{
+	const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (h > t.mask) return;
+	const int32_t s = sizes[h];
+	if (s > 0) t.vals[h] = (static_cast<uint64_t>(starts[h]) << kCountBits) | static_cast<uint64_t>(s);
+}
+
+__device__ inline bool indexed(uint64_t v) { return v != 0 && v != kRepMark; }
+
+__global__ void fillKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
+						   uint64_t* ordKeys)
+{
+	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (w >= nWin) return;
+	const int r = findSeg(winOff, reads.n, w);
+	const int32_t pos = static_cast<int32_t>(w - winOff[r]);
+	bool flip;
+	const uint64_t c = canonical(kmerAt(reads.words, reads.wordOff[r], pos, k), k, flip);
+	const int64_t h = findSlot(t.keys, t.mask, c);
+	if (h < 0) return;
+	const uint64_t v = t.vals[h];
+	if (!indexed(v)) return;
+	const uint32_t slot = atomicAdd(&t.cursor[h], 1U);
+	const uint64_t start = v >> kCountBits;
+	ordKeys[start + slot] = (static_cast<uint64_t>(r) << 33) | (static_cast<uint64_t>(pos) << 1) | (flip ? 1ULL : 0ULL);
+}
+
+__global__ void bucketFinishKernel(HashTable t, uint64_t* ordKeys, uint64_t* entries, const int32_t* readLen, int k)
+{
+	const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (h > t.mask) return;
+	const uint64_t v = t.vals[h];
+	if (!indexed(v)) return;
+	const uint64_t start = v >> kCountBits;
+	const int n = static_cast<int>(v & kCountMask);
+	uint64_t* o = ordKeys + start;
+	for (int i = 1; i < n; ++i)
+	{
+		const uint64_t x = o[i];
+		int j = i;
+		while (j > 0 && o[j - 1] > x)
+		{
+			o[j] = o[j - 1];
+			--j;
+		}
+		o[j] = x;
+	}
+	for (int i = 0; i < n; ++i)
+	{
+		const uint64_t x = o[i];
+		const uint32_t r = static_cast<uint32_t>(x >> 33);
+		const int32_t pos = static_cast<int32_t>((x >> 1) & 0xffffffffULL);
+		uint64_t e;
+		if (x & 1ULL) e = (static_cast<uint64_t>(2 * r + 1) << 32) | static_cast<uint32_t>(readLen[r] - pos - k);
+		else e = (static_cast<uint64_t>(2 * r) << 32) | static_cast<uint32_t>(pos);
+		entries[start + i] = e;
+	}
+}
+
+// ---- join -----------------------------------------------------------------
+
+struct WindowKmer
+{
+	int q;
+	int32_t pos;
+	uint64_t y;     // query k-mer (numRepr)
+	uint64_t c;     // canonical
+	bool g;         // flipped
+	uint64_t xf;    // forward-strand k-mer of the same window
+};
+
+__device__ inline WindowKmer windowAt(const JoinArgs& a, int64_t w)
+{
+	WindowKmer r;
+	r.q = findSeg(a.winOff, a.nQueries, w);
+	r.pos = static_cast<int32_t>(w - a.winOff[r.q]);
+	const JoinQuery Q = a.queries[r.q];
+	const int64_t wo = a.seqs.wordOff[Q.seq];
+	if (Q.strand)
+	{
+		r.xf = kmerAt(a.seqs.words, wo, r.pos, a.k);
+		r.y = r.xf;
+	}
+	else
+	{
+		r.xf = kmerAt(a.seqs.words, wo, Q.len - r.pos - a.k, a.k);
+		r.y = rcKmer(r.xf, a.k);
+	}
+	r.c = canonical(r.y, a.k, r.g);
+	return r;
+}
+
+__device__ inline void transformEntry(uint64_t e, bool g, const int32_t* targetLen, int k, uint32_t& id, int32_t& pos)
+{
+	id = static_cast<uint32_t>(e >> 32);
+	pos = static_cast<int32_t>(static_cast<uint32_t>(e));
+	if (g)
+	{
+		id ^= 1U;
+		pos = targetLen[id >> 1] - pos - k;
+	}
+}
+
+__global__ void joinCountKernel(JoinArgs a)
+{
+	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (w >= a.nWin) return;
+	const WindowKmer wk = windowAt(a, w);
+	int32_t count = 0;
+	int32_t filt = 0;
+	uint32_t info = 0xffffffffU;
+	const int64_t h = findSlot(a.table.keys, a.table.mask, wk.c);
+	if (h >= 0)
+	{
+		const uint64_t v = a.table.vals[h];
+		if (v == kRepMark) filt = 1;
+		else if (v != 0)
+		{
+			count = static_cast<int32_t>(v & kCountMask);
+			const JoinQuery Q = a.queries[wk.q];
+			int self = 0;
+			if (Q.selfMode == 1)
+			{
+				// the query read's own stored entry for this window
+				const int32_t pf = Q.strand ? wk.pos : Q.len - wk.pos - a.k;
+				bool ff;
+				canonical(wk.xf, a.k, ff);
+				uint64_t e = ff ? ((static_cast<uint64_t>(2 * Q.seq + 1) << 32) | static_cast<uint32_t>(Q.len - pf - a.k))
+								: ((static_cast<uint64_t>(2 * Q.seq) << 32) | static_cast<uint32_t>(pf));
+				uint32_t id;
+				int32_t p;
+				transformEntry(e, wk.g, a.targetLen, a.k, id, p);
+				self = (id == Q.iid && p == wk.pos) ? 1 : 0;
+			}
+			else if (Q.selfMode == 2)
+			{
+				const uint64_t start = v >> kCountBits;
+				for (int i = 0; i < count; ++i)
+				{
+					uint32_t id;
+					int32_t p;
+					transformEntry(a.entries[start + i], wk.g, a.targetLen, a.k, id, p);
+					if (id == Q.iid && p == wk.pos) self = 1;
+				}
+			}
+			count -= self;
+			info = static_cast<uint32_t>(h) | (wk.g ? 0x80000000U : 0U);
+		}
+	}
+	a.winCount[w] = count;
+	a.winFilt[w] = filt;
+	a.winInfo[w] = info;
+}
+
+__global__ void joinFillKernel(JoinArgs a)
+{
+	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (w >= a.nWin) return;
+	const int q = findSeg(a.winOff, a.nQueries, w);
+	const int32_t pos = static_cast<int32_t>(w - a.winOff[q]);
+	if (a.winFilt[w]) a.filtered[a.filtOff[w]] = pos;
+	if (a.winCount[w] == 0) return;
+	const uint32_t info = a.winInfo[w];
+	const bool g = (info & 0x80000000U) != 0;
+	const uint64_t h = info & 0x7fffffffU;
+	const uint64_t v = a.table.vals[h];
+	const uint64_t start = v >> kCountBits;
+	const int n = static_cast<int>(v & kCountMask);
+	const uint32_t qiid = a.queries[q].iid;
+	const bool maySelf = a.queries[q].selfMode != 0;
+	int64_t o = a.matchOff[w];
+	for (int i = 0; i < n; ++i)
+	{
+		uint32_t id;
+		int32_t p;
+		transformEntry(a.entries[start + i], g, a.targetLen, a.k, id, p);
+		if (maySelf && id == qiid && p == pos) continue;
+		a.mkey[o] = (static_cast<uint64_t>(id) << 32) | static_cast<uint32_t>(pos);
+		a.mext[o] = p;
+		++o;
+	}
+}
+
+__global__ void writeCandidatesKernel(JoinArgs a, const int64_t* qMatchOff, const int64_t* queryIds,
+									  const uint32_t* perm, flyehip_candidate_t* out, int64_t nMatches)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= nMatches) return;
+	const int q = findSeg(qMatchOff, a.nQueries, i);
+	const int64_t qb = qMatchOff[q];
+	const uint64_t key = a.mkey[i];
+	const int32_t ext = perm ? a.mext[qb + perm[i]] : a.mext[i];
+	const uint32_t eid = static_cast<uint32_t>(key >> 32);
+	const int32_t pos = static_cast<int32_t>(static_cast<uint32_t>(key));
+	const JoinQuery Q = a.queries[q];
+	const int64_t wo = a.seqs.wordOff[Q.seq];
+	uint64_t y = Q.strand ? kmerAt(a.seqs.words, wo, pos, a.k) : rcKmer(kmerAt(a.seqs.words, wo, Q.len - pos - a.k, a.k), a.k);
+	flyehip_candidate_t c;
+	c.query_id = queryIds[q];
+	c.target_id = toSigned(eid);
+	c.kmer = y;
+	c.query_pos = pos;
+	c.target_pos = ext;
+	out[i] = c;
+}
+
+// ---- per-query match sort -------------------------------------------------------
+
+constexpr int kSortNT = 256;
+
+__global__ void __launch_bounds__(kSortNT) matchSortKernel(MatchSortArgs a)
+{
+	__shared__ SortShared<kSortNT> sh;
+	__shared__ int ties;
+	const int q = blockIdx.x;
+	const int64_t b = a.qMatchOff[q];
+	const int n = static_cast<int>(a.qMatchOff[q + 1] - b);
+	uint64_t* key = a.key + b;
+	uint32_t* val = a.val + b;
+	for (int i = threadIdx.x; i < n; i += kSortNT) val[i] = static_cast<uint32_t>(i);
+	if (threadIdx.x == 0) ties = 0;
+	__syncthreads();
+	blockStdSort<kSortNT>(exsort::Span{key, val}, n, a.posL + b, a.posR + b, a.small + b, n, 48, sh);
+	int t = 0;
+	for (int i = threadIdx.x + 1; i < n; i += kSortNT) t |= (key[i] == key[i - 1]);
+	if (t) atomicOr(&ties, 1);
+	__syncthreads();
+	if (threadIdx.x == 0 && ties) atomicAdd(a.tieQueries, 1);
+}
+
+__global__ void gatherHitsKernel(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const uint32_t* val,
+								 const int32_t* mext, int32_t* hcur, int32_t* hext, int32_t* groupFlag, int64_t nMatches)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= nMatches) return;
+	const int q = findSeg(qMatchOff, nQueries, i);
+	const int64_t qb = qMatchOff[q];
+	const uint64_t k = key[i];
+	hcur[i] = static_cast<int32_t>(static_cast<uint32_t>(k));
+	hext[i] = mext[qb + val[i]];
+	groupFlag[i] = (i == qb || (key[i - 1] >> 32) != (k >> 32)) ? 1 : 0;
+}
+
+__global__ void buildGroupsKernel(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const int32_t* groupFlag,
+								  const int64_t* groupOff, const int32_t* targetLen, GroupDesc* groups, int64_t nMatches)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= nMatches) return;
+	if (!groupFlag[i]) return;
+	const int64_t g = groupOff[i];
+	const int q = findSeg(qMatchOff, nQueries, i);
+	const uint32_t eid = static_cast<uint32_t>(key[i] >> 32);
+	// group end: next flag position (scan forward; groups are contiguous)
+	int64_t e = i + 1;
+	while (e < nMatches && !groupFlag[e]) ++e;
+	GroupDesc d;
+	d.hitStart = i;
+	d.extId = toSigned(eid);
+	d.n = static_cast<int32_t>(e - i);
+	d.query = q;
+	d.extLen = targetLen[eid >> 1];
+	d.reserved = 0;
+	groups[g] = d;
+}
+
+__global__ void divStatsKernel(const flyehip_overlap_t* rows, const int64_t* qRowOff, const int32_t* curLen,
+							   int32_t nQueries, int32_t* divCount, float* divOut, const int64_t* divOff)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= nQueries) return;
+	const int kWnd = 10000;
+	const int nw = curLen[q] / kWnd + 1;
+	const int64_t rb = qRowOff[q], re = qRowOff[q + 1];
+	int cnt = 0;
+	for (int w = 0; w < nw; ++w)
+	{
+		int bestRange = 0;
+		float bestDiv = 0.0f;
+		for (int64_t r = rb; r < re; ++r)
+		{
+			const flyehip_overlap_t& o = rows[r];
+			if (o.cur_begin < 0) continue;
+			if (o.cur_begin / kWnd != w) continue;
+			const int range = o.cur_end - o.cur_begin;
+			if (range > bestRange)
+			{
+				bestRange = range;
+				bestDiv = o.seq_divergence;
+			}
+		}
+		if (bestRange > 0) divOut[divOff[q] + cnt++] = bestDiv;
+	}
+	divCount[q] = cnt;
+}
+
+inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n + tpb - 1) / tpb)); }
+
+} // namespace
+
+size_t scanTempWords(int64_t n)
+{
+	return static_cast<size_t>((n + kScanTile - 1) / kScanTile + 1);
+}
+
+hipError_t exclusiveScan(const int32_t* in, int64_t* out, int64_t n, int64_t* tmp, hipStream_t s)
+{
+	const int64_t nb = (n + kScanTile - 1) / kScanTile;
+	if (nb == 0)
+	{
+		return hipMemsetAsync(out, 0, sizeof(int64_t), s);
+	}
+	hipLaunchKernelGGL(scanReduceKernel, dim3(static_cast<unsigned>(nb)), dim3(kScanNT), 0, s, in, n, tmp);
+	hipLaunchKernelGGL(scanBlockSumsKernel, dim3(1), dim3(1024), 0, s, tmp, nb);
+	hipLaunchKernelGGL(scanDownKernel, dim3(static_cast<unsigned>(nb)), dim3(kScanNT), 0, s, in, out, n, tmp, nb);
+	return hipGetLastError();
+}
+
+hipError_t launchCount(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t, hipStream_t s)
+{
+	if (nWin == 0) return hipSuccess;
+	hipLaunchKernelGGL(countKernel, gridFor(nWin, 256), dim3(256), 0, s, reads, winOff, nWin, k, t);
+	return hipGetLastError();
+}
+
+hipError_t launchHistogram(HashTable t, uint64_t* hist, int hmax, uint64_t* acc, hipStream_t s)
+{
+	hipLaunchKernelGGL(histKernel, dim3(2048), dim3(256), 0, s, t, hist, hmax, acc);
+	return hipGetLastError();
+}
+
+hipError_t launchClassify(HashTable t, int solidMin, int repMax, int32_t* sizes, uint64_t* counters, hipStream_t s)
+{
+	hipLaunchKernelGGL(classifyKernel, gridFor(static_cast<int64_t>(t.mask + 1), 256), dim3(256), 0, s, t, solidMin,
+					   repMax, sizes, counters);
+	return hipGetLastError();
+}
+
+hipError_t launchAssign(HashTable t, const int32_t* sizes, const int64_t* starts, hipStream_t s)
+{
+	hipLaunchKernelGGL(assignKernel, gridFor(static_cast<int64_t>(t.mask + 1), 256), dim3(256), 0, s, t, sizes, starts);
+	return hipGetLastError();
+}
+
+hipError_t launchFill(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
+					  uint64_t* ordKeys, hipStream_t s)
+{
+	if (nWin == 0) return hipSuccess;
+	hipLaunchKernelGGL(fillKernel, gridFor(nWin, 256), dim3(256), 0, s, reads, winOff, nWin, k, t, ordKeys);
+	return hipGetLastError();
+}
+
+hipError_t launchBucketFinish(HashTable t, uint64_t* ordKeys, uint64_t* entries, const int32_t* readLen, int k,
+							  hipStream_t s)
+{
+	hipLaunchKernelGGL(bucketFinishKernel, gridFor(static_cast<int64_t>(t.mask + 1), 256), dim3(256), 0, s, t, ordKeys,
+					   entries, readLen, k);
+	return hipGetLastError();
+}
+
+hipError_t launchJoinCount(const JoinArgs& a, hipStream_t s)
+{
+	if (a.nWin == 0) return hipSuccess;
+	hipLaunchKernelGGL(joinCountKernel, gridFor(a.nWin, 256), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launchJoinFill(const JoinArgs& a, hipStream_t s)
+{
+	if (a.nWin == 0) return hipSuccess;
+	hipLaunchKernelGGL(joinFillKernel, gridFor(a.nWin, 256), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launchWriteCandidates(const JoinArgs& a, const int64_t* qMatchOff, const int64_t* queryIds,
+								 const uint32_t* perm, flyehip_candidate_t* out, int64_t nMatches, hipStream_t s)
+{
+	if (nMatches == 0) return hipSuccess;
+	hipLaunchKernelGGL(writeCandidatesKernel, gridFor(nMatches, 256), dim3(256), 0, s, a, qMatchOff, queryIds, perm,
+					   out, nMatches);
+	return hipGetLastError();
+}
+
+hipError_t launchMatchSort(const MatchSortArgs& a, hipStream_t s)
+{
+	if (a.nQueries == 0) return hipSuccess;
+	hipLaunchKernelGGL(matchSortKernel, dim3(static_cast<unsigned>(a.nQueries)), dim3(kSortNT), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t launchGatherHits(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const uint32_t* val,
+							const int32_t* mext, int32_t* hcur, int32_t* hext, int32_t* groupFlag, int64_t nMatches,
+							hipStream_t s)
+{
+	if (nMatches == 0) return hipSuccess;
+	hipLaunchKernelGGL(gatherHitsKernel, gridFor(nMatches, 256), dim3(256), 0, s, qMatchOff, nQueries, key, val, mext,
+					   hcur, hext, groupFlag, nMatches);
+	return hipGetLastError();
+}
+
+hipError_t launchBuildGroups(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const int32_t* groupFlag,
+							 const int64_t* groupOff, const int32_t* targetLen, GroupDesc* groups, int64_t nMatches,
+							 hipStream_t s)
+{
+	if (nMatches == 0) return hipSuccess;
+	hipLaunchKernelGGL(buildGroupsKernel, gridFor(nMatches, 256), dim3(256), 0, s, qMatchOff, nQueries, key, groupFlag,
+					   groupOff, targetLen, groups, nMatches);
+	return hipGetLastError();
+}
+
+hipError_t launchDivStats(const flyehip_overlap_t* rows, const int64_t* qRowOff, const int32_t* curLen, int32_t nQueries,
+						  int32_t* divCount, float* divOut, const int64_t* divOff, hipStream_t s)
+{
+	if (nQueries == 0) return hipSuccess;
+	hipLaunchKernelGGL(divStatsKernel, gridFor(nQueries, 128), dim3(128), 0, s, rows, qRowOff, curLen, nQueries,
+					   divCount, divOut, divOff);
+	return hipGetLastError();
+}
+
+} // namespace fh

@@ -1,0 +1,121 @@
+// pipeline.hpp - launch interface of the index, join, match-sort and group
+// kernels (pipeline.hip) and of the device scan.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "chain.hpp"
+
+namespace fh
+{
+
+// 2-bit packed sequences: sequence i starts at word wordOff[i] and has len[i]
+// bases (16 per word, MSB first, two zero words after each sequence).
+struct PackedSeqs
+{
+	const uint32_t* words;
+	const int64_t* wordOff;
+	const int32_t* len;
+	int32_t n;
+};
+
+struct HashTable
+{
+	uint64_t* keys;
+	uint64_t* vals;    // count while counting; then (start << 24 | count), kRepMark or 0
+	uint32_t* cursor;  // fill cursors
+	uint64_t mask;     // slots - 1
+};
+
+// ---- scans ----------------------------------------------------------------
+// Exclusive scan of n int32 values into int64; out has n + 1 entries (the last
+// is the total).  `tmp` needs scanTempWords(n) int64 words.
+size_t scanTempWords(int64_t n);
+hipError_t exclusiveScan(const int32_t* in, int64_t* out, int64_t n, int64_t* tmp, hipStream_t s);
+
+// ---- index build ------------------------------------------------------------
+hipError_t launchCount(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k,
+					   HashTable t, hipStream_t s);
+// hist[0..hmax) counts, plus acc[0] = sum(count) and acc[1] = distinct for
+// counts >= hmax, acc[2] = total distinct.
+hipError_t launchHistogram(HashTable t, uint64_t* hist, int hmax, uint64_t* acc, hipStream_t s);
+// sizes[h] = count for indexed slots, 0 otherwise; marks repetitive slots.
+hipError_t launchClassify(HashTable t, int solidMin, int repMax, int32_t* sizes, uint64_t* counters,
+						  hipStream_t s);
+hipError_t launchAssign(HashTable t, const int32_t* sizes, const int64_t* starts, hipStream_t s);
+hipError_t launchFill(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
+					  uint64_t* ordKeys, hipStream_t s);
+hipError_t launchBucketFinish(HashTable t, uint64_t* ordKeys, uint64_t* entries, const int32_t* readLen,
+							  int k, hipStream_t s);
+
+// ---- join ---------------------------------------------------------------------
+// One query record of a join batch.
+struct JoinQuery
+{
+	int32_t seq;        // index into the packed sequences it is read from
+	int32_t strand;     // 1: forward, 0: reverse complement
+	uint32_t iid;       // internal id used by the self-hit test
+	int32_t selfMode;   // 0: never, 1: resident read (O(1) test), 2: scan bucket
+	int32_t len;
+	int32_t reserved;
+};
+
+struct JoinArgs
+{
+	PackedSeqs seqs;           // where query sequences are read from
+	const JoinQuery* queries;
+	int32_t nQueries;
+	int k;
+	const int64_t* winOff;     // per query window offsets (nQueries + 1)
+	int64_t nWin;
+	HashTable table;
+	const uint64_t* entries;   // (internal id << 32) | position
+	const int32_t* targetLen;  // by read number - 1
+	int32_t* winCount;         // per window: matches
+	int32_t* winFilt;          // per window: 1 if repetitive
+	uint32_t* winInfo;         // slot | flip << 31 (0xffffffff when none)
+	const int64_t* matchOff;   // exclusive scan of winCount (nWin + 1)
+	const int64_t* filtOff;    // exclusive scan of winFilt (nWin + 1)
+	uint64_t* mkey;            // (extIid << 32) | curPos, emission order
+	int32_t* mext;             // extPos, emission order
+	int32_t* filtered;         // filtered positions
+};
+
+hipError_t launchJoinCount(const JoinArgs& a, hipStream_t s);
+hipError_t launchJoinFill(const JoinArgs& a, hipStream_t s);
+
+// Candidate rows for the ABI: emission order (sorted == null) or through the
+// sort permutation.
+hipError_t launchWriteCandidates(const JoinArgs& a, const int64_t* qMatchOff, const int64_t* queryIds,
+								 const uint32_t* perm, flyehip_candidate_t* out, int64_t nMatches,
+								 hipStream_t s);
+
+// ---- per-query exact match sort and grouping ----------------------------------
+struct MatchSortArgs
+{
+	const int64_t* qMatchOff;  // per query match offsets (nQueries + 1)
+	int32_t nQueries;
+	uint64_t* key;             // sorted in place
+	uint32_t* val;             // permutation (emission index within query)
+	int32_t* posL;
+	int32_t* posR;
+	Seg* small;
+	int32_t* tieQueries;       // counter
+};
+hipError_t launchMatchSort(const MatchSortArgs& a, hipStream_t s);
+
+// After the sort: hit arrays, group-start flags.
+hipError_t launchGatherHits(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const uint32_t* val,
+							const int32_t* mext, int32_t* hcur, int32_t* hext, int32_t* groupFlag,
+							int64_t nMatches, hipStream_t s);
+hipError_t launchBuildGroups(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key,
+							 const int32_t* groupFlag, const int64_t* groupOff, const int32_t* targetLen,
+							 GroupDesc* groups, int64_t nMatches, hipStream_t s);
+
+// Per-query divStats windows (patch 0023:245-269) over the written rows.
+hipError_t launchDivStats(const flyehip_overlap_t* rows, const int64_t* qRowOff, const int32_t* curLen,
+						  int32_t nQueries, int32_t* divCount, float* divOut, const int64_t* divOff,
+						  hipStream_t s);
+
+} // namespace fh

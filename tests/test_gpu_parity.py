@@ -1,0 +1,170 @@
+"""GPU parity: libflyehip.so (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Integer fields and the float divergence must match bit
+for bit; candidate lists must match in order (Flye's emission order, and
+std::sort's tie order after the match sort)."""
+import numpy as np
+import pytest
+
+import cuflye_amd.synth as S
+from cuflye_amd import native as N
+from cuflye_amd.overlap import OverlapConfig
+
+pytestmark = pytest.mark.gpu
+
+
+def _bytes(a):
+    return np.ascontiguousarray(a).view(np.uint8)
+
+
+def assert_rows_equal(got, exp, what):
+    assert len(got) == len(exp), f"{what}: {len(got)} rows vs {len(exp)}"
+    if len(got) and not np.array_equal(_bytes(got), _bytes(exp)):
+        bad = np.nonzero(got != exp)[0]
+        i = int(bad[0]) if len(bad) else 0
+        raise AssertionError(f"{what}: first mismatch at row {i}: {got[i]} vs {exp[i]} ({len(bad)} rows differ)")
+
+
+def both_strands(n):
+    ids = np.empty(2 * n, np.int64)
+    ids[0::2] = np.arange(1, n + 1)
+    ids[1::2] = -np.arange(1, n + 1)
+    return ids
+
+
+@pytest.fixture(scope="module")
+def c1(gpu, oracle_mod):
+    r = S.config1(scale=0.1, seed=21)
+    print(f"[c1] reads={r.n_reads} bases={len(r.bases)}", flush=True)
+    orc = oracle_mod.Oracle(r.bases, r.offsets)
+    oinfo = orc.build_index(15)
+    gpu.store_reads(r.bases, r.offsets)
+    ginfo = gpu.build_index(15)
+    return r, orc, oinfo, ginfo
+
+
+def test_index_matches_oracle(c1):
+    r, orc, oinfo, ginfo = c1
+    assert ginfo.solid_min == oinfo["solid_min"]
+    assert ginfo.rep_max == oinfo["rep_max"]
+    assert ginfo.distinct_kmers == oinfo["distinct"]
+    assert ginfo.solid_kmers == oinfo["solid"]
+    assert ginfo.repetitive_kmers == oinfo["repetitive"]
+    assert ginfo.indexed_positions == oinfo["positions"]
+
+
+def test_candidates_emission_order(gpu, c1):
+    r, orc, _, _ = c1
+    ids = np.array([1, -1, 5, -7, r.n_reads, -r.n_reads], np.int64)
+    m, moff, f, foff = gpu.collect_candidates(ids)
+    for i, q in enumerate(ids):
+        em, ef = orc.candidates(int(q), sorted_=False)
+        assert_rows_equal(m[moff[i]:moff[i + 1]], em, f"emission q={q}")
+        assert np.array_equal(f[foff[i]:foff[i + 1]], ef)
+
+
+def test_candidates_sorted_tie_order(gpu, c1):
+    r, orc, _, _ = c1
+    ids = both_strands(min(r.n_reads, 40))
+    m, moff = gpu.collect_sorted_candidates(ids)
+    st = gpu.stats()
+    for i, q in enumerate(ids):
+        em, _ = orc.candidates(int(q), sorted_=True)
+        assert_rows_equal(m[moff[i]:moff[i + 1]], em, f"sorted q={q}")
+    print("tie queries", st["tie_queries"], "of", len(ids))
+
+
+def test_overlaps_config1(gpu, c1):
+    r, orc, _, _ = c1
+    ids = both_strands(r.n_reads)
+    cfg = OverlapConfig(kmer_size=15, minimum_overlap=5000)
+    rows, roff, divs, doff = gpu.get_seq_overlaps(ids, cfg, with_div_stats=True)
+    erows, eroff, edivs, edoff = orc.overlaps(ids, cfg, threads=8, with_div_stats=True)
+    assert np.array_equal(roff, eroff)
+    assert_rows_equal(rows, erows, "config1 overlaps")
+    assert np.array_equal(doff, edoff)
+    assert np.array_equal(divs.view(np.uint32), edivs.view(np.uint32))
+    assert len(rows) > 0
+
+
+def test_overlaps_per_query_path_equal(gpu, c1):
+    r, orc, _, _ = c1
+    ids = both_strands(min(r.n_reads, 12))
+    cfg = OverlapConfig(kmer_size=15, minimum_overlap=5000)
+    a, ao = gpu.get_seq_overlaps(ids, cfg)
+    b, bo = gpu.get_seq_overlaps(ids, cfg, per_query=True)
+    assert np.array_equal(ao, bo)
+    assert_rows_equal(a, b, "batched vs per-query")
+
+
+def test_overlaps_config2_small(gpu, oracle_mod):
+    r = S.config2(scale=0.03, seed=22)
+    print(f"[c2] reads={r.n_reads} bases={len(r.bases)}", flush=True)
+    orc = oracle_mod.Oracle(r.bases, r.offsets)
+    orc.build_index(15)
+    gpu.store_reads(r.bases, r.offsets)
+    gpu.build_index(15)
+    ids = both_strands(r.n_reads)
+    cfg = OverlapConfig(kmer_size=15, minimum_overlap=3000)
+    rows, roff = gpu.get_seq_overlaps(ids, cfg)
+    erows, eroff = orc.overlaps(ids, cfg, threads=8)
+    assert np.array_equal(roff, eroff)
+    assert_rows_equal(rows, erows, "config2 overlaps")
+    # read-to-graph detector shape on the same data: containment filter path
+    cfg2 = OverlapConfig.read_to_graph(15, 1000)
+    rows, roff = gpu.get_seq_overlaps(ids[:60], cfg2)
+    erows, eroff = orc.overlaps(ids[:60], cfg2, threads=8)
+    assert_rows_equal(rows, erows, "config2 read-to-graph shape")
+
+
+def test_chain_lists_config3_small(gpu, oracle_mod):
+    h = S.config3(n_lists=4000, seed=23, max_hits=20000)
+    print(f"[c3] lists={len(h.offsets) - 1} hits={len(h.cur)}", flush=True)
+    cfg = OverlapConfig(kmer_size=15, minimum_overlap=200, check_overhang=False)
+    rows, roff = gpu.chain_lists(h.cur, h.ext, h.offsets, h.cur_id, h.ext_id, h.cur_len, h.ext_len, cfg)
+    erows, eroff = oracle_mod.chain_lists(h.cur, h.ext, h.offsets, h.cur_id, h.ext_id, h.cur_len, h.ext_len,
+                                          cfg, threads=8)
+    assert np.array_equal(roff, eroff)
+    assert_rows_equal(rows, erows, "config3 lists")
+    assert len(rows) > 100
+    cfg2 = OverlapConfig(kmer_size=15, minimum_overlap=200, check_overhang=False, only_max_ext=False)
+    rows, roff = gpu.chain_lists(h.cur, h.ext, h.offsets, h.cur_id, h.ext_id, h.cur_len, h.ext_len, cfg2)
+    erows, eroff = oracle_mod.chain_lists(h.cur, h.ext, h.offsets, h.cur_id, h.ext_id, h.cur_len, h.ext_len,
+                                          cfg2, threads=8)
+    assert_rows_equal(rows, erows, "config3 lists, containment selection")
+
+
+def _random_alignments(rng, n_reads):
+    rows = []
+    offs = [0]
+    for r in range(n_reads):
+        m = int(rng.integers(0, 40))
+        pos = 0
+        for _ in range(m):
+            pos += int(rng.integers(-200, 3000))
+            pos = max(pos, 0)
+            ln = int(rng.integers(100, 5000))
+            elen = int(rng.integers(500, 20000))
+            eb = int(rng.integers(0, 2000))
+            ee = min(elen - 1, eb + ln + int(rng.integers(-50, 50)))
+            left = int(rng.integers(0, 12))
+            right = int(rng.integers(0, 12))
+            rows.append((r + 1, int(rng.integers(1, 30)), int(rng.integers(1, 30)), pos, pos + ln, 100000, left,
+                         right, eb, max(eb, ee), elen, int(rng.integers(1, 5)) * 100, 0.01))
+        offs.append(len(rows))
+    a = np.zeros(len(rows), N.EDGE_ALIGNMENT_DTYPE)
+    for i, t in enumerate(rows):
+        a[i] = t
+    return a, np.array(offs, np.int64)
+
+
+def test_read_align_chains(gpu, oracle_mod):
+    rng = np.random.default_rng(5)
+    a, off = _random_alignments(rng, 500)
+    params = N.ChainParams(1500, 50, 1000, 500)
+    gi, gk, gs, go = gpu.read_align_chains(a, off, 900, 500, params)
+    ei, ek, es, eo = oracle_mod.read_align_chains(a, off, 900, 500, params)
+    assert np.array_equal(gk, ek)
+    assert np.array_equal(gi, ei)
+    assert np.array_equal(go, eo)
+    assert_rows_equal(gs, es, "read-align segments")
+    assert len(gs) > 0
