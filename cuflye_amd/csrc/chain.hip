@@ -5,20 +5,21 @@
 // tools/replay_overlap_chains.py:228-380 and
 // tools/replay_read_to_graph_source_pack.py:352-491).
 //
-// Work distribution: a persistent kernel; each 64-lane block pulls the next
-// group from a device work queue (an atomic counter over groups sorted by
-// decreasing size, so the heavy tail starts first).  Inside a group:
-//   * prefilters are wave reductions;
-//   * the two sorts are the exact std::sort emulation of exsort_dev.hpp;
-//   * DP row i runs its lookback 64 predecessors per step across the lanes:
-//     each lane scores one j, a prefix max over lanes reproduces the serial
-//     "strictly greater" running maximum, and a ballot finds the first lane
-//     where the serial loop would have broken (an improvement with
-//     jumpDiv == 0 && dc < k, or the sorted-axis delta passing maxJump);
-//     scores of the last kRing rows live in LDS;
-//   * the backtrack and filters run on lane 0 (O(n) per group).
+// Three persistent kernels pull groups from device work queues over a
+// size-descending order (big groups first, so the heavy tail starts early):
+//   prep    one wave per group: prefilters (wave reductions), the exact
+//           std::sort by extPos when extLen > curLen, group-order hit copies;
+//   dp      G lanes per group (G in {1, 4, 8, 16, 32, 64}, the lookback tile
+//           width the autotuner picks): row i scores G predecessors per step,
+//           a prefix max over the tile reproduces the serial strictly-greater
+//           running maximum, a ballot finds the first predecessor where the
+//           serial loop breaks (an improvement with jumpDiv == 0 && dc < k, or
+//           the sorted-axis delta passing maxJump); recent scores stay in an
+//           LDS ring per group;
+//   finish  one wave per group: exact std::sort of the rows by score, the
+//           backtrack (lane 0, O(n)), overlapTest, divergence, selection.
 #include "chain.hpp"
-#include "exsort_dev.hpp"
+#include "exsort_wave.hpp"
 
 namespace fh
 {
@@ -26,17 +27,9 @@ namespace fh
 namespace
 {
 
-constexpr int kNT = 64;
-constexpr int kRing = 2048;
-constexpr int kLaneSortMax = 48;
-
-struct ChainShared
-{
-	int ring[kRing];
-	SortShared<kNT> sort;
-	int red[8];
-	int group;
-};
+constexpr int kTile = 1024;
+constexpr int kLaneMax = 48;
+constexpr int kRingTotal = 4096;  // LDS score-ring entries per wave, split over groups
 
 __device__ inline float flyeLogf(float x, const uint32_t* tabX, const uint32_t* tabY, int tabN)
 {
@@ -61,7 +54,7 @@ __device__ bool overlapTestDev(const Proposal& o, int64_t curId, int64_t extId, 
 	const int32_t er = o.extEnd - o.extBegin;
 	if (cr < p.minimum_overlap || er < p.minimum_overlap) return false;
 	const int32_t lengthDiff = iabs(cr - er);
-	// 0.5 * min(cr, er) compared in double exactly as the int operands allow
+	// lengthDiff > 0.5 * min(cr, er) and intersect > curRange / 2, on integers
 	if (2LL * lengthDiff > static_cast<long long>(min(cr, er))) return false;
 	if (curId == extId)
 	{
@@ -104,18 +97,23 @@ __device__ inline int upperBound(const int32_t* a, int n, int32_t v)
 	return lo;
 }
 
-__global__ void __launch_bounds__(kNT) chainGroupsKernel(ChainArgs a)
+__device__ inline int pullGroup(int32_t* queue)
 {
-	__shared__ ChainShared sh;
-	const int lane = static_cast<int>(threadIdx.x);
+	int g = 0;
+	if (laneId() == 0) g = atomicAdd(queue, 1);
+	return __shfl(g, 0, FH_WAVE);
+}
+
+// ---- prep ------------------------------------------------------------------
+
+__global__ void __launch_bounds__(FH_WAVE) chainPrepKernel(ChainArgs a)
+{
+	__shared__ WaveSortLds<kTile> lds;
+	const int lane = laneId();
 	const flyehip_overlap_params_t& p = a.params;
-	const int32_t k = p.kmer_size;
 	for (;;)
 	{
-		if (lane == 0) sh.group = atomicAdd(a.queue, 1);
-		__syncthreads();
-		const int qi = sh.group;
-		__syncthreads();
+		const int qi = pullGroup(a.queue);
 		if (qi >= a.nGroups) break;
 		const int g = a.order ? a.order[qi] : qi;
 		const GroupDesc gd = a.groups[g];
@@ -124,12 +122,12 @@ __global__ void __launch_bounds__(kNT) chainGroupsKernel(ChainArgs a)
 		const int64_t base = gd.hitStart;
 		const int32_t* cur0 = a.cur + base;
 		const int32_t* ext0 = a.ext + base;
-		int32_t outCount = 0;
 
-		// ---- prefilters (replay_overlap_chains.py:245-270) ----
+		// prefilters (replay_overlap_chains.py:245-270); unique curPos counted
+		// from prevPos = 0 on the (extId, curPos) order
 		int uniq = 0;
 		int mnE = 0x7fffffff, mxE = -0x7fffffff - 1;
-		for (int i = lane; i < n; i += kNT)
+		for (int i = lane; i < n; i += FH_WAVE)
 		{
 			const int32_t c = cur0[i];
 			const int32_t prev = i == 0 ? 0 : cur0[i - 1];
@@ -150,35 +148,29 @@ __global__ void __launch_bounds__(kNT) chainGroupsKernel(ChainArgs a)
 			if (min(mnC, mnE) > p.maximum_overhang) pass = false;
 			if (min(qd.curLen - mxC, gd.extLen - mxE) > p.maximum_overhang) pass = false;
 		}
-		if (!pass || n < 2)
+		if (n < 2) pass = false;
+		if (lane == 0)
 		{
-			if (lane == 0) a.groupCount[g] = 0;
-			continue;
+			a.groupPass[g] = pass ? 1 : 0;
+			a.groupCount[g] = 0;
 		}
+		if (!pass) continue;
 
-		uint64_t* sk = a.sortKey + base;
-		uint32_t* sv = a.sortVal + base;
-		int* posL = a.posL + base;
-		int* posR = a.posR + base;
-		Seg* small = a.small + base;
 		int32_t* hc = a.hcur + base;
 		int32_t* he = a.hext + base;
-		int32_t* sc = a.score + base;
-		int32_t* bt = a.bt + base;
-		const exsort::Span span{sk, sv};
-
-		// ---- extPos re-sort when extLen > curLen (std::sort by extPos) ----
 		const bool extSorted = gd.extLen > qd.curLen;
 		if (extSorted)
 		{
-			for (int i = lane; i < n; i += kNT)
+			uint64_t* sk = a.sortKey + base;
+			uint32_t* sv = a.sortVal + base;
+			for (int i = lane; i < n; i += FH_WAVE)
 			{
 				sk[i] = static_cast<uint64_t>(static_cast<uint32_t>(ext0[i]));
 				sv[i] = static_cast<uint32_t>(i);
 			}
-			__syncthreads();
-			blockStdSort<kNT>(span, n, posL, posR, small, n, kLaneSortMax, sh.sort);
-			for (int i = lane; i < n; i += kNT)
+			waveSync();
+			waveStdSort<kTile, kLaneMax>(exsort::Span{sk, sv}, n, a.posL + base, a.posR + base, lds);
+			for (int i = lane; i < n; i += FH_WAVE)
 			{
 				const uint32_t src = sv[i];
 				hc[i] = cur0[src];
@@ -187,32 +179,88 @@ __global__ void __launch_bounds__(kNT) chainGroupsKernel(ChainArgs a)
 		}
 		else
 		{
-			for (int i = lane; i < n; i += kNT)
+			for (int i = lane; i < n; i += FH_WAVE)
 			{
 				hc[i] = cur0[i];
 				he[i] = ext0[i];
 			}
 		}
-		__syncthreads();
+		waveSync();
+	}
+}
 
-		// ---- chaining DP (replay_overlap_chains.py:280-313) ----
-		if (lane == 0)
+// ---- dp ----------------------------------------------------------------------
+
+template <int G>
+__global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
+{
+	constexpr int kSubs = FH_WAVE / G;
+	constexpr int kRing = kRingTotal / kSubs;
+	__shared__ int ring[kSubs][kRing];
+	const int lane = laneId();
+	const int sub = lane / G;
+	const int sl = lane % G;
+	const flyehip_overlap_params_t& p = a.params;
+	const int32_t k = p.kmer_size;
+	const int32_t maxJump = p.maximum_jump;
+	const unsigned long long subMask = (G == 64 ? ~0ULL : ((1ULL << G) - 1)) << (sub * G);
+	int* rg = ring[sub];
+
+	// per-subgroup state (uniform inside a subgroup)
+	int g = -1;
+	int n = 0;
+	int i = 0;
+	bool extSorted = false;
+	const int32_t* hc = nullptr;
+	const int32_t* he = nullptr;
+	int32_t* sc = nullptr;
+	int32_t* bt = nullptr;
+	bool done = false;
+	int iter = 0;
+	for (;;)
+	{
+		// subgroups without a group pull the next passing one
+		bool need = !done && (g < 0 || i >= n);
+		if (g >= 0 && i >= n) g = -1;
+		while (need)
 		{
-			sc[0] = 0;
-			bt[0] = -1;
-			sh.ring[0] = 0;
+			int qi = 0;
+			if (sl == 0) qi = atomicAdd(a.queue, 1);
+			qi = __shfl(qi, sub * G, FH_WAVE);
+			if (qi >= a.nGroups)
+			{
+				done = true;
+				break;
+			}
+			const int gg = a.order ? a.order[qi] : qi;
+			if (!a.groupPass[gg]) continue;
+			const GroupDesc gd = a.groups[gg];
+			g = gg;
+			n = gd.n;
+			i = 1;
+			extSorted = gd.extLen > a.queries[gd.query].curLen;
+			hc = a.hcur + gd.hitStart;
+			he = a.hext + gd.hitStart;
+			sc = a.score + gd.hitStart;
+			bt = a.bt + gd.hitStart;
+			if (sl == 0)
+			{
+				sc[0] = 0;
+				bt[0] = -1;
+				rg[0] = 0;
+			}
+			need = false;
 		}
-		__syncthreads();
-		const int32_t maxJump = p.maximum_jump;
-		for (int i = 1; i < n; ++i)
+		if (__ballot(!done) == 0) break;
+		if (!done)
 		{
 			const int32_t curN = hc[i];
 			const int32_t extN = he[i];
 			int carry = 0;
 			int carryId = 0;
-			for (int tileBase = i - 1; tileBase >= 0; tileBase -= kNT)
+			for (int tileBase = i - 1; tileBase >= 0; tileBase -= G)
 			{
-				const int j = tileBase - lane;
+				const int j = tileBase - sl;
 				const bool active = j >= 0;
 				int cand = -0x7fffffff - 1;
 				bool special = false;
@@ -227,26 +275,32 @@ __global__ void __launch_bounds__(kNT) chainGroupsKernel(ChainArgs a)
 						const int32_t ms = min(min(dc, de), k);
 						const float pen = jd > p.gap_jump_threshold ? p.large_gap_penalty : p.small_gap_penalty;
 						const int32_t gc = static_cast<int32_t>(pen * static_cast<float>(jd));
-						const int32_t sj = (i - j <= kRing - 1) ? sh.ring[j & (kRing - 1)] : sc[j];
+						const int32_t sj = (i - j < kRing) ? rg[j & (kRing - 1)] : sc[j];
 						cand = sj + ms - gc;
 						special = (jd == 0 && dc < k);
 					}
 					stop = extSorted ? (de > maxJump) : (dc > maxJump);
 				}
 				// running maximum before this lane, in scan order
-				const int incl = waveInclusiveMax(cand);
-				int before = __shfl_up(incl, 1, FH_WAVE);
-				if (lane == 0) before = -0x7fffffff - 1;
+				int incl = cand;
+				for (int o = 1; o < G; o <<= 1)
+				{
+					const int t = __shfl_up(incl, o, G);
+					if (sl >= o) incl = max(incl, t);
+				}
+				int before = __shfl_up(incl, 1, G);
+				if (sl == 0) before = -0x7fffffff - 1;
 				before = max(before, carry);
 				const bool improved = active && cand > before;
 				const bool brk = active && ((improved && special) || stop);
-				const unsigned long long bm = __ballot(brk);
-				const int lastLane = bm ? (__ffsll(static_cast<long long>(bm)) - 1) : (kNT - 1);
-				const int masked = (active && lane <= lastLane) ? cand : (-0x7fffffff - 1);
-				const int mx = waveMax(masked);
+				const unsigned long long bm = (__ballot(brk) & subMask) >> (sub * G);
+				const int lastLane = bm ? (__ffsll(static_cast<long long>(bm)) - 1) : (G - 1);
+				const int masked = (active && sl <= lastLane) ? cand : (-0x7fffffff - 1);
+				int mx = masked;
+				for (int o = G / 2; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, G));
 				if (mx > carry)
 				{
-					const unsigned long long hit = __ballot(masked == mx);
+					const unsigned long long hit = (__ballot(masked == mx) & subMask) >> (sub * G);
 					const int l0 = __ffsll(static_cast<long long>(hit)) - 1;
 					carry = mx;
 					carryId = tileBase - l0;
@@ -254,32 +308,61 @@ __global__ void __launch_bounds__(kNT) chainGroupsKernel(ChainArgs a)
 				if (bm) break;
 			}
 			const int32_t si = max(carry, k);
-			if (lane == 0)
+			if (sl == 0)
 			{
 				sc[i] = si;
 				bt[i] = carry > k ? carryId : -1;
-				sh.ring[i & (kRing - 1)] = si;
-				if ((i & 255) == 0) __threadfence_block();
+				rg[i & (kRing - 1)] = si;
 			}
-			// one wave: LDS traffic of a wave is processed in order, so a
-			// compiler barrier is enough for lane 0's ring write to reach the
-			// next row's readers
-			__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-			__builtin_amdgcn_wave_barrier();
+			++i;
 		}
-		__threadfence_block();
-		__syncthreads();
+		// ring writes of this row are read by the next row's lanes; stores to
+		// sc[] become visible to far lookbacks after the periodic fence
+		++iter;
+		if ((iter & 31) == 0) __threadfence_block();
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+		__builtin_amdgcn_wave_barrier();
+	}
+}
 
-		// ---- score order: std::sort(indices, score[a] > score[b]) ----
-		for (int i = lane; i < n; i += kNT)
+// ---- finish ----------------------------------------------------------------------
+
+__global__ void __launch_bounds__(FH_WAVE) chainFinishKernel(ChainArgs a)
+{
+	__shared__ WaveSortLds<kTile> lds;
+	__shared__ int red[2];
+	const int lane = laneId();
+	const flyehip_overlap_params_t& p = a.params;
+	const int32_t k = p.kmer_size;
+	for (;;)
+	{
+		const int qi = pullGroup(a.queue);
+		if (qi >= a.nGroups) break;
+		const int g = a.order ? a.order[qi] : qi;
+		if (!a.groupPass[g]) continue;
+		const GroupDesc gd = a.groups[g];
+		const QueryDesc qd = a.queries[gd.query];
+		const int n = gd.n;
+		const int64_t base = gd.hitStart;
+		const int32_t* hc = a.hcur + base;
+		const int32_t* he = a.hext + base;
+		const int32_t* sc = a.score + base;
+		int32_t* bt = a.bt + base;
+		uint64_t* sk = a.sortKey + base;
+		uint32_t* sv = a.sortVal + base;
+		int32_t* posL = a.posL + base;
+		int32_t* posR = a.posR + base;
+
+		// score order: std::sort(indices, score[a] > score[b])
+		for (int i = lane; i < n; i += FH_WAVE)
 		{
 			sk[i] = static_cast<uint64_t>(static_cast<uint32_t>(0x7fffffff - sc[i]));
 			sv[i] = static_cast<uint32_t>(i);
 		}
-		__syncthreads();
-		blockStdSort<kNT>(span, n, posL, posR, small, n, kLaneSortMax, sh.sort);
+		waveSync();
+		waveStdSort<kTile, kLaneMax>(exsort::Span{sk, sv}, n, posL, posR, lds);
 
-		// ---- backtrack, overlapTest, divergence (lane 0) ----
+		// backtrack, overlapTest, divergence (lane 0)
 		Proposal* props = a.props + base;
 		Proposal* outp = a.outProps + base;
 		if (lane == 0)
@@ -332,29 +415,24 @@ __global__ void __launch_bounds__(kNT) chainGroupsKernel(ChainArgs a)
 					props[np++] = o;
 				}
 			}
-			if (p.only_max_ext)
-			{
-				outCount = (best >= 0 && outp[0].div < p.max_divergence) ? 1 : 0;
-			}
-			sh.red[0] = np;
-			sh.red[1] = outCount;
+			red[0] = np;
+			red[1] = (p.only_max_ext && best >= 0 && outp[0].div < p.max_divergence) ? 1 : 0;
 		}
-		__syncthreads();
+		waveSync();
 		if (!p.only_max_ext)
 		{
 			// std::sort(extOverlaps, score desc), containment filter, divergence filter
-			const int np = sh.red[0];
-			for (int i = lane; i < np; i += kNT)
+			const int np = red[0];
+			for (int i = lane; i < np; i += FH_WAVE)
 			{
 				sk[i] = static_cast<uint64_t>(static_cast<uint32_t>(0x7fffffff - props[i].score));
 				sv[i] = static_cast<uint32_t>(i);
 			}
-			__syncthreads();
-			blockStdSort<kNT>(span, np, posL, posR, small, np, kLaneSortMax, sh.sort);
+			waveSync();
+			waveStdSort<kTile, kLaneMax>(exsort::Span{sk, sv}, np, posL, posR, lds);
 			if (lane == 0)
 			{
-				// primary proposal indices, in sorted order
-				int32_t* prim = posL;
+				int32_t* prim = posL;  // free again after the sort
 				int nprim = 0;
 				for (int q = 0; q < np; ++q)
 				{
@@ -374,15 +452,13 @@ __global__ void __launch_bounds__(kNT) chainGroupsKernel(ChainArgs a)
 				}
 				int w = 0;
 				for (int r = 0; r < nprim; ++r)
-				{
 					if (props[prim[r]].div < p.max_divergence) outp[w++] = props[prim[r]];
-				}
-				sh.red[1] = w;
+				red[1] = w;
 			}
-			__syncthreads();
+			waveSync();
 		}
-		if (lane == 0) a.groupCount[g] = sh.red[1];
-		__syncthreads();
+		if (lane == 0) a.groupCount[g] = red[1];
+		waveSync();
 	}
 }
 
@@ -418,11 +494,75 @@ __global__ void writeRowsKernel(ChainArgs a, const int64_t* rowStart, flyehip_ov
 	}
 }
 
+// Size-descending processing order: groups binned by floor(log2 n) (the order
+// inside a bin is irrelevant to the results).
+__global__ void binCountKernel(const GroupDesc* groups, int32_t nGroups, int32_t* binCount)
+{
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nGroups) return;
+	const int n = max(1, groups[g].n);
+	const int bin = __clz(n);  // 31 - floor(log2 n): big groups first
+	atomicAdd(&binCount[bin], 1);
+}
+
+__global__ void binScatterKernel(const GroupDesc* groups, int32_t nGroups, int32_t* binCursor, int32_t* order)
+{
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nGroups) return;
+	const int n = max(1, groups[g].n);
+	const int bin = __clz(n);
+	order[atomicAdd(&binCursor[bin], 1)] = g;
+}
+
+__global__ void binPrefixKernel(int32_t* binCount, int32_t* binCursor)
+{
+	if (threadIdx.x != 0) return;
+	int s = 0;
+	for (int b = 0; b < 33; ++b)
+	{
+		binCursor[b] = s;
+		s += binCount[b];
+	}
+}
+
 } // namespace
 
-hipError_t launchChainGroups(const ChainArgs& a, int blocks, hipStream_t stream)
+hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* bins, int32_t* order,
+							hipStream_t stream)
 {
-	hipLaunchKernelGGL(chainGroupsKernel, dim3(blocks), dim3(kNT), 0, stream, a);
+	if (nGroups == 0) return hipSuccess;
+	const int tpb = 256;
+	const dim3 grid((nGroups + tpb - 1) / tpb);
+	hipLaunchKernelGGL(binCountKernel, grid, dim3(tpb), 0, stream, groups, nGroups, bins);
+	hipLaunchKernelGGL(binPrefixKernel, dim3(1), dim3(64), 0, stream, bins, bins + 40);
+	hipLaunchKernelGGL(binScatterKernel, grid, dim3(tpb), 0, stream, groups, nGroups, bins + 40, order);
+	return hipGetLastError();
+}
+
+hipError_t launchChainPrep(const ChainArgs& a, int blocks, hipStream_t stream)
+{
+	hipLaunchKernelGGL(chainPrepKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a);
+	return hipGetLastError();
+}
+
+hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t stream)
+{
+	switch (lanes)
+	{
+	case 1: hipLaunchKernelGGL(chainDpKernel<1>, dim3(blocks), dim3(FH_WAVE), 0, stream, a); break;
+	case 4: hipLaunchKernelGGL(chainDpKernel<4>, dim3(blocks), dim3(FH_WAVE), 0, stream, a); break;
+	case 8: hipLaunchKernelGGL(chainDpKernel<8>, dim3(blocks), dim3(FH_WAVE), 0, stream, a); break;
+	case 16: hipLaunchKernelGGL(chainDpKernel<16>, dim3(blocks), dim3(FH_WAVE), 0, stream, a); break;
+	case 32: hipLaunchKernelGGL(chainDpKernel<32>, dim3(blocks), dim3(FH_WAVE), 0, stream, a); break;
+	case 64: hipLaunchKernelGGL(chainDpKernel<64>, dim3(blocks), dim3(FH_WAVE), 0, stream, a); break;
+	default: return hipErrorInvalidValue;
+	}
+	return hipGetLastError();
+}
+
+hipError_t launchChainFinish(const ChainArgs& a, int blocks, hipStream_t stream)
+{
+	hipLaunchKernelGGL(chainFinishKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a);
 	return hipGetLastError();
 }
 

@@ -5,7 +5,6 @@
 #include <stdint.h>
 
 #include "../../include/flyehip.h"
-#include "exsort_dev.hpp"
 
 namespace fh
 {
@@ -59,25 +58,29 @@ struct ChainArgs
 	uint32_t* sortVal;
 	int32_t* posL;
 	int32_t* posR;
-	Seg* small;
-	int32_t* hcur;
+	int32_t* hcur;             // group-order copies (after the extPos re-sort)
 	int32_t* hext;
 	int32_t* score;
 	int32_t* bt;
 	Proposal* props;
 	Proposal* outProps;
-	// outputs
+	// per group
+	int32_t* groupPass;
 	int32_t* groupCount;
-	int32_t* queue;
-	// glibc logf exception table (sorted by argument bits)
+	int32_t* queue;            // work-queue counter of the kernel being launched
+	// glibc logf agreement table (sorted by argument bits)
 	const uint32_t* logX;
 	const uint32_t* logY;
 };
 
-// Per-hit scratch bytes the chaining kernel needs.
-constexpr size_t kChainScratchPerHit = 8 + 4 + 4 + 4 + sizeof(Seg) + 4 + 4 + 4 + 4 + 2 * sizeof(Proposal);
+// Per-hit scratch bytes the chaining kernels need.
+constexpr size_t kChainScratchPerHit = 8 + 4 + 4 + 4 + 4 + 4 + 4 + 4 + 2 * sizeof(Proposal);
 
-hipError_t launchChainGroups(const ChainArgs& a, int blocks, hipStream_t stream);
+hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* bins, int32_t* order,
+							hipStream_t stream);
+hipError_t launchChainPrep(const ChainArgs& a, int blocks, hipStream_t stream);
+hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t stream);
+hipError_t launchChainFinish(const ChainArgs& a, int blocks, hipStream_t stream);
 hipError_t launchWriteRows(const ChainArgs& a, const int64_t* rowStart, flyehip_overlap_t* rows,
 						   hipStream_t stream);
 

@@ -196,10 +196,12 @@ EXS_HD void insertionSort(const Span& s, int first, int last)
 }
 
 // The introsort loop on [first, last) with a given remaining depth budget,
-// without the final insertion pass.  Uses an explicit stack (the recursion
-// goes to the right part, the loop continues on the left part; the order in
-// which disjoint ranges are processed does not change the result).
-template <int kStack = 64>
+// without the final insertion pass.  libstdc++ recurses on the right part and
+// loops on the left; both parts get the decremented budget and are disjoint,
+// so the order they are processed in does not change the result.  Here the
+// smaller part is processed first and the larger one waits on an explicit
+// stack, which bounds the stack by log2(n) + 1 entries.
+template <int kStack = 40>
 EXS_HD void introLoop(const Span& s, int first, int last, int depth)
 {
 	int stF[kStack];
@@ -224,13 +226,23 @@ EXS_HD void introLoop(const Span& s, int first, int last, int depth)
 				break;
 			}
 			--d;
-			int cut = partitionPivot(s, f, l);
-			// right part [cut, l) with depth d is independent of the left part
-			stF[top] = cut;
-			stL[top] = l;
-			stD[top] = d;
-			++top;
-			l = cut;
+			const int cut = partitionPivot(s, f, l);
+			if (cut - f < l - cut)
+			{
+				stF[top] = cut;
+				stL[top] = l;
+				stD[top] = d;
+				++top;
+				l = cut;
+			}
+			else
+			{
+				stF[top] = f;
+				stL[top] = cut;
+				stD[top] = d;
+				++top;
+				f = cut;
+			}
 		}
 	}
 }
@@ -252,12 +264,8 @@ inline int partitionByRanks(const Span& s, int first, int last, int* posL, int* 
 	// caller has already moved the median to `first`
 	const uint64_t p = s.k[first];
 	const int lo = first + 1;
-	int geTotal = 0, leTotal = 0;
-	for (int i = lo; i < last; ++i)
-	{
-		geTotal += !(s.k[i] < p);
-		leTotal += !(p < s.k[i]);
-	}
+	int leTotal = 0;
+	for (int i = lo; i < last; ++i) leTotal += !(p < s.k[i]);
 	int geLeft = 0, leLeft = 0, m = 0, lm = -1;
 	for (int i = lo; i < last; ++i)
 	{

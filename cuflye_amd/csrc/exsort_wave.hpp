@@ -1,0 +1,213 @@
+// exsort_wave.hpp - one wave64 sorts one array exactly like libstdc++'s
+// std::sort (see exsort.hpp for the algorithm and its rank formulation).
+//
+//  * ranges larger than the LDS tile are partitioned in place in global
+//    memory by the whole wave: rows of 64 consecutive elements, ranks from
+//    ballot + popcount, swap partners scattered to posL/posR, swaps in
+//    parallel;
+//  * a range of at most kTile elements is copied into the wave's LDS tile,
+//    partitioned there the same way down to kLaneMax elements, and the
+//    remaining ranges are finished one per lane (introsort loop with the
+//    remaining depth budget + insertion pass), 64 ranges at a time;
+//  * the heapsort fallback (depth budget exhausted on a large range) runs on
+//    lane 0 in global memory; it needs adversarial inputs and is rare.
+//
+// No block barriers: every wave is independent, so one kernel can give each
+// wave a different array (a query's k-mer matches, a target group).  Memory
+// written by one lane and read by another is ordered with a workgroup-scope
+// fence (all lanes of a wave share the CU's L1) plus a wave barrier.
+#pragma once
+
+#include "dev_common.hpp"
+#include "exsort.hpp"
+
+namespace fh
+{
+
+struct WSeg
+{
+	int f;
+	int l;
+	int d;
+};
+
+template <int kTile>
+struct WaveSortLds
+{
+	uint64_t k[kTile];
+	uint32_t v[kTile];
+	int posL[kTile / 2 + 1];
+	int posR[kTile / 2 + 1];
+	WSeg stack[40];
+	WSeg lanes[FH_WAVE];
+};
+
+__device__ inline void waveSync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+	__builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline unsigned long long laneMaskLt()
+{
+	const int l = laneId();
+	return l == 0 ? 0ULL : (~0ULL >> (64 - l));
+}
+
+// Partition s[f, l) around the median moved to f; all 64 lanes call it.
+// posL / posR need (l - f) / 2 + 1 entries.
+__device__ inline int wavePartition(const exsort::Span& s, int f, int l, int* posL, int* posR)
+{
+	const int lane = laneId();
+	if (lane == 0) exsort::medianToFirst(s, f, f + 1, f + (l - f) / 2, l - 1);
+	waveSync();
+	const uint64_t p = s.k[f];
+	const int lo = f + 1;
+	int LE = 0;
+	for (int base = lo; base < l; base += FH_WAVE)
+	{
+		const int i = base + lane;
+		const bool le = i < l && !(p < s.k[i]);
+		LE += __popcll(__ballot(le));
+	}
+	const unsigned long long lt = laneMaskLt();
+	int geLeft = 0, leLeft = 0, m = 0;
+	int lm = 0x7fffffff;
+	for (int base = lo; base < l; base += FH_WAVE)
+	{
+		const int i = base + lane;
+		bool ge = false, le = false;
+		if (i < l)
+		{
+			const uint64_t v = s.k[i];
+			ge = !(v < p);
+			le = !(p < v);
+		}
+		const unsigned long long bg = __ballot(ge);
+		const unsigned long long bl = __ballot(le);
+		const int myGe = geLeft + __popcll(bg & lt);
+		const int myLe = leLeft + __popcll(bl & lt);
+		const int leRight = LE - myLe - (le ? 1 : 0);
+		const bool swapGe = ge && myGe < leRight;
+		if (swapGe) posL[myGe] = i;
+		if (le && leRight < myGe) posR[leRight] = i;
+		const unsigned long long bns = __ballot(ge && !swapGe);
+		if (lm == 0x7fffffff && bns) lm = base + __ffsll(static_cast<long long>(bns)) - 1;
+		m += __popcll(__ballot(swapGe));
+		geLeft += __popcll(bg);
+		leLeft += __popcll(bl);
+	}
+	waveSync();
+	for (int q = lane; q < m; q += FH_WAVE) s.swap(posL[q], posR[q]);
+	waveSync();
+	int cut = lm;
+	if (m > 0)
+	{
+		const int r = posR[m - 1];
+		if (r < cut) cut = r;
+	}
+	waveSync();
+	return cut;
+}
+
+// Sorts the range g[f, l) (l - f <= kTile) with remaining depth budget d,
+// through the wave's LDS tile; ends with the stable insertion pass.
+template <int kTile, int kLaneMax>
+__device__ void waveTileSort(const exsort::Span& g, int f, int l, int d, WaveSortLds<kTile>& t)
+{
+	const int lane = laneId();
+	const int n = l - f;
+	for (int i = lane; i < n; i += FH_WAVE)
+	{
+		t.k[i] = g.k[f + i];
+		t.v[i] = g.v[f + i];
+	}
+	waveSync();
+	const exsort::Span s{t.k, t.v};
+	int top = 0;
+	int nl = 0;
+	if (lane == 0) t.stack[0] = WSeg{0, n, d};
+	top = 1;
+	waveSync();
+	auto flush = [&]() {
+		waveSync();
+		if (lane < nl)
+		{
+			const WSeg w = t.lanes[lane];
+			exsort::introLoop<16>(s, w.f, w.l, w.d);
+			exsort::insertionSort(s, w.f, w.l);
+		}
+		nl = 0;
+		waveSync();
+	};
+	while (top > 0)
+	{
+		--top;
+		const WSeg w = t.stack[top];
+		waveSync();
+		if (w.l - w.f <= kLaneMax || w.d == 0)
+		{
+			if (lane == 0) t.lanes[nl] = w;
+			++nl;
+			if (nl == FH_WAVE) flush();
+			continue;
+		}
+		const int cut = wavePartition(s, w.f, w.l, t.posL, t.posR);
+		if (lane == 0)
+		{
+			// larger part below, smaller on top: the stack stays within log2(n) + 1
+			const WSeg a{w.f, cut, w.d - 1}, b{cut, w.l, w.d - 1};
+			const bool leftSmaller = cut - w.f < w.l - cut;
+			t.stack[top] = leftSmaller ? b : a;
+			t.stack[top + 1] = leftSmaller ? a : b;
+		}
+		top += 2;
+		waveSync();
+	}
+	if (nl) flush();
+	for (int i = lane; i < n; i += FH_WAVE)
+	{
+		g.k[f + i] = t.k[i];
+		g.v[f + i] = t.v[i];
+	}
+	waveSync();
+}
+
+// Exact std::sort of g[0, n) by one wave.  posL/posR: global scratch of
+// n / 2 + 1 ints each (only touched when n > kTile).
+template <int kTile, int kLaneMax>
+__device__ void waveStdSort(const exsort::Span& g, int n, int* posL, int* posR, WaveSortLds<kTile>& t)
+{
+	if (n < 2) return;
+	if (n <= kTile)
+	{
+		waveTileSort<kTile, kLaneMax>(g, 0, n, exsort::depthLimit(static_cast<uint32_t>(n)), t);
+		return;
+	}
+	// global ranges: stack kept in registers of every lane (uniform values)
+	WSeg st[40];
+	int top = 0;
+	st[top++] = WSeg{0, n, exsort::depthLimit(static_cast<uint32_t>(n))};
+	while (top > 0)
+	{
+		const WSeg w = st[--top];
+		if (w.l - w.f <= kTile)
+		{
+			waveTileSort<kTile, kLaneMax>(g, w.f, w.l, w.d, t);
+			continue;
+		}
+		if (w.d == 0)
+		{
+			if (laneId() == 0) exsort::heapSort(g, w.f, w.l);
+			waveSync();
+			continue;
+		}
+		const int cut = wavePartition(g, w.f, w.l, posL, posR);
+		const WSeg a{w.f, cut, w.d - 1}, b{cut, w.l, w.d - 1};
+		const bool leftSmaller = cut - w.f < w.l - cut;
+		st[top++] = leftSmaller ? b : a;
+		st[top++] = leftSmaller ? a : b;
+	}
+}
+
+} // namespace fh

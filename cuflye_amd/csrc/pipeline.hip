@@ -12,6 +12,7 @@
 //   join windows   count i32 + filtered i32 + slot/flip u32 per query window
 //   matches        key u64 (internal extId << 32 | curPos) + extPos i32
 #include "pipeline.hpp"
+#include "exsort_wave.hpp"
 
 namespace fh
 {
@@ -412,26 +413,32 @@ __global__ void writeCandidatesKernel(JoinArgs a, const int64_t* qMatchOff, cons
 
 // ---- per-query match sort -------------------------------------------------------
 
-constexpr int kSortNT = 256;
+constexpr int kMatchTile = 1024;
 
-__global__ void __launch_bounds__(kSortNT) matchSortKernel(MatchSortArgs a)
+// One wave per query, queries pulled from a work queue; the wave runs the
+// exact std::sort of the query's matches by (internal extId, curPos).
+__global__ void __launch_bounds__(FH_WAVE) matchSortKernel(MatchSortArgs a)
 {
-	__shared__ SortShared<kSortNT> sh;
-	__shared__ int ties;
-	const int q = blockIdx.x;
-	const int64_t b = a.qMatchOff[q];
-	const int n = static_cast<int>(a.qMatchOff[q + 1] - b);
-	uint64_t* key = a.key + b;
-	uint32_t* val = a.val + b;
-	for (int i = threadIdx.x; i < n; i += kSortNT) val[i] = static_cast<uint32_t>(i);
-	if (threadIdx.x == 0) ties = 0;
-	__syncthreads();
-	blockStdSort<kSortNT>(exsort::Span{key, val}, n, a.posL + b, a.posR + b, a.small + b, n, 48, sh);
-	int t = 0;
-	for (int i = threadIdx.x + 1; i < n; i += kSortNT) t |= (key[i] == key[i - 1]);
-	if (t) atomicOr(&ties, 1);
-	__syncthreads();
-	if (threadIdx.x == 0 && ties) atomicAdd(a.tieQueries, 1);
+	__shared__ WaveSortLds<kMatchTile> lds;
+	const int lane = laneId();
+	for (;;)
+	{
+		int q = 0;
+		if (lane == 0) q = atomicAdd(a.queue, 1);
+		q = __shfl(q, 0, FH_WAVE);
+		if (q >= a.nQueries) break;
+		const int64_t b = a.qMatchOff[q];
+		const int n = static_cast<int>(a.qMatchOff[q + 1] - b);
+		uint64_t* key = a.key + b;
+		uint32_t* val = a.val + b;
+		for (int i = lane; i < n; i += FH_WAVE) val[i] = static_cast<uint32_t>(i);
+		waveSync();
+		waveStdSort<kMatchTile, 48>(exsort::Span{key, val}, n, a.posL + b, a.posR + b, lds);
+		int t = 0;
+		for (int i = lane + 1; i < n; i += FH_WAVE) t |= (key[i] == key[i - 1]);
+		if (__ballot(t) && lane == 0) atomicAdd(a.tieQueries, 1);
+		waveSync();
+	}
 }
 
 __global__ void gatherHitsKernel(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const uint32_t* val,
@@ -589,7 +596,11 @@ hipError_t launchWriteCandidates(const JoinArgs& a, const int64_t* qMatchOff, co
 hipError_t launchMatchSort(const MatchSortArgs& a, hipStream_t s)
 {
 	if (a.nQueries == 0) return hipSuccess;
-	hipLaunchKernelGGL(matchSortKernel, dim3(static_cast<unsigned>(a.nQueries)), dim3(kSortNT), 0, s, a);
+	int dev = 0, cus = 0;
+	hipGetDevice(&dev);
+	hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	const int blocks = min(a.nQueries, cus * 8);
+	hipLaunchKernelGGL(matchSortKernel, dim3(static_cast<unsigned>(blocks)), dim3(FH_WAVE), 0, s, a);
 	return hipGetLastError();
 }
 

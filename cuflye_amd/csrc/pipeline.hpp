@@ -98,10 +98,10 @@ struct MatchSortArgs
 	int32_t nQueries;
 	uint64_t* key;             // sorted in place
 	uint32_t* val;             // permutation (emission index within query)
-	int32_t* posL;
+	int32_t* posL;             // partition scratch, one int per match
 	int32_t* posR;
-	Seg* small;
-	int32_t* tieQueries;       // counter
+	int32_t* queue;            // work-queue counter
+	int32_t* tieQueries;       // counter: queries with equal (extId, curPos) keys
 };
 hipError_t launchMatchSort(const MatchSortArgs& a, hipStream_t s);
 

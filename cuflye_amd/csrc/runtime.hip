@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "chain.hpp"
+#include "dev_common.hpp"
 #include "pipeline.hpp"
 #include "readalign.hpp"
 
@@ -116,9 +117,10 @@ struct flyehip_ctx
 
 	// batch scratch (grow-only)
 	fh::DevBuf bQueries, bWinOff, bWinCount, bWinFilt, bWinInfo, bMatchOff, bFiltOff, bScanTmp;
-	fh::DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bSmall, bHCur, bHExt, bGroupFlag, bGroupOff;
+	fh::DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bHCur, bHExt, bGroupFlag, bGroupOff;
 	fh::DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
-	fh::DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR, bCSmall;
+	fh::DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
+	fh::DevBuf bGroupPass, bOrder, bBins;
 	fh::DevBuf bQueue, bCounters, bQRowCount, bCurLen, bDivOff, bDivCount, bDivOut, bQueryIds, bCand;
 	fh::DevBuf bExtWords, bExtWordOff, bExtLen;
 	fh::DevBuf bAln, bAlnOff, bRA;
@@ -440,14 +442,48 @@ void runMatchSort(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches)
 	ms.val = ensureT<uint32_t>(c, c->bVal, static_cast<size_t>(nMatches) + 1);
 	ms.posL = ensureT<int32_t>(c, c->bPosL, static_cast<size_t>(nMatches) + 1);
 	ms.posR = ensureT<int32_t>(c, c->bPosR, static_cast<size_t>(nMatches) + 1);
-	ms.small = ensureT<Seg>(c, c->bSmall, static_cast<size_t>(nMatches) + 1);
 	ms.tieQueries = ensureT<int32_t>(c, c->bCounters, 16);
+	ms.queue = ms.tieQueries + 1;
 	zero32(c, ms.tieQueries, 16);
 	HIPCHECK(launchMatchSort(ms, c->stream));
 	int32_t ties = 0;
 	download(c, &ties, ms.tieQueries, sizeof(int32_t));
 	sync(c);
 	c->stats.tie_queries += ties;
+}
+
+// Order + prep + DP + finish over ca.nGroups groups; leaves groupCount set.
+void runChainKernels(flyehip_ctx* c, ChainArgs& ca)
+{
+	const int64_t ng = ca.nGroups;
+	zero32(c, ca.groupCount, ng + 1);
+	if (ng == 0) return;
+	int32_t* bins = ensureT<int32_t>(c, c->bBins, 96);
+	int32_t* order = ensureT<int32_t>(c, c->bOrder, static_cast<size_t>(ng) + 1);
+	int32_t* queues = ensureT<int32_t>(c, c->bQueue, 16);
+	zero32(c, bins, 96);
+	zero32(c, queues, 16);
+	HIPCHECK(launchGroupOrder(ca.groups, ca.nGroups, bins, order, c->stream));
+	ca.order = order;
+	const int blocks = chainBlocks(c);
+	HIPCHECK(hipEventRecord(c->ev[2], c->stream));
+	ca.queue = queues + 0;
+	HIPCHECK(launchChainPrep(ca, blocks, c->stream));
+	HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+	ca.queue = queues + 1;
+	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, c->stream));
+	HIPCHECK(hipEventRecord(c->ev[4], c->stream));
+	ca.queue = queues + 2;
+	HIPCHECK(launchChainFinish(ca, blocks, c->stream));
+	HIPCHECK(hipEventRecord(c->ev[5], c->stream));
+	HIPCHECK(hipEventSynchronize(c->ev[5]));
+	float a = 0, b = 0, d = 0;
+	HIPCHECK(hipEventElapsedTime(&a, c->ev[2], c->ev[3]));
+	HIPCHECK(hipEventElapsedTime(&b, c->ev[3], c->ev[4]));
+	HIPCHECK(hipEventElapsedTime(&d, c->ev[4], c->ev[5]));
+	c->stats.ms_group += a;
+	c->stats.ms_chain += b;
+	c->stats.ms_extract += d;
 }
 
 // Groups + chaining for a batch whose sorted matches sit in bMKey/bVal.
@@ -500,7 +536,6 @@ BatchOut runChain(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches, const st
 	ca.sortVal = ensureT<uint32_t>(c, c->bCVal, nm);
 	ca.posL = ensureT<int32_t>(c, c->bCPosL, nm);
 	ca.posR = ensureT<int32_t>(c, c->bCPosR, nm);
-	ca.small = ensureT<Seg>(c, c->bCSmall, nm);
 	ca.hcur = ensureT<int32_t>(c, c->bCHCur, nm);
 	ca.hext = ensureT<int32_t>(c, c->bCHExt, nm);
 	ca.score = ensureT<int32_t>(c, c->bCScore, nm);
@@ -508,12 +543,10 @@ BatchOut runChain(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches, const st
 	ca.props = ensureT<Proposal>(c, c->bCProps, nm);
 	ca.outProps = ensureT<Proposal>(c, c->bCOut, nm);
 	ca.groupCount = ensureT<int32_t>(c, c->bGroupCount, static_cast<size_t>(nGroups) + 1);
-	ca.queue = ensureT<int32_t>(c, c->bQueue, 16);
+	ca.groupPass = ensureT<int32_t>(c, c->bGroupPass, static_cast<size_t>(nGroups) + 1);
 	ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
 	ca.logY = static_cast<const uint32_t*>(c->dLogY.p);
-	zero32(c, ca.queue, 16);
-	zero32(c, ca.groupCount, nGroups + 1);
-	if (nGroups > 0) HIPCHECK(launchChainGroups(ca, chainBlocks(c), c->stream));
+	runChainKernels(c, ca);
 	int64_t* rowStart = ensureT<int64_t>(c, c->bRowStart, static_cast<size_t>(nGroups) + 1);
 	HIPCHECK(exclusiveScan(ca.groupCount, rowStart, nGroups, tmp, c->stream));
 	int64_t nRows = 0;
@@ -734,10 +767,10 @@ int flyehip_close(flyehip_ctx* c)
 	DevBuf* all[] = {&c->dWords, &c->dWordOff, &c->dReadLen, &c->dReadWinOff, &c->dKeys, &c->dVals, &c->dCursor,
 					 &c->dEntries, &c->dLogX, &c->dLogY, &c->bQueries, &c->bWinOff, &c->bWinCount, &c->bWinFilt,
 					 &c->bWinInfo, &c->bMatchOff, &c->bFiltOff, &c->bScanTmp, &c->bMKey, &c->bMExt, &c->bVal,
-					 &c->bPosL, &c->bPosR, &c->bSmall, &c->bHCur, &c->bHExt, &c->bGroupFlag, &c->bGroupOff,
+					 &c->bPosL, &c->bPosR, &c->bHCur, &c->bHExt, &c->bGroupFlag, &c->bGroupOff,
 					 &c->bFiltered, &c->bQMatchOff, &c->bQFiltOff, &c->bQDesc, &c->bGroups, &c->bGroupCount,
 					 &c->bRowStart, &c->bRows, &c->bCScore, &c->bCBt, &c->bCProps, &c->bCOut, &c->bCHCur,
-					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bCSmall, &c->bQueue,
+					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bGroupPass, &c->bOrder, &c->bBins, &c->bQueue,
 					 &c->bCounters, &c->bQRowCount, &c->bCurLen, &c->bDivOff, &c->bDivCount, &c->bDivOut,
 					 &c->bQueryIds, &c->bCand, &c->bExtWords, &c->bExtWordOff, &c->bExtLen, &c->bAln, &c->bAlnOff,
 					 &c->bRA};
@@ -763,7 +796,8 @@ int flyehip_set_tuning(flyehip_ctx* c, int32_t lanes, int32_t qpb, int32_t gpp)
 	return guarded(c, [&]() {
 		if (lanes)
 		{
-			if (lanes != 64) fail(FLYEHIP_ERR_ARG, "dp_lanes_per_group %d not supported (64)", lanes);
+			if (lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+				fail(FLYEHIP_ERR_ARG, "dp_lanes_per_group %d not supported (1, 4, 8, 16, 32, 64)", lanes);
 			c->dpLanes = lanes;
 		}
 		if (qpb)
@@ -1039,7 +1073,6 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 		ca.sortVal = ensureT<uint32_t>(c, c->bCVal, nm);
 		ca.posL = ensureT<int32_t>(c, c->bCPosL, nm);
 		ca.posR = ensureT<int32_t>(c, c->bCPosR, nm);
-		ca.small = ensureT<Seg>(c, c->bCSmall, nm);
 		ca.hcur = ensureT<int32_t>(c, c->bCHCur, nm);
 		ca.hext = ensureT<int32_t>(c, c->bCHExt, nm);
 		ca.score = ensureT<int32_t>(c, c->bCScore, nm);
@@ -1047,24 +1080,16 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 		ca.props = ensureT<Proposal>(c, c->bCProps, nm);
 		ca.outProps = ensureT<Proposal>(c, c->bCOut, nm);
 		ca.groupCount = ensureT<int32_t>(c, c->bGroupCount, gs.size() + 1);
-		ca.queue = ensureT<int32_t>(c, c->bQueue, 16);
+		ca.groupPass = ensureT<int32_t>(c, c->bGroupPass, gs.size() + 1);
 		ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
 		ca.logY = static_cast<const uint32_t*>(c->dLogY.p);
-		zero32(c, ca.queue, 16);
-		zero32(c, ca.groupCount, static_cast<int64_t>(gs.size()) + 1);
-		Timer tc;
-		HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-		if (!gs.empty()) HIPCHECK(launchChainGroups(ca, chainBlocks(c), c->stream));
-		HIPCHECK(hipEventRecord(c->ev[1], c->stream));
+		runChainKernels(c, ca);
 		int64_t* rowStart = ensureT<int64_t>(c, c->bRowStart, gs.size() + 1);
 		int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(static_cast<int64_t>(gs.size())) + 64);
 		HIPCHECK(exclusiveScan(ca.groupCount, rowStart, static_cast<int64_t>(gs.size()), tmp, c->stream));
 		int64_t nRows = 0;
 		download(c, &nRows, rowStart + gs.size(), 8);
 		sync(c);
-		float evMs = 0.0f;
-		HIPCHECK(hipEventElapsedTime(&evMs, c->ev[0], c->ev[1]));
-		c->stats.ms_chain = evMs;
 		flyehip_overlap_t* dr = ensureT<flyehip_overlap_t>(c, c->bRows, static_cast<size_t>(nRows) + 1);
 		HIPCHECK(launchWriteRows(ca, rowStart, dr, c->stream));
 		std::vector<int32_t> gc(gs.size());
