@@ -525,7 +525,23 @@ __global__ void binPrefixKernel(int32_t* binCount, int32_t* binCursor)
 	}
 }
 
+__global__ void logfKernel(const float* x, int64_t n, float* out, const uint32_t* tabX, const uint32_t* tabY,
+						   int tabN)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i < n) out[i] = flyeLogf(x[i], tabX, tabY, tabN);
+}
+
 } // namespace
+
+hipError_t launchLogf(const float* x, int64_t n, float* out, const uint32_t* tabX, const uint32_t* tabY, int tabN,
+					  hipStream_t stream)
+{
+	if (n == 0) return hipSuccess;
+	hipLaunchKernelGGL(logfKernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, x, n, out, tabX,
+					   tabY, tabN);
+	return hipGetLastError();
+}
 
 hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* bins, int32_t* order,
 							hipStream_t stream)

@@ -76,6 +76,9 @@ struct ChainArgs
 // Per-hit scratch bytes the chaining kernels need.
 constexpr size_t kChainScratchPerHit = 8 + 4 + 4 + 4 + 4 + 4 + 4 + 4 + 2 * sizeof(Proposal);
 
+// Flye's divergence logf (glibc-exact) for an array of arguments.
+hipError_t launchLogf(const float* x, int64_t n, float* out, const uint32_t* tabX, const uint32_t* tabY, int tabN,
+					  hipStream_t stream);
 hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* bins, int32_t* order,
 							hipStream_t stream);
 hipError_t launchChainPrep(const ChainArgs& a, int blocks, hipStream_t stream);

@@ -1110,6 +1110,25 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 	});
 }
 
+int flyehip_divergence_logf(flyehip_ctx* c, const float* x, int64_t n, float* out)
+{
+	return guarded(c, [&]() {
+		if (n < 0 || (n > 0 && (!x || !out))) fail(FLYEHIP_ERR_ARG, "invalid logf arguments");
+		const int64_t chunk = 1 << 26;
+		float* dx = ensureT<float>(c, c->bDivOut, static_cast<size_t>(std::min(n, chunk)) + 1);
+		float* dy = ensureT<float>(c, c->bCKey, static_cast<size_t>(std::min(n, chunk)) + 1);
+		for (int64_t b = 0; b < n; b += chunk)
+		{
+			const int64_t m = std::min(chunk, n - b);
+			upload(c, dx, x + b, static_cast<size_t>(m) * 4);
+			HIPCHECK(launchLogf(dx, m, dy, static_cast<const uint32_t*>(c->dLogX.p),
+								static_cast<const uint32_t*>(c->dLogY.p), c->logN, c->stream));
+			download(c, out + b, dy, static_cast<size_t>(m) * 4);
+			sync(c);
+		}
+	});
+}
+
 int flyehip_read_align_chains(flyehip_ctx* c, const flyehip_edge_alignment_t* alignments, const int64_t* aln_offsets,
 							  int32_t n_reads, int32_t long_edge, int32_t big_aln, const flyehip_chain_params_t* params,
 							  int32_t* sorted_index, int64_t* kept_offsets, flyehip_chain_segment_t* segments,

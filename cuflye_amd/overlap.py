@@ -272,6 +272,12 @@ class FlyeHip:
             self._check(rc)
             return rows[:need.value], roff
 
+    def divergence_logf(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty_like(x)
+        self._check(self.lib.flyehip_divergence_logf(self.ctx, N.ptr(x), len(x), N.ptr(out)))
+        return out
+
     def read_align_chains(self, alignments: np.ndarray, aln_offsets, long_edge: int, big_aln: int,
                           params: N.ChainParams):
         a = np.ascontiguousarray(alignments, N.EDGE_ALIGNMENT_DTYPE)

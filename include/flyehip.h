@@ -304,6 +304,14 @@ int flyehip_read_align_chains(flyehip_ctx* ctx,
                               int64_t segment_capacity, int64_t* segment_offsets,
                               int64_t* segments_needed);
 
+/* ---- the divergence logarithm --------------------------------------------- */
+
+/* out[i] = logf(x[i]) exactly as glibc's logf returns it, computed on the GPU
+ * the way the chaining kernels compute seq_divergence
+ * (tools/replay_overlap_chains.py:87-111, 348-361: float logf of 1 / matchRate).
+ * Exact for every float in [1, 2^25), the values 1 / matchRate takes. */
+int flyehip_divergence_logf(flyehip_ctx* ctx, const float* x, int64_t n, float* out);
+
 #ifdef __cplusplus
 }
 #endif
