@@ -9,9 +9,16 @@ reads and the VertexIndex are resident in HBM before the timed region; one
 step is getSeqOverlaps (k-mer join, exact match sort, chaining DP, filters)
 for every query record (both strands of every read), rows copied back.
 
-Prints one JSON line (see the keys below); the CPU baseline is the oracle
-(a C++ restatement of Flye's CPU stage) timed on a bounded sample of query
-records with every host core, and that sample doubles as a parity spot check.
+Beside the headline number the JSON line carries:
+  roofline      - the dominant kernel's achieved algorithmic GB/s (HIP events
+                  on the library stream) against MI355X's 8 TB/s;
+  cpu_baseline  - the oracle (C++ restatement of Flye's CPU stage) on a
+                  bounded sample of query records, every host core;
+  per_query     - the plain per-query launch path on a sample;
+  serving       - p50/p99 request latency and reads/s of the continuous-
+                  batching loop over a resident 200 Mb genome index, Poisson
+                  arrivals (disable with --serve-seconds 0).
+Parity: the CPU sample's rows must equal the GPU's bit for bit.
 """
 from __future__ import annotations
 
@@ -37,9 +44,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="genome scale of the read set")
     ap.add_argument("--queries", type=int, default=0, help="limit query records (0 = all)")
-    ap.add_argument("--cpu-sample", type=int, default=96, help="query records timed on the CPU oracle")
-    ap.add_argument("--batch", type=int, default=0, help="queries per GPU batch (0 = library default)")
-    ap.add_argument("--per-query", action="store_true", help="time the plain per-query launch path")
+    ap.add_argument("--cpu-sample", type=int, default=1500, help="query records timed on the CPU oracle")
+    ap.add_argument("--per-query-sample", type=int, default=200, help="query records on the per-query path")
+    ap.add_argument("--batch", type=int, default=0, help="queries per GPU batch (0 = autotune)")
+    ap.add_argument("--dp-lanes", type=int, default=0, help="DP lanes per group (0 = autotune)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson request rate (1/s)")
+    ap.add_argument("--serve-genome", type=int, default=200_000_000)
+    ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
     return ap.parse_args()
 
@@ -48,16 +60,72 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def serving_leg(args, cores: int) -> dict:
+    import cuflye_amd.synth as S
+    from cuflye_amd.overlap import FlyeHip, OverlapConfig
+    from cuflye_amd.serve import OverlapServer, Request
+
+    t = time.time()
+    genome = S.genome_with_repeats(args.serve_genome, 0.42, 0.08, 1000, 10000, 2, 50, 0.005, 0.05, seed=5)
+    n_contigs = max(1, len(genome) // 10_000_000)
+    step = len(genome) // n_contigs
+    offs = np.array([i * step for i in range(n_contigs)] + [len(genome)], np.int64)
+    pool_b, pool_o = S.reads_from(genome, 20000, seed=55)
+    log(f"serve: genome {len(genome)} bp in {n_contigs} contigs, read pool {len(pool_o) - 1} "
+        f"({time.time() - t:.1f}s)")
+    ctx = FlyeHip(device=0)
+    t = time.time()
+    ctx.store_reads(genome, offs)
+    info = ctx.build_index(15, solid_min=1, rep_max=64)
+    log(f"serve: index {info.indexed_positions} positions, {info.repetitive_kmers} repetitive k-mers, "
+        f"{time.time() - t:.1f}s")
+    del genome
+    cfg = OverlapConfig(kmer_size=15, minimum_overlap=1000)
+    srv = OverlapServer(ctx, cfg, admission_bytes=64 << 30, max_iteration_reads=2048)
+    srv.start()
+    rng = np.random.default_rng(7)
+    t0 = time.perf_counter()
+    nid = 1_000_000_001
+    reqs = []
+    next_t = t0
+    pool_n = len(pool_o) - 1
+    while True:
+        next_t += rng.exponential(1.0 / args.serve_rate)
+        if next_t - t0 > args.serve_seconds:
+            break
+        while time.perf_counter() < next_t:
+            time.sleep(0.0005)
+        n = int(np.exp(rng.uniform(np.log(500), np.log(20000))))
+        start = int(rng.integers(0, pool_n - n + 1)) if n < pool_n else 0
+        n = min(n, pool_n)
+        b = pool_b[pool_o[start]:pool_o[start + n]]
+        o = pool_o[start:start + n + 1] - pool_o[start]
+        ids = np.arange(nid, nid + n, dtype=np.int64)
+        nid += n
+        reqs.append(srv.submit(Request(len(reqs), b, o, ids)))
+    for r in reqs:
+        r.done_event.wait(600)
+    srv.stop()
+    rep = srv.latency_report()
+    rep["rate_per_s"] = args.serve_rate
+    rep["seconds"] = args.serve_seconds
+    rep["request_reads"] = "log-uniform 500-20000"
+    ctx.close()
+    return rep
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the path shards by query batches "
-                          "across GPUs (see DESIGN.md) but no multi-GPU run is part of it",
+                          "across GPUs (DESIGN.md section 9) but no multi-GPU run is part of it",
                           "n_gpus": args.gpus}))
         return 2
     import cuflye_amd.synth as S
+    from cuflye_amd import autotune
     from cuflye_amd.overlap import FlyeHip, OverlapConfig
 
+    cores = os.cpu_count() or 1
     t0 = time.time()
     reads = S.config2(seed=args.seed, scale=args.scale)
     log(f"reads={reads.n_reads} bases={len(reads.bases)} gen={time.time() - t0:.1f}s")
@@ -70,8 +138,6 @@ def main() -> int:
     cfg = OverlapConfig(kmer_size=15, minimum_overlap=5000)
 
     g = FlyeHip(device=0)
-    if args.batch:
-        g.set_tuning(queries_per_batch=args.batch)
     t = time.time()
     g.store_reads(reads.bases, reads.offsets)
     info = g.build_index(15)
@@ -79,12 +145,21 @@ def main() -> int:
     log(f"index: solid_min={info.solid_min} rep_max={info.rep_max} positions={info.indexed_positions} "
         f"distinct={info.distinct_kmers} build={index_s:.2f}s")
 
+    tuning = None
+    if args.batch and args.dp_lanes:
+        g.set_tuning(dp_lanes_per_group=args.dp_lanes, queries_per_batch=args.batch)
+    else:
+        sample = ids[: min(len(ids), 1024)]
+        tuning = autotune.tune(g, sample, cfg, batches=(args.batch,) if args.batch else (512, 1024),
+                               lanes=(args.dp_lanes,) if args.dp_lanes else autotune.DP_LANES, log=log)
+        log(f"autotune best: {tuning['best']}")
+
     def step():
-        return g.get_seq_overlaps(ids, cfg, per_query=args.per_query)
+        return g.get_seq_overlaps(ids, cfg)
 
     for w in range(args.warmup):
         rows, roff = step()
-        log(f"warmup {w}: rows={len(rows)} stats={g.stats()}")
+        log(f"warmup {w}: rows={len(rows)}")
     times = []
     stats = None
     for s in range(args.steps):
@@ -92,33 +167,60 @@ def main() -> int:
         rows, roff = step()
         times.append(time.perf_counter() - t)
         stats = g.stats()
-        log(f"step {s}: {times[-1]:.3f}s rows={len(rows)}")
+        log(f"step {s}: {times[-1]:.3f}s rows={len(rows)} stats={stats}")
     ms = 1000.0 * float(np.mean(times))
     value = len(ids) / (ms / 1000.0)
 
-    # roofline of the join (k-mer index probe + bucket walk + match write):
-    # algorithmic bytes per query = ceil(L/4) + 8 W + 8 H + 16 H
-    lens = (reads.offsets[1:] - reads.offsets[:-1])[np.abs(ids) - 1]
-    W = np.maximum(lens - 15 + 1, 0).sum()
-    H = stats["matches"]
-    join_bytes = float(np.ceil(lens / 4).sum() + 8 * W + 8 * H + 16 * H)
-    join_s = stats["ms_join"] / 1000.0
-    achieved = join_bytes / join_s / 1e9 if join_s > 0 else 0.0
+    # dominant kernel and its algorithmic bytes
+    lens = (reads.offsets[1:] - reads.offsets[:-1])[np.abs(ids) - 1].astype(np.int64)
+    W = int(np.maximum(lens - 15 + 1, 0).sum())
+    H = int(stats["matches"])
+    G = int(stats["groups"])
+    R = int(stats["overlaps"])
+    stage_bytes = {
+        "ms_join": float(np.ceil(lens / 4).sum() + 8 * W + 8 * H + 12 * H),
+        "ms_sort": float(2 * 12 * H),
+        "ms_group": float(8 * H + 8 * H),
+        "ms_chain": float(8 * H + 8 * H),
+        "ms_extract": float(8 * H + 40 * R),
+    }
+    dom = max(stage_bytes, key=lambda k: stats[k])
+    dom_s = stats[dom] / 1000.0
+    achieved = stage_bytes[dom] / dom_s / 1e9 if dom_s > 0 else 0.0
+
+    # plain per-query launch path on a sample
+    pq = ids[: min(args.per_query_sample, len(ids))]
+    t = time.perf_counter()
+    prows, proff = g.get_seq_overlaps(pq, cfg, per_query=True)
+    pq_s = time.perf_counter() - t
+    brows, broff = g.get_seq_overlaps(pq, cfg)
+    assert np.array_equal(proff, broff) and prows.tobytes() == brows.tobytes(), "per-query path differs"
+    per_query_value = len(pq) / pq_s
 
     # CPU baseline + parity spot check on a bounded sample
-    from oracle import oracle as O
-    cores = os.cpu_count() or 1
-    sample = ids[: min(args.cpu_sample, len(ids))]
-    orc = O.Oracle(reads.bases, reads.offsets)
-    t = time.time()
-    orc.build_index(15)
-    log(f"oracle index {time.time() - t:.1f}s")
-    t = time.perf_counter()
-    erows, eroff = orc.overlaps(sample, cfg, threads=cores)
-    cpu_s = time.perf_counter() - t
-    grows, groff = g.get_seq_overlaps(sample, cfg)
-    parity = bool(np.array_equal(groff, eroff) and grows.tobytes() == erows.tobytes())
-    log(f"cpu sample {len(sample)} queries in {cpu_s:.2f}s on {cores} threads; parity={parity}")
+    cpu = None
+    parity = None
+    if not args.skip_cpu:
+        from oracle import oracle as O
+        sample = ids[: min(args.cpu_sample, len(ids))]
+        orc = O.Oracle(reads.bases, reads.offsets)
+        t = time.time()
+        oinfo = orc.build_index(15)
+        log(f"oracle index {time.time() - t:.1f}s")
+        assert oinfo["solid_min"] == info.solid_min and oinfo["positions"] == info.indexed_positions
+        t = time.perf_counter()
+        erows, eroff = orc.overlaps(sample, cfg, threads=cores)
+        cpu_s = time.perf_counter() - t
+        grows, groff = g.get_seq_overlaps(sample, cfg)
+        parity = bool(np.array_equal(groff, eroff) and grows.tobytes() == erows.tobytes())
+        cpu = {"value": len(sample) / cpu_s, "unit": "query records/s", "cores": cores, "kind": "port",
+               "sample": f"first {len(sample)} query records of the same workload ({cpu_s:.1f} s)"}
+        log(f"cpu sample {len(sample)} in {cpu_s:.2f}s on {cores} threads; parity={parity}")
+    g.close()
+
+    serving = serving_leg(args, cores) if args.serve_seconds > 0 else None
+    if serving:
+        log(f"serving: {serving}")
 
     out = {
         "metric": "overlap reads/s (bit-exact vs Flye CPU), % HBM peak; p50/p99 batch latency",
@@ -136,18 +238,21 @@ def main() -> int:
         "config": {"workload": "5 Mb ONT-like read set, 40x, k=15, getSeqOverlaps for both strands of every read",
                    "genome_bp": reads.genome_len, "reads": int(n), "query_records": int(len(ids)),
                    "bases": int(len(reads.bases)), "minimum_overlap": cfg.minimum_overlap,
-                   "per_query_launch": bool(args.per_query)},
+                   "tuning": tuning["best"] if tuning else {"dp_lanes": args.dp_lanes, "batch": args.batch}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "join (count+fill)"},
-        "cpu_baseline": {"value": len(sample) / cpu_s, "unit": "query records/s", "cores": cores,
-                         "kind": "port", "sample": f"first {len(sample)} query records of the same workload"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": dom, "algorithmic_bytes": stage_bytes[dom], "kernel_ms": stats[dom]},
+        "cpu_baseline": cpu,
         "parity_sample": parity,
+        "per_query": {"value": per_query_value, "unit": "query records/s", "sample": int(len(pq)),
+                      "speedup_batched": value / per_query_value},
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "serving": serving,
         "stage_ms": {k: stats[k] for k in ("ms_join", "ms_sort", "ms_group", "ms_chain", "ms_extract", "ms_total")},
         "counts": {k: stats[k] for k in ("windows", "matches", "groups", "overlaps", "batches", "tie_queries")},
         "index_build_s": index_s,
     }
     print(json.dumps(out))
-    g.close()
     return 0
 
 

@@ -359,33 +359,118 @@ __global__ void joinCountKernel(JoinArgs a)
 	a.winInfo[w] = info;
 }
 
-__global__ void joinFillKernel(JoinArgs a)
+// Self entry of a window inside its bucket: -1 when the window cannot hit
+// itself.  Resident reads: the bucket is in (read, forward position) order, so
+// the entry stored for the query read's own window is found by binary search;
+// other queries scan the bucket.
+__device__ inline int selfIndex(const JoinArgs& a, const JoinQuery& Q, int32_t pos, bool g, uint64_t start, int n)
 {
-	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (w >= a.nWin) return;
-	const int q = findSeg(a.winOff, a.nQueries, w);
-	const int32_t pos = static_cast<int32_t>(w - a.winOff[q]);
-	if (a.winFilt[w]) a.filtered[a.filtOff[w]] = pos;
-	if (a.winCount[w] == 0) return;
-	const uint32_t info = a.winInfo[w];
-	const bool g = (info & 0x80000000U) != 0;
-	const uint64_t h = info & 0x7fffffffU;
-	const uint64_t v = a.table.vals[h];
-	const uint64_t start = v >> kCountBits;
-	const int n = static_cast<int>(v & kCountMask);
-	const uint32_t qiid = a.queries[q].iid;
-	const bool maySelf = a.queries[q].selfMode != 0;
-	int64_t o = a.matchOff[w];
-	for (int i = 0; i < n; ++i)
+	if (Q.selfMode == 0) return -1;
+	if (Q.selfMode == 2)
 	{
-		uint32_t id;
-		int32_t p;
-		transformEntry(a.entries[start + i], g, a.targetLen, a.k, id, p);
-		if (maySelf && id == qiid && p == pos) continue;
-		a.mkey[o] = (static_cast<uint64_t>(id) << 32) | static_cast<uint32_t>(pos);
-		a.mext[o] = p;
-		++o;
+		for (int i = 0; i < n; ++i)
+		{
+			uint32_t id;
+			int32_t p;
+			transformEntry(a.entries[start + i], g, a.targetLen, a.k, id, p);
+			if (id == Q.iid && p == pos) return i;
+		}
+		return -1;
 	}
+	const uint32_t r = static_cast<uint32_t>(Q.seq);
+	const int32_t pf = Q.strand ? pos : Q.len - pos - a.k;
+	int lo = 0, hi = n;
+	while (lo < hi)
+	{
+		const int mid = (lo + hi) >> 1;
+		const uint64_t e = a.entries[start + mid];
+		const uint32_t id = static_cast<uint32_t>(e >> 32);
+		const uint32_t rd = id >> 1;
+		const int32_t ep = static_cast<int32_t>(static_cast<uint32_t>(e));
+		const int32_t fp = (id & 1U) ? a.targetLen[rd] - ep - a.k : ep;
+		if (rd < r || (rd == r && fp < pf)) lo = mid + 1;
+		else hi = mid;
+	}
+	if (lo >= n) return -1;
+	uint32_t id;
+	int32_t p;
+	transformEntry(a.entries[start + lo], g, a.targetLen, a.k, id, p);
+	return (id == Q.iid && p == pos) ? lo : -1;
+}
+
+// One wave fills the matches of 64 consecutive windows together: the raw
+// bucket entries of the 64 windows are enumerated across the lanes (entry t
+// of the wave goes to lane t % 64), so bucket reads and match writes are
+// contiguous runs instead of one run per lane.
+__global__ void __launch_bounds__(256) joinFillKernel(JoinArgs a)
+{
+	const int lane = laneId();
+	const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) - lane;
+	const int64_t w = w0 + lane;
+	int raw = 0;
+	uint64_t start = 0;
+	int self = -1;
+	int64_t outBase = 0;
+	uint32_t qiid = 0;
+	int32_t pos = 0;
+	int gflag = 0;
+	if (w < a.nWin)
+	{
+		const int q = findSeg(a.winOff, a.nQueries, w);
+		pos = static_cast<int32_t>(w - a.winOff[q]);
+		if (a.winFilt[w]) a.filtered[a.filtOff[w]] = pos;
+		if (a.winCount[w] > 0)
+		{
+			const uint32_t info = a.winInfo[w];
+			const bool g = (info & 0x80000000U) != 0;
+			const uint64_t v = a.table.vals[info & 0x7fffffffU];
+			start = v >> kCountBits;
+			raw = static_cast<int>(v & kCountMask);
+			const JoinQuery Q = a.queries[q];
+			qiid = Q.iid;
+			gflag = g ? 1 : 0;
+			self = selfIndex(a, Q, pos, g, start, raw);
+			outBase = a.matchOff[w];
+		}
+	}
+	const int incl = waveInclusiveSum(raw);
+	const int total = __shfl(incl, FH_WAVE - 1, FH_WAVE);
+	const int excl = incl - raw;
+	for (int t0 = 0; t0 < total; t0 += FH_WAVE)
+	{
+		const int t = t0 + lane;
+		// window of entry t: largest lane l with excl(l) <= t
+		int lo = 0;
+		for (int step = 32; step > 0; step >>= 1)
+		{
+			const int cand = lo + step;
+			const int e = __shfl(excl, cand < FH_WAVE ? cand : FH_WAVE - 1, FH_WAVE);
+			if (cand < FH_WAVE && e <= t) lo = cand;
+		}
+		const int l = lo;
+		const int rawL = __shfl(raw, l, FH_WAVE);
+		const int exL = __shfl(excl, l, FH_WAVE);
+		const uint64_t stL = __shfl(start, l, FH_WAVE);
+		const int selfL = __shfl(self, l, FH_WAVE);
+		const int64_t obL = __shfl(outBase, l, FH_WAVE);
+		const int gL = __shfl(gflag, l, FH_WAVE);
+		const int32_t posL = __shfl(pos, l, FH_WAVE);
+		if (t < total)
+		{
+			const int j = t - exL;
+			(void)rawL;
+			if (j != selfL)
+			{
+				uint32_t id;
+				int32_t p;
+				transformEntry(a.entries[stL + j], gL != 0, a.targetLen, a.k, id, p);
+				const int64_t o = obL + j - ((selfL >= 0 && j > selfL) ? 1 : 0);
+				a.mkey[o] = (static_cast<uint64_t>(id) << 32) | static_cast<uint32_t>(posL);
+				a.mext[o] = p;
+			}
+		}
+	}
+	(void)qiid;
 }
 
 __global__ void writeCandidatesKernel(JoinArgs a, const int64_t* qMatchOff, const int64_t* queryIds,
@@ -463,17 +548,22 @@ __global__ void buildGroupsKernel(const int64_t* qMatchOff, int32_t nQueries, co
 	const int64_t g = groupOff[i];
 	const int q = findSeg(qMatchOff, nQueries, i);
 	const uint32_t eid = static_cast<uint32_t>(key[i] >> 32);
-	// group end: next flag position (scan forward; groups are contiguous)
-	int64_t e = i + 1;
-	while (e < nMatches && !groupFlag[e]) ++e;
 	GroupDesc d;
 	d.hitStart = i;
 	d.extId = toSigned(eid);
-	d.n = static_cast<int32_t>(e - i);
+	d.n = 0;  // set by groupSizeKernel from the next group's start
 	d.query = q;
 	d.extLen = targetLen[eid >> 1];
 	d.reserved = 0;
 	groups[g] = d;
+}
+
+__global__ void groupSizeKernel(GroupDesc* groups, int64_t nGroups, int64_t nMatches)
+{
+	const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (g >= nGroups) return;
+	const int64_t end = g + 1 < nGroups ? groups[g + 1].hitStart : nMatches;
+	groups[g].n = static_cast<int32_t>(end - groups[g].hitStart);
 }
 
 __global__ void divStatsKernel(const flyehip_overlap_t* rows, const int64_t* qRowOff, const int32_t* curLen,
@@ -621,6 +711,15 @@ hipError_t launchBuildGroups(const int64_t* qMatchOff, int32_t nQueries, const u
 	if (nMatches == 0) return hipSuccess;
 	hipLaunchKernelGGL(buildGroupsKernel, gridFor(nMatches, 256), dim3(256), 0, s, qMatchOff, nQueries, key, groupFlag,
 					   groupOff, targetLen, groups, nMatches);
+	const int64_t nGroups = 0;  // filled below from the scan total
+	(void)nGroups;
+	return hipGetLastError();
+}
+
+hipError_t launchGroupSizes(GroupDesc* groups, int64_t nGroups, int64_t nMatches, hipStream_t s)
+{
+	if (nGroups == 0) return hipSuccess;
+	hipLaunchKernelGGL(groupSizeKernel, gridFor(nGroups, 256), dim3(256), 0, s, groups, nGroups, nMatches);
 	return hipGetLastError();
 }
 

@@ -113,6 +113,8 @@ hipError_t launchBuildGroups(const int64_t* qMatchOff, int32_t nQueries, const u
 							 const int32_t* groupFlag, const int64_t* groupOff, const int32_t* targetLen,
 							 GroupDesc* groups, int64_t nMatches, hipStream_t s);
 
+hipError_t launchGroupSizes(GroupDesc* groups, int64_t nGroups, int64_t nMatches, hipStream_t s);
+
 // Per-query divStats windows (patch 0023:245-269) over the written rows.
 hipError_t launchDivStats(const flyehip_overlap_t* rows, const int64_t* qRowOff, const int32_t* curLen,
 						  int32_t nQueries, int32_t* divCount, float* divOut, const int64_t* divOff,
