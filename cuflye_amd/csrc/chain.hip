@@ -27,7 +27,7 @@ namespace fh
 namespace
 {
 
-constexpr int kTile = 1024;
+constexpr int kTile = 2048;  // elements of a wave's LDS sort tile (32-bit keys)
 constexpr int kLaneMax = 48;
 constexpr int kRingTotal = 4096;  // LDS score-ring entries per wave, split over groups
 
@@ -108,7 +108,7 @@ __device__ inline int pullGroup(int32_t* queue)
 
 __global__ void __launch_bounds__(FH_WAVE) chainPrepKernel(ChainArgs a)
 {
-	__shared__ WaveSortLds<kTile> lds;
+	__shared__ WaveSortLds<uint32_t, kTile> lds;
 	const int lane = laneId();
 	const flyehip_overlap_params_t& p = a.params;
 	for (;;)
@@ -161,15 +161,16 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrepKernel(ChainArgs a)
 		const bool extSorted = gd.extLen > qd.curLen;
 		if (extSorted)
 		{
-			uint64_t* sk = a.sortKey + base;
+			uint32_t* sk = a.sortKey + base;
 			uint32_t* sv = a.sortVal + base;
 			for (int i = lane; i < n; i += FH_WAVE)
 			{
-				sk[i] = static_cast<uint64_t>(static_cast<uint32_t>(ext0[i]));
+				sk[i] = static_cast<uint32_t>(ext0[i]);
 				sv[i] = static_cast<uint32_t>(i);
 			}
 			waveSync();
-			waveStdSort<kTile, kLaneMax>(exsort::Span{sk, sv}, n, a.posL + base, a.posR + base, lds);
+			waveStdSort<uint32_t, kTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, n, a.posL + base, a.posR + base,
+												  lds);
 			for (int i = lane; i < n; i += FH_WAVE)
 			{
 				const uint32_t src = sv[i];
@@ -316,11 +317,13 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			}
 			++i;
 		}
-		// ring writes of this row are read by the next row's lanes; stores to
-		// sc[] become visible to far lookbacks after the periodic fence
+		// The ring write of this row is read by the next row's lanes: LDS
+		// operations of one wave execute in order, so only the compiler must
+		// not move the next row's loads above it (wave barrier).  Stores to
+		// sc[] reach far lookbacks (more than kRing rows back) through the
+		// periodic fence, which waits for them.
 		++iter;
 		if ((iter & 31) == 0) __threadfence_block();
-		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 		__builtin_amdgcn_wave_barrier();
 	}
 }
@@ -329,7 +332,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 
 __global__ void __launch_bounds__(FH_WAVE) chainFinishKernel(ChainArgs a)
 {
-	__shared__ WaveSortLds<kTile> lds;
+	__shared__ WaveSortLds<uint32_t, kTile> lds;
+	__shared__ int btS[kTile];
 	__shared__ int red[2];
 	const int lane = laneId();
 	const flyehip_overlap_params_t& p = a.params;
@@ -347,74 +351,95 @@ __global__ void __launch_bounds__(FH_WAVE) chainFinishKernel(ChainArgs a)
 		const int32_t* hc = a.hcur + base;
 		const int32_t* he = a.hext + base;
 		const int32_t* sc = a.score + base;
-		int32_t* bt = a.bt + base;
-		uint64_t* sk = a.sortKey + base;
+		uint32_t* sk = a.sortKey + base;
 		uint32_t* sv = a.sortVal + base;
 		int32_t* posL = a.posL + base;
 		int32_t* posR = a.posR + base;
+		// backtrack pointers in LDS when the group fits, else in place
+		const bool inLds = n <= kTile;
+		int32_t* bt = inLds ? btS : a.bt + base;
+		const uint32_t* order = inLds ? lds.v : sv;
 
 		// score order: std::sort(indices, score[a] > score[b])
 		for (int i = lane; i < n; i += FH_WAVE)
 		{
-			sk[i] = static_cast<uint64_t>(static_cast<uint32_t>(0x7fffffff - sc[i]));
+			sk[i] = static_cast<uint32_t>(0x7fffffff - sc[i]);
 			sv[i] = static_cast<uint32_t>(i);
+			if (inLds) btS[i] = a.bt[base + i];
 		}
 		waveSync();
-		waveStdSort<kTile, kLaneMax>(exsort::Span{sk, sv}, n, posL, posR, lds);
+		waveStdSort<uint32_t, kTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, n, posL, posR, lds);
 
-		// backtrack, overlapTest, divergence (lane 0)
+		// backtrack, overlapTest, divergence.  Chunks of 64 rows in score order:
+		// the lanes test which starts still have a backtrack pointer, lane 0
+		// walks those in order (a walk clears the pointers it passes, so lane 0
+		// re-checks each start before walking it).
 		Proposal* props = a.props + base;
 		Proposal* outp = a.outProps + base;
-		if (lane == 0)
+		const int32_t* filt = a.filtered + qd.filtStart;
+		const int filtN = qd.filtCount;
+		int np = 0;
+		int best = -1;
+		for (int q0 = 0; q0 < n; q0 += FH_WAVE)
 		{
-			const int32_t* filt = a.filtered + qd.filtStart;
-			const int filtN = qd.filtCount;
-			int np = 0;
-			int best = -1;
-			for (int q = 0; q < n; ++q)
+			const int q = q0 + lane;
+			const int myStart = q < n ? static_cast<int>(order[q]) : 0;
+			const bool live = q < n && bt[myStart] != -1;
+			unsigned long long m = __ballot(live);
+			waveSync();
+			if (lane == 0)
 			{
-				const int start = static_cast<int>(sv[q]);
-				if (bt[start] == -1) continue;
-				int first = 0, len = 0, pos = start;
-				while (pos != -1)
+				while (m)
 				{
-					first = pos;
-					++len;
-					const int nxt = bt[pos];
-					bt[pos] = -1;
-					pos = nxt;
-				}
-				Proposal o;
-				o.curBegin = hc[first];
-				o.curEnd = hc[start] + k - 1;
-				o.extBegin = he[first];
-				o.extEnd = he[start] + k - 1;
-				o.score = sc[start] - sc[first] + k - 1;
-				o.chainLength = len;
-				o.div = 0.0f;
-				if (!overlapTestDev(o, qd.curId, gd.extId, qd.curLen, gd.extLen, p)) continue;
-				const int fc = upperBound(filt, filtN, o.curEnd) - lowerBound(filt, filtN, o.curBegin);
-				const float normLen = static_cast<float>(max(o.curEnd - o.curBegin, o.extEnd - o.extBegin) - fc);
-				if (!(normLen > 0.0f)) continue;
-				float rate = static_cast<float>(len) * p.sample_rate / normLen;
-				rate = fminf(rate, 1.0f);
-				if (!(rate > 0.0f)) continue;
-				const float inv = 1.0f / rate;
-				const float lg = flyeLogf(inv, a.logX, a.logY, a.logN);
-				o.div = lg / static_cast<float>(k);
-				if (p.only_max_ext)
-				{
-					if (best < 0 || o.score > outp[0].score)
+					const int l = __ffsll(static_cast<long long>(m)) - 1;
+					m &= m - 1;
+					const int start = static_cast<int>(order[q0 + l]);
+					if (bt[start] == -1) continue;
+					int first = 0, len = 0, pos = start;
+					while (pos != -1)
 					{
-						outp[0] = o;
-						best = 0;
+						first = pos;
+						++len;
+						const int nxt = bt[pos];
+						bt[pos] = -1;
+						pos = nxt;
+					}
+					Proposal o;
+					o.curBegin = hc[first];
+					o.curEnd = hc[start] + k - 1;
+					o.extBegin = he[first];
+					o.extEnd = he[start] + k - 1;
+					o.score = sc[start] - sc[first] + k - 1;
+					o.chainLength = len;
+					o.div = 0.0f;
+					if (!overlapTestDev(o, qd.curId, gd.extId, qd.curLen, gd.extLen, p)) continue;
+					const int fc = upperBound(filt, filtN, o.curEnd) - lowerBound(filt, filtN, o.curBegin);
+					const float normLen = static_cast<float>(max(o.curEnd - o.curBegin, o.extEnd - o.extBegin) - fc);
+					if (!(normLen > 0.0f)) continue;
+					float rate = static_cast<float>(len) * p.sample_rate / normLen;
+					rate = fminf(rate, 1.0f);
+					if (!(rate > 0.0f)) continue;
+					const float inv = 1.0f / rate;
+					const float lg = flyeLogf(inv, a.logX, a.logY, a.logN);
+					o.div = lg / static_cast<float>(k);
+					if (p.only_max_ext)
+					{
+						if (best < 0 || o.score > outp[0].score)
+						{
+							outp[0] = o;
+							best = 0;
+						}
+					}
+					else
+					{
+						props[np++] = o;
 					}
 				}
-				else
-				{
-					props[np++] = o;
-				}
 			}
+			waveSync();
+		}
+		if (lane == 0)
+		{
 			red[0] = np;
 			red[1] = (p.only_max_ext && best >= 0 && outp[0].div < p.max_divergence) ? 1 : 0;
 		}
@@ -425,11 +450,11 @@ __global__ void __launch_bounds__(FH_WAVE) chainFinishKernel(ChainArgs a)
 			const int np = red[0];
 			for (int i = lane; i < np; i += FH_WAVE)
 			{
-				sk[i] = static_cast<uint64_t>(static_cast<uint32_t>(0x7fffffff - props[i].score));
+				sk[i] = static_cast<uint32_t>(0x7fffffff - props[i].score);
 				sv[i] = static_cast<uint32_t>(i);
 			}
 			waveSync();
-			waveStdSort<kTile, kLaneMax>(exsort::Span{sk, sv}, np, posL, posR, lds);
+			waveStdSort<uint32_t, kTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, np, posL, posR, lds);
 			if (lane == 0)
 			{
 				int32_t* prim = posL;  // free again after the sort

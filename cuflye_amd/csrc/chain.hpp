@@ -54,7 +54,7 @@ struct ChainArgs
 	const int32_t* ext;
 	const int32_t* filtered;   // per-query ascending filtered positions
 	// per-hit scratch (indexed like the hits)
-	uint64_t* sortKey;
+	uint32_t* sortKey;         // 32-bit sort keys (extPos, INT_MAX - score)
 	uint32_t* sortVal;
 	int32_t* posL;
 	int32_t* posR;

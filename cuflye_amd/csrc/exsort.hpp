@@ -31,8 +31,9 @@
 //    kr < (number of ge elements left of j).  Both need only prefix counts,
 //    so a partition runs as scans over all lanes.
 //
-// Keys are uint64 (every sort of the path maps its comparator onto an
-// ascending uint64 key); values are uint32 payloads moved with their keys.
+// Keys are unsigned integers (uint64 or uint32: every sort of the path maps
+// its comparator onto an ascending unsigned key); values are uint32 payloads
+// moved with their keys.
 
 #pragma once
 
@@ -63,13 +64,14 @@ EXS_HD int floorLog2(uint32_t n)
 EXS_HD int depthLimit(uint32_t n) { return 2 * floorLog2(n); }
 
 // Array view over separate key/value arrays (global or LDS memory).
+template <class K = uint64_t>
 struct Span
 {
-	uint64_t* k;
+	K* k;
 	uint32_t* v;
 	EXS_HD void swap(int a, int b) const
 	{
-		uint64_t tk = k[a];
+		K tk = k[a];
 		k[a] = k[b];
 		k[b] = tk;
 		uint32_t tv = v[a];
@@ -79,9 +81,10 @@ struct Span
 };
 
 // move_median_to_first(result, a, b, c)
-EXS_HD void medianToFirst(const Span& s, int result, int a, int b, int c)
+template <class K>
+EXS_HD void medianToFirst(const Span<K>& s, int result, int a, int b, int c)
 {
-	const uint64_t ka = s.k[a], kb = s.k[b], kc = s.k[c];
+	const K ka = s.k[a], kb = s.k[b], kc = s.k[c];
 	if (ka < kb)
 	{
 		if (kb < kc) s.swap(result, b);
@@ -94,9 +97,10 @@ EXS_HD void medianToFirst(const Span& s, int result, int a, int b, int c)
 }
 
 // Hoare partition of [first, last) around the value at `pivot` (unguarded).
-EXS_HD int unguardedPartition(const Span& s, int first, int last, int pivot)
+template <class K>
+EXS_HD int unguardedPartition(const Span<K>& s, int first, int last, int pivot)
 {
-	const uint64_t p = s.k[pivot];
+	const K p = s.k[pivot];
 	for (;;)
 	{
 		while (s.k[first] < p) ++first;
@@ -108,7 +112,8 @@ EXS_HD int unguardedPartition(const Span& s, int first, int last, int pivot)
 	}
 }
 
-EXS_HD int partitionPivot(const Span& s, int first, int last)
+template <class K>
+EXS_HD int partitionPivot(const Span<K>& s, int first, int last)
 {
 	int mid = first + (last - first) / 2;
 	medianToFirst(s, first, first + 1, mid, last - 1);
@@ -116,7 +121,8 @@ EXS_HD int partitionPivot(const Span& s, int first, int last)
 }
 
 // __push_heap: sift `value` up from holeIndex toward topIndex.
-EXS_HD void pushHeap(const Span& s, int base, int hole, int top, uint64_t vk, uint32_t vv)
+template <class K>
+EXS_HD void pushHeap(const Span<K>& s, int base, int hole, int top, K vk, uint32_t vv)
 {
 	int parent = (hole - 1) / 2;
 	while (hole > top && s.k[base + parent] < vk)
@@ -131,7 +137,8 @@ EXS_HD void pushHeap(const Span& s, int base, int hole, int top, uint64_t vk, ui
 }
 
 // __adjust_heap over heap [base, base + len).
-EXS_HD void adjustHeap(const Span& s, int base, int hole, int len, uint64_t vk, uint32_t vv)
+template <class K>
+EXS_HD void adjustHeap(const Span<K>& s, int base, int hole, int len, K vk, uint32_t vv)
 {
 	const int top = hole;
 	int child = hole;
@@ -154,13 +161,14 @@ EXS_HD void adjustHeap(const Span& s, int base, int hole, int len, uint64_t vk, 
 }
 
 // partial_sort(first, last, last): make_heap then sort_heap.
-EXS_HD void heapSort(const Span& s, int first, int last)
+template <class K>
+EXS_HD void heapSort(const Span<K>& s, int first, int last)
 {
 	const int len = last - first;
 	if (len < 2) return;
 	for (int parent = (len - 2) / 2;; --parent)
 	{
-		uint64_t vk = s.k[first + parent];
+		K vk = s.k[first + parent];
 		uint32_t vv = s.v[first + parent];
 		adjustHeap(s, first, parent, len, vk, vv);
 		if (parent == 0) break;
@@ -168,7 +176,7 @@ EXS_HD void heapSort(const Span& s, int first, int last)
 	for (int end = last; end - first > 1;)
 	{
 		--end;
-		uint64_t vk = s.k[end];
+		K vk = s.k[end];
 		uint32_t vv = s.v[end];
 		s.k[end] = s.k[first];
 		s.v[end] = s.v[first];
@@ -177,11 +185,12 @@ EXS_HD void heapSort(const Span& s, int first, int last)
 }
 
 // Stable insertion sort of [first, last): the final pass of std::sort.
-EXS_HD void insertionSort(const Span& s, int first, int last)
+template <class K>
+EXS_HD void insertionSort(const Span<K>& s, int first, int last)
 {
 	for (int i = first + 1; i < last; ++i)
 	{
-		uint64_t vk = s.k[i];
+		K vk = s.k[i];
 		uint32_t vv = s.v[i];
 		int j = i;
 		while (j > first && vk < s.k[j - 1])
@@ -201,8 +210,8 @@ EXS_HD void insertionSort(const Span& s, int first, int last)
 // so the order they are processed in does not change the result.  Here the
 // smaller part is processed first and the larger one waits on an explicit
 // stack, which bounds the stack by log2(n) + 1 entries.
-template <int kStack = 40>
-EXS_HD void introLoop(const Span& s, int first, int last, int depth)
+template <int kStack = 40, class K = uint64_t>
+EXS_HD void introLoop(const Span<K>& s, int first, int last, int depth)
 {
 	int stF[kStack];
 	int stL[kStack];
@@ -248,7 +257,8 @@ EXS_HD void introLoop(const Span& s, int first, int last, int depth)
 }
 
 // Whole std::sort on [first, last): introsort loop + final insertion sort.
-EXS_HD void stdSort(const Span& s, int first, int last)
+template <class K>
+EXS_HD void stdSort(const Span<K>& s, int first, int last)
 {
 	if (last - first < 2) return;
 	introLoop(s, first, last, depthLimit(static_cast<uint32_t>(last - first)));
@@ -259,10 +269,11 @@ EXS_HD void stdSort(const Span& s, int first, int last)
 // loop; the GPU runs this formulation with scans over many lanes.  Host
 // reference implementation (sequential), used by the tests to check the
 // formulation against unguardedPartition.
-inline int partitionByRanks(const Span& s, int first, int last, int* posL, int* posR)
+template <class K>
+inline int partitionByRanks(const Span<K>& s, int first, int last, int* posL, int* posR)
 {
 	// caller has already moved the median to `first`
-	const uint64_t p = s.k[first];
+	const K p = s.k[first];
 	const int lo = first + 1;
 	int leTotal = 0;
 	for (int i = lo; i < last; ++i) leTotal += !(p < s.k[i]);

@@ -31,10 +31,10 @@ struct WSeg
 	int d;
 };
 
-template <int kTile>
+template <class K, int kTile>
 struct WaveSortLds
 {
-	uint64_t k[kTile];
+	K k[kTile];
 	uint32_t v[kTile];
 	int posL[kTile / 2 + 1];
 	int posR[kTile / 2 + 1];
@@ -56,12 +56,13 @@ __device__ inline unsigned long long laneMaskLt()
 
 // Partition s[f, l) around the median moved to f; all 64 lanes call it.
 // posL / posR need (l - f) / 2 + 1 entries.
-__device__ inline int wavePartition(const exsort::Span& s, int f, int l, int* posL, int* posR)
+template <class K>
+__device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int* posL, int* posR)
 {
 	const int lane = laneId();
 	if (lane == 0) exsort::medianToFirst(s, f, f + 1, f + (l - f) / 2, l - 1);
 	waveSync();
-	const uint64_t p = s.k[f];
+	const K p = s.k[f];
 	const int lo = f + 1;
 	int LE = 0;
 	for (int base = lo; base < l; base += FH_WAVE)
@@ -79,7 +80,7 @@ __device__ inline int wavePartition(const exsort::Span& s, int f, int l, int* po
 		bool ge = false, le = false;
 		if (i < l)
 		{
-			const uint64_t v = s.k[i];
+			const K v = s.k[i];
 			ge = !(v < p);
 			le = !(p < v);
 		}
@@ -112,8 +113,8 @@ __device__ inline int wavePartition(const exsort::Span& s, int f, int l, int* po
 
 // Sorts the range g[f, l) (l - f <= kTile) with remaining depth budget d,
 // through the wave's LDS tile; ends with the stable insertion pass.
-template <int kTile, int kLaneMax>
-__device__ void waveTileSort(const exsort::Span& g, int f, int l, int d, WaveSortLds<kTile>& t)
+template <class K, int kTile, int kLaneMax>
+__device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, WaveSortLds<K, kTile>& t)
 {
 	const int lane = laneId();
 	const int n = l - f;
@@ -123,7 +124,7 @@ __device__ void waveTileSort(const exsort::Span& g, int f, int l, int d, WaveSor
 		t.v[i] = g.v[f + i];
 	}
 	waveSync();
-	const exsort::Span s{t.k, t.v};
+	const exsort::Span<K> s{t.k, t.v};
 	int top = 0;
 	int nl = 0;
 	if (lane == 0) t.stack[0] = WSeg{0, n, d};
@@ -175,13 +176,13 @@ __device__ void waveTileSort(const exsort::Span& g, int f, int l, int d, WaveSor
 
 // Exact std::sort of g[0, n) by one wave.  posL/posR: global scratch of
 // n / 2 + 1 ints each (only touched when n > kTile).
-template <int kTile, int kLaneMax>
-__device__ void waveStdSort(const exsort::Span& g, int n, int* posL, int* posR, WaveSortLds<kTile>& t)
+template <class K, int kTile, int kLaneMax>
+__device__ void waveStdSort(const exsort::Span<K>& g, int n, int* posL, int* posR, WaveSortLds<K, kTile>& t)
 {
 	if (n < 2) return;
 	if (n <= kTile)
 	{
-		waveTileSort<kTile, kLaneMax>(g, 0, n, exsort::depthLimit(static_cast<uint32_t>(n)), t);
+		waveTileSort<K, kTile, kLaneMax>(g, 0, n, exsort::depthLimit(static_cast<uint32_t>(n)), t);
 		return;
 	}
 	// global ranges: stack kept in registers of every lane (uniform values)
@@ -193,7 +194,7 @@ __device__ void waveStdSort(const exsort::Span& g, int n, int* posL, int* posR, 
 		const WSeg w = st[--top];
 		if (w.l - w.f <= kTile)
 		{
-			waveTileSort<kTile, kLaneMax>(g, w.f, w.l, w.d, t);
+			waveTileSort<K, kTile, kLaneMax>(g, w.f, w.l, w.d, t);
 			continue;
 		}
 		if (w.d == 0)

@@ -1,8 +1,8 @@
 // pipeline.hip - gfx950 kernels of the k-mer stage of getSeqOverlaps:
 // the VertexIndex build (count in an open-addressed HBM hash table, histogram,
 // thresholds applied, bucket fill in (read, position) order), the query k-mer
-// join in Flye's emission order (window count, scan, fill), the per-query exact
-// std::sort of the matches, the group split, divStats, and the device scan.
+// join in Flye's emission order (window count, scan, fill), the group split,
+// divStats, and the device scan.
 //
 // Data layout in HBM (all sizes per element):
 //   packed reads   2 bits/base, 16 bases per uint32 word, each read word aligned
@@ -12,7 +12,7 @@
 //   join windows   count i32 + filtered i32 + slot/flip u32 per query window
 //   matches        key u64 (internal extId << 32 | curPos) + extPos i32
 #include "pipeline.hpp"
-#include "exsort_wave.hpp"
+#include "dev_common.hpp"
 
 namespace fh
 {
@@ -498,34 +498,6 @@ __global__ void writeCandidatesKernel(JoinArgs a, const int64_t* qMatchOff, cons
 
 // ---- per-query match sort -------------------------------------------------------
 
-constexpr int kMatchTile = 1024;
-
-// One wave per query, queries pulled from a work queue; the wave runs the
-// exact std::sort of the query's matches by (internal extId, curPos).
-__global__ void __launch_bounds__(FH_WAVE) matchSortKernel(MatchSortArgs a)
-{
-	__shared__ WaveSortLds<kMatchTile> lds;
-	const int lane = laneId();
-	for (;;)
-	{
-		int q = 0;
-		if (lane == 0) q = atomicAdd(a.queue, 1);
-		q = __shfl(q, 0, FH_WAVE);
-		if (q >= a.nQueries) break;
-		const int64_t b = a.qMatchOff[q];
-		const int n = static_cast<int>(a.qMatchOff[q + 1] - b);
-		uint64_t* key = a.key + b;
-		uint32_t* val = a.val + b;
-		for (int i = lane; i < n; i += FH_WAVE) val[i] = static_cast<uint32_t>(i);
-		waveSync();
-		waveStdSort<kMatchTile, 48>(exsort::Span{key, val}, n, a.posL + b, a.posR + b, lds);
-		int t = 0;
-		for (int i = lane + 1; i < n; i += FH_WAVE) t |= (key[i] == key[i - 1]);
-		if (__ballot(t) && lane == 0) atomicAdd(a.tieQueries, 1);
-		waveSync();
-	}
-}
-
 __global__ void gatherHitsKernel(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const uint32_t* val,
 								 const int32_t* mext, int32_t* hcur, int32_t* hext, int32_t* groupFlag, int64_t nMatches)
 {
@@ -680,17 +652,6 @@ hipError_t launchWriteCandidates(const JoinArgs& a, const int64_t* qMatchOff, co
 	if (nMatches == 0) return hipSuccess;
 	hipLaunchKernelGGL(writeCandidatesKernel, gridFor(nMatches, 256), dim3(256), 0, s, a, qMatchOff, queryIds, perm,
 					   out, nMatches);
-	return hipGetLastError();
-}
-
-hipError_t launchMatchSort(const MatchSortArgs& a, hipStream_t s)
-{
-	if (a.nQueries == 0) return hipSuccess;
-	int dev = 0, cus = 0;
-	hipGetDevice(&dev);
-	hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-	const int blocks = min(a.nQueries, cus * 8);
-	hipLaunchKernelGGL(matchSortKernel, dim3(static_cast<unsigned>(blocks)), dim3(FH_WAVE), 0, s, a);
 	return hipGetLastError();
 }
 

@@ -91,20 +91,7 @@ hipError_t launchWriteCandidates(const JoinArgs& a, const int64_t* qMatchOff, co
 								 const uint32_t* perm, flyehip_candidate_t* out, int64_t nMatches,
 								 hipStream_t s);
 
-// ---- per-query exact match sort and grouping ----------------------------------
-struct MatchSortArgs
-{
-	const int64_t* qMatchOff;  // per query match offsets (nQueries + 1)
-	int32_t nQueries;
-	uint64_t* key;             // sorted in place
-	uint32_t* val;             // permutation (emission index within query)
-	int32_t* posL;             // partition scratch, one int per match
-	int32_t* posR;
-	int32_t* queue;            // work-queue counter
-	int32_t* tieQueries;       // counter: queries with equal (extId, curPos) keys
-};
-hipError_t launchMatchSort(const MatchSortArgs& a, hipStream_t s);
-
+// ---- grouping after the exact match sort (msort.hip) --------------------------
 // After the sort: hit arrays, group-start flags.
 hipError_t launchGatherHits(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const uint32_t* val,
 							const int32_t* mext, int32_t* hcur, int32_t* hext, int32_t* groupFlag,

@@ -51,7 +51,7 @@ __global__ void readAlignChainKernel(ReadAlignArgs a)
 			++m;
 		}
 	}
-	exsort::stdSort(exsort::Span{key, val}, 0, m);
+	exsort::stdSort(exsort::Span<uint64_t>{key, val}, 0, m);
 	for (int i = 0; i < m; ++i) kept[i] = static_cast<int32_t>(val[i]);
 	a.keptCount[r] = m;
 
@@ -129,7 +129,7 @@ __global__ void readAlignChainKernel(ReadAlignArgs a)
 		key[nAct + c] = static_cast<uint64_t>(static_cast<int64_t>(0x7fffffff) - cs[frz[c]]);
 		val[nAct + c] = static_cast<uint32_t>(frz[c]);
 	}
-	exsort::stdSort(exsort::Span{key, val}, 0, nAll);
+	exsort::stdSort(exsort::Span<uint64_t>{key, val}, 0, nAll);
 	// acceptance: first alignment of a chain is found by walking back
 	int nAcc = 0;
 	int segs = 0;

@@ -24,6 +24,7 @@
 
 #include "chain.hpp"
 #include "dev_common.hpp"
+#include "msort.hpp"
 #include "pipeline.hpp"
 #include "readalign.hpp"
 
@@ -120,7 +121,7 @@ struct flyehip_ctx
 	fh::DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bHCur, bHExt, bGroupFlag, bGroupOff;
 	fh::DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
 	fh::DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
-	fh::DevBuf bGroupPass, bOrder, bBins;
+	fh::DevBuf bGroupPass, bOrder, bBins, bSKey, bAVal, bTied, bFromA, bQTied, bMsortTemp;
 	fh::DevBuf bQueue, bCounters, bQRowCount, bCurLen, bDivOff, bDivCount, bDivOut, bQueryIds, bCand;
 	fh::DevBuf bExtWords, bExtWordOff, bExtLen;
 	fh::DevBuf bAln, bAlnOff, bRA;
@@ -442,20 +443,39 @@ BatchOut runJoin(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQ
 	return bo;
 }
 
+// Exact per-query std::sort of the batch's matches; the sorted keys land in
+// bSKey and the permutation (emission index within the query) in bVal.
 void runMatchSort(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches)
 {
-	MatchSortArgs ms;
-	ms.qMatchOff = static_cast<const int64_t*>(c->bQMatchOff.p);
-	ms.nQueries = ja.nQueries;
-	ms.key = ja.mkey;
-	ms.val = ensureT<uint32_t>(c, c->bVal, static_cast<size_t>(nMatches) + 1);
-	ms.posL = ensureT<int32_t>(c, c->bPosL, static_cast<size_t>(nMatches) + 1);
-	ms.posR = ensureT<int32_t>(c, c->bPosR, static_cast<size_t>(nMatches) + 1);
-	ms.tieQueries = ensureT<int32_t>(c, c->bCounters, 16);
-	ms.queue = ms.tieQueries + 1;
-	zero32(c, ms.tieQueries, 16);
+	const size_t nm = static_cast<size_t>(nMatches) + 1;
+	const int32_t nq = ja.nQueries;
+	MsortArgs ms{};
+	ms.qOff = static_cast<const int64_t*>(c->bQMatchOff.p);
+	ms.nQueries = nq;
+	ms.order = nullptr;
+	ms.akey = ja.mkey;
+	ms.aval = ensureT<uint32_t>(c, c->bAVal, nm);
+	ms.skey = ensureT<uint64_t>(c, c->bSKey, nm);
+	ms.sval = ensureT<uint32_t>(c, c->bVal, nm);
+	ms.posL = ensureT<int32_t>(c, c->bPosL, nm);
+	ms.posR = ensureT<int32_t>(c, c->bPosR, nm);
+	ms.tiedByIdx = ensureT<uint8_t>(c, c->bTied, nm);
+	ms.fromA = ensureT<uint8_t>(c, c->bFromA, nm);
+	ms.qTied = ensureT<int32_t>(c, c->bQTied, static_cast<size_t>(nq) + 1);
+	int32_t* counters = ensureT<int32_t>(c, c->bCounters, 16);
+	ms.tieQueries = counters;
+	ms.queue = counters + 1;
+	zero32(c, counters, 16);
+	zero32(c, ms.qTied, nq + 1);
+	HIPCHECK(hipMemsetAsync(ms.fromA, 0, nm, c->stream));
+	int bits = 1;
+	while ((int64_t{1} << bits) < 2 * static_cast<int64_t>(std::max(1, c->nReads))) ++bits;
+	const int endBit = std::min(64, 32 + bits);
+	size_t tempBytes = 0;
+	HIPCHECK(msortTempBytes(nMatches, nq, endBit, &tempBytes));
+	void* temp = ensure(c, c->bMsortTemp, tempBytes + 256);
 	HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-	HIPCHECK(launchMatchSort(ms, c->stream));
+	HIPCHECK(launchMatchSortExact(ms, nMatches, endBit, temp, tempBytes, chainBlocks(c) / 2, c->stream));
 	HIPCHECK(hipEventRecord(c->ev[1], c->stream));
 	HIPCHECK(hipEventSynchronize(c->ev[1]));
 	float es = 0;
@@ -515,14 +535,15 @@ BatchOut runChain(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches, const st
 	int64_t* goff = ensureT<int64_t>(c, c->bGroupOff, static_cast<size_t>(nMatches) + 1);
 	int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(nMatches) + 64);
 	const int64_t* qMatchOff = static_cast<const int64_t*>(c->bQMatchOff.p);
-	HIPCHECK(launchGatherHits(qMatchOff, nq, ja.mkey, static_cast<const uint32_t*>(c->bVal.p), ja.mext, hcur, hext, gflag,
+	const uint64_t* skey = static_cast<const uint64_t*>(c->bSKey.p);
+	HIPCHECK(launchGatherHits(qMatchOff, nq, skey, static_cast<const uint32_t*>(c->bVal.p), ja.mext, hcur, hext, gflag,
 							  nMatches, c->stream));
 	HIPCHECK(exclusiveScan(gflag, goff, nMatches, tmp, c->stream));
 	int64_t nGroups = 0;
 	download(c, &nGroups, goff + nMatches, sizeof(int64_t));
 	sync(c);
 	GroupDesc* groups = ensureT<GroupDesc>(c, c->bGroups, static_cast<size_t>(nGroups) + 1);
-	HIPCHECK(launchBuildGroups(qMatchOff, nq, ja.mkey, gflag, goff, ja.targetLen, groups, nMatches, c->stream));
+	HIPCHECK(launchBuildGroups(qMatchOff, nq, skey, gflag, goff, ja.targetLen, groups, nMatches, c->stream));
 	HIPCHECK(launchGroupSizes(groups, nGroups, nMatches, c->stream));
 	// query descriptors
 	int64_t* dIds = ensureT<int64_t>(c, c->bQueryIds, static_cast<size_t>(nq) + 1);
@@ -549,7 +570,7 @@ BatchOut runChain(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches, const st
 	ca.ext = hext;
 	ca.filtered = ja.filtered;
 	const size_t nm = static_cast<size_t>(nMatches) + 1;
-	ca.sortKey = ensureT<uint64_t>(c, c->bCKey, nm);
+	ca.sortKey = ensureT<uint32_t>(c, c->bCKey, nm);
 	ca.sortVal = ensureT<uint32_t>(c, c->bCVal, nm);
 	ca.posL = ensureT<int32_t>(c, c->bCPosL, nm);
 	ca.posR = ensureT<int32_t>(c, c->bCPosR, nm);
@@ -786,7 +807,7 @@ int flyehip_close(flyehip_ctx* c)
 					 &c->bPosL, &c->bPosR, &c->bHCur, &c->bHExt, &c->bGroupFlag, &c->bGroupOff,
 					 &c->bFiltered, &c->bQMatchOff, &c->bQFiltOff, &c->bQDesc, &c->bGroups, &c->bGroupCount,
 					 &c->bRowStart, &c->bRows, &c->bCScore, &c->bCBt, &c->bCProps, &c->bCOut, &c->bCHCur,
-					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bGroupPass, &c->bOrder, &c->bBins, &c->bQueue,
+					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bGroupPass, &c->bOrder, &c->bBins, &c->bSKey, &c->bAVal, &c->bTied, &c->bFromA, &c->bQTied, &c->bMsortTemp, &c->bQueue,
 					 &c->bCounters, &c->bQRowCount, &c->bCurLen, &c->bDivOff, &c->bDivCount, &c->bDivOut,
 					 &c->bQueryIds, &c->bCand, &c->bExtWords, &c->bExtWordOff, &c->bExtLen, &c->bAln, &c->bAlnOff,
 					 &c->bRA};
@@ -971,7 +992,9 @@ int flyehip_collect_sorted_candidates(flyehip_ctx* c, const int64_t* query_ids, 
 		int64_t* dIds = ensureT<int64_t>(c, c->bQueryIds, static_cast<size_t>(n) + 1);
 		upload(c, dIds, query_ids, static_cast<size_t>(n) * 8);
 		flyehip_candidate_t* dc = ensureT<flyehip_candidate_t>(c, c->bCand, static_cast<size_t>(bo.matches) + 1);
-		HIPCHECK(launchWriteCandidates(ja, static_cast<const int64_t*>(c->bQMatchOff.p), dIds,
+		JoinArgs sorted = ja;
+		sorted.mkey = static_cast<uint64_t*>(c->bSKey.p);
+		HIPCHECK(launchWriteCandidates(sorted, static_cast<const int64_t*>(c->bQMatchOff.p), dIds,
 									   static_cast<const uint32_t*>(c->bVal.p), dc, bo.matches, c->stream));
 		download(c, matches, dc, static_cast<size_t>(bo.matches) * sizeof(flyehip_candidate_t));
 		sync(c);
@@ -1085,7 +1108,7 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 		ca.cur = dc;
 		ca.ext = de;
 		ca.filtered = dFilt;
-		ca.sortKey = ensureT<uint64_t>(c, c->bCKey, nm);
+		ca.sortKey = ensureT<uint32_t>(c, c->bCKey, nm);
 		ca.sortVal = ensureT<uint32_t>(c, c->bCVal, nm);
 		ca.posL = ensureT<int32_t>(c, c->bCPosL, nm);
 		ca.posR = ensureT<int32_t>(c, c->bCPosR, nm);

@@ -22,12 +22,29 @@ static std::vector<uint32_t> refPerm(const std::vector<uint64_t>& keys)
 	return idx;
 }
 
+static void check32(const std::vector<uint64_t>& keys, const char* what)
+{
+	std::vector<uint32_t> k(keys.size());
+	for (size_t i = 0; i < k.size(); ++i) k[i] = static_cast<uint32_t>(keys[i]);
+	std::vector<uint32_t> v(keys.size());
+	for (size_t i = 0; i < v.size(); ++i) v[i] = static_cast<uint32_t>(i);
+	exsort::stdSort(exsort::Span<uint32_t>{k.data(), v.data()}, 0, static_cast<int>(k.size()));
+	std::vector<uint64_t> trunc(keys.size());
+	for (size_t i = 0; i < k.size(); ++i) trunc[i] = static_cast<uint32_t>(keys[i]);
+	if (v != refPerm(trunc))
+	{
+		if (failures < 10) std::printf("MISMATCH32 %s n=%zu\n", what, keys.size());
+		++failures;
+	}
+}
+
 static void check(const std::vector<uint64_t>& keys, const char* what)
 {
+	check32(keys, what);
 	std::vector<uint64_t> k = keys;
 	std::vector<uint32_t> v(keys.size());
 	for (size_t i = 0; i < v.size(); ++i) v[i] = static_cast<uint32_t>(i);
-	exsort::stdSort(exsort::Span{k.data(), v.data()}, 0, static_cast<int>(k.size()));
+	exsort::stdSort(exsort::Span<uint64_t>{k.data(), v.data()}, 0, static_cast<int>(k.size()));
 	if (v != refPerm(keys))
 	{
 		if (failures < 10) std::printf("MISMATCH %s n=%zu\n", what, keys.size());
@@ -43,7 +60,7 @@ static void checkPartition(const std::vector<uint64_t>& keys)
 	std::vector<uint64_t> k1 = keys, k2 = keys;
 	std::vector<uint32_t> v1(n), v2(n);
 	for (int i = 0; i < n; ++i) v1[i] = v2[i] = static_cast<uint32_t>(i);
-	exsort::Span s1{k1.data(), v1.data()}, s2{k2.data(), v2.data()};
+	exsort::Span<uint64_t> s1{k1.data(), v1.data()}, s2{k2.data(), v2.data()};
 	int c1 = exsort::partitionPivot(s1, 0, n);
 	exsort::medianToFirst(s2, 0, 1, n / 2, n - 1);
 	std::vector<int> pl(n), pr(n);
