@@ -29,7 +29,7 @@ namespace
 
 constexpr int kTile = 2048;  // elements of a wave's LDS sort tile (32-bit keys)
 constexpr int kLaneMax = 48;
-constexpr int kRingTotal = 4096;  // LDS score-ring entries per wave, split over groups
+constexpr int kRingTotal = 2048;  // LDS ring entries per wave (two rings), split over groups
 
 __device__ inline float flyeLogf(float x, const uint32_t* tabX, const uint32_t* tabY, int tabN)
 {
@@ -197,7 +197,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 {
 	constexpr int kSubs = FH_WAVE / G;
 	constexpr int kRing = kRingTotal / kSubs;
-	__shared__ int ring[kSubs][kRing];
+	__shared__ int ring[kSubs][kRing];   // score[j] of the last kRing rows
+	__shared__ int ringM[kSubs][kRing];  // max(score[0..j]) of the same rows
 	const int lane = laneId();
 	const int sub = lane / G;
 	const int sl = lane % G;
@@ -206,6 +207,14 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 	const int32_t maxJump = p.maximum_jump;
 	const unsigned long long subMask = (G == 64 ? ~0ULL : ((1ULL << G) - 1)) << (sub * G);
 	int* rg = ring[sub];
+	int* rm = ringM[sub];
+	// A predecessor j contributes score[j] + min(dc, de, k) - gapCost, which is
+	// at most score[j] + k when the gap penalties are non-negative.  Once
+	// max(score[0..j]) + k cannot beat the running maximum, no deeper
+	// predecessor can improve it, so the serial scan's result is final and
+	// the remaining lookback is skipped (it could neither change the maximum
+	// nor trigger the improvement break).
+	const bool bounded = p.large_gap_penalty >= 0.0f && p.small_gap_penalty >= 0.0f;
 
 	// per-subgroup state (uniform inside a subgroup)
 	int g = -1;
@@ -216,6 +225,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 	const int32_t* he = nullptr;
 	int32_t* sc = nullptr;
 	int32_t* bt = nullptr;
+	int32_t* pmG = nullptr;
+	int pm = 0;
 	bool done = false;
 	int iter = 0;
 	for (;;)
@@ -244,11 +255,15 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			he = a.hext + gd.hitStart;
 			sc = a.score + gd.hitStart;
 			bt = a.bt + gd.hitStart;
+			pmG = a.posR + gd.hitStart;  // free during the DP: holds max(score[0..j])
+			pm = 0;
 			if (sl == 0)
 			{
 				sc[0] = 0;
 				bt[0] = -1;
+				pmG[0] = 0;
 				rg[0] = 0;
+				rm[0] = 0;
 			}
 			need = false;
 		}
@@ -307,13 +322,22 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 					carryId = tileBase - l0;
 				}
 				if (bm) break;
+				const int jn = tileBase - G;
+				if (bounded && jn >= 0)
+				{
+					const int mj = (i - jn < kRing) ? rm[jn & (kRing - 1)] : pmG[jn];
+					if (mj + k <= carry) break;
+				}
 			}
 			const int32_t si = max(carry, k);
+			pm = max(pm, si);
 			if (sl == 0)
 			{
 				sc[i] = si;
 				bt[i] = carry > k ? carryId : -1;
+				pmG[i] = pm;
 				rg[i & (kRing - 1)] = si;
+				rm[i & (kRing - 1)] = pm;
 			}
 			++i;
 		}
