@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "lib"
 OBJ = LIB / "obj"
 
-HIP_SOURCES = ["pipeline.hip", "msort.hip", "chain.hip", "readalign.hip", "runtime.hip"]
+HIP_SOURCES = ["pipeline.hip", "msort.hip", "segsort.hip", "chain.hip", "readalign.hip", "runtime.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -31,6 +31,11 @@ __device__ inline int waveSum(int v)
 	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, FH_WAVE);
 	return v;
 }
+__device__ inline long long waveSum64(long long v)
+{
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, FH_WAVE);
+	return v;
+}
 __device__ inline int waveMax(int v)
 {
 	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, FH_WAVE));

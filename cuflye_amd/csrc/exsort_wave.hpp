@@ -112,9 +112,12 @@ __device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int*
 }
 
 // Sorts the range g[f, l) (l - f <= kTile) with remaining depth budget d,
-// through the wave's LDS tile; ends with the stable insertion pass.
+// through the wave's LDS tile; ends with the stable insertion pass.  The
+// sorted range goes back to g, or, with outV set, only its values go to
+// outV[f, l) (the keys are then left as they were).
 template <class K, int kTile, int kLaneMax>
-__device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, WaveSortLds<K, kTile>& t)
+__device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, WaveSortLds<K, kTile>& t,
+							 uint32_t* outV = nullptr)
 {
 	const int lane = laneId();
 	const int n = l - f;
@@ -166,10 +169,17 @@ __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, Wave
 		waveSync();
 	}
 	if (nl) flush();
-	for (int i = lane; i < n; i += FH_WAVE)
+	if (outV)
 	{
-		g.k[f + i] = t.k[i];
-		g.v[f + i] = t.v[i];
+		for (int i = lane; i < n; i += FH_WAVE) outV[f + i] = t.v[i];
+	}
+	else
+	{
+		for (int i = lane; i < n; i += FH_WAVE)
+		{
+			g.k[f + i] = t.k[i];
+			g.v[f + i] = t.v[i];
+		}
 	}
 	waveSync();
 }

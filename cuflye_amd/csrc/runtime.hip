@@ -27,6 +27,7 @@
 #include "msort.hpp"
 #include "pipeline.hpp"
 #include "readalign.hpp"
+#include "segsort.hpp"
 
 extern "C" const unsigned char fh_logf_table[];
 extern "C" const unsigned char fh_logf_table_end[];
@@ -121,7 +122,10 @@ struct flyehip_ctx
 	fh::DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bHCur, bHExt, bGroupFlag, bGroupOff;
 	fh::DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
 	fh::DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
-	fh::DevBuf bGroupPass, bOrder, bBins, bSKey, bAVal, bTied, bFromA, bQTied, bMsortTemp;
+	fh::DevBuf bGroupPass, bOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
+	fh::DevBuf bSegBig0, bSegBig1, bSegLeaf, bSegHeap, bSegCounters, bSegChunkOff, bSegChunkRange, bSegChunkCnt;
+	fh::DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp;
+	int64_t* hostSeg = nullptr;  // pinned counter words of the segmented sort
 	fh::DevBuf bQueue, bCounters, bQRowCount, bCurLen, bDivOff, bDivCount, bDivOut, bQueryIds, bCand;
 	fh::DevBuf bExtWords, bExtWordOff, bExtLen;
 	fh::DevBuf bAln, bAlnOff, bRA;
@@ -445,45 +449,84 @@ BatchOut runJoin(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQ
 
 // Exact per-query std::sort of the batch's matches; the sorted keys land in
 // bSKey and the permutation (emission index within the query) in bVal.
+// Buffers of the segmented exact sort for n elements in nArrays arrays; the
+// caller sets key / val / outVal / dup / posL / posR.
+SegSortBufs segBufs(flyehip_ctx* c, int64_t n, int64_t nArrays)
+{
+	SegSortCaps caps{};
+	HIPCHECK(segSortCaps(n, nArrays, &caps));
+	SegSortBufs b{};
+	b.big[0] = ensureT<SegRange>(c, c->bSegBig0, static_cast<size_t>(caps.bigCap));
+	b.big[1] = ensureT<SegRange>(c, c->bSegBig1, static_cast<size_t>(caps.bigCap));
+	b.leaf = ensureT<SegRange>(c, c->bSegLeaf, static_cast<size_t>(caps.leafCap));
+	b.heap = ensureT<SegRange>(c, c->bSegHeap, static_cast<size_t>(caps.heapCap));
+	b.counters = ensureT<int64_t>(c, c->bSegCounters, kSegCounters);
+	b.chunkOff = ensureT<int64_t>(c, c->bSegChunkOff, static_cast<size_t>(caps.bigCap) + 1);
+	b.chunkRange = ensureT<int32_t>(c, c->bSegChunkRange, static_cast<size_t>(caps.chunkCap));
+	b.chunkCnt = ensureT<uint64_t>(c, c->bSegChunkCnt, static_cast<size_t>(caps.chunkCap) + 1);
+	b.chunkPre = ensureT<uint64_t>(c, c->bSegChunkPre, static_cast<size_t>(std::max(caps.chunkCap, caps.bigCap)) + 1);
+	b.rangeM = ensureT<int32_t>(c, c->bSegRangeM, static_cast<size_t>(caps.bigCap));
+	b.rangeLm = ensureT<int32_t>(c, c->bSegRangeLm, static_cast<size_t>(caps.bigCap));
+	b.scanTemp = ensure(c, c->bSegTemp, caps.scanTempBytes + 256);
+	b.scanTempBytes = caps.scanTempBytes;
+	b.bigCap = caps.bigCap;
+	b.leafCap = caps.leafCap;
+	b.heapCap = caps.heapCap;
+	b.chunkCap = caps.chunkCap;
+	b.hostCounters = c->hostSeg;
+	return b;
+}
+
+void runSegSort(flyehip_ctx* c, SegSortBufs& b, int64_t* tieArrays)
+{
+	const hipError_t e = segSortRun(b, c->stream, tieArrays);
+	if (e == hipErrorIllegalState) fail(FLYEHIP_ERR_HIP, "segmented sort: inconsistent partition");
+	HIPCHECK(e);
+}
+
+// Exact std::sort of every query's matches: stable order, then the tie replay.
+// Leaves the sorted keys in bSKey and the permutation (emission index within
+// the query) in bVal.
 void runMatchSort(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches)
 {
+	if (nMatches >= 0x7fffffffLL) fail(FLYEHIP_ERR_BUDGET, "batch of %lld matches exceeds 2^31 - 1", static_cast<long long>(nMatches));
 	const size_t nm = static_cast<size_t>(nMatches) + 1;
 	const int32_t nq = ja.nQueries;
 	MsortArgs ms{};
 	ms.qOff = static_cast<const int64_t*>(c->bQMatchOff.p);
 	ms.nQueries = nq;
-	ms.order = nullptr;
 	ms.akey = ja.mkey;
 	ms.aval = ensureT<uint32_t>(c, c->bAVal, nm);
 	ms.skey = ensureT<uint64_t>(c, c->bSKey, nm);
 	ms.sval = ensureT<uint32_t>(c, c->bVal, nm);
-	ms.posL = ensureT<int32_t>(c, c->bPosL, nm);
-	ms.posR = ensureT<int32_t>(c, c->bPosR, nm);
-	ms.tiedByIdx = ensureT<uint8_t>(c, c->bTied, nm);
-	ms.fromA = ensureT<uint8_t>(c, c->bFromA, nm);
-	ms.qTied = ensureT<int32_t>(c, c->bQTied, static_cast<size_t>(nq) + 1);
-	int32_t* counters = ensureT<int32_t>(c, c->bCounters, 16);
-	ms.tieQueries = counters;
-	ms.queue = counters + 1;
-	zero32(c, counters, 16);
-	zero32(c, ms.qTied, nq + 1);
-	HIPCHECK(hipMemsetAsync(ms.fromA, 0, nm, c->stream));
+	ms.rkey = ensureT<uint32_t>(c, c->bRKey, nm);
+	ms.dupFlag = ensureT<int32_t>(c, c->bDupFlag, nm);
+	int64_t* dup = ensureT<int64_t>(c, c->bDup, nm);
+	int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(nMatches) + 64);
 	int bits = 1;
 	while ((int64_t{1} << bits) < 2 * static_cast<int64_t>(std::max(1, c->nReads))) ++bits;
-	const int endBit = std::min(64, 32 + bits);
 	size_t tempBytes = 0;
-	HIPCHECK(msortTempBytes(nMatches, nq, endBit, &tempBytes));
+	HIPCHECK(msortTempBytes(nMatches, nq, bits, &tempBytes));
 	void* temp = ensure(c, c->bMsortTemp, tempBytes + 256);
+	SegSortBufs sb = segBufs(c, nMatches, nq);
+	sb.key = ms.rkey;
+	sb.val = ms.aval;
+	sb.outVal = ms.sval;
+	sb.dup = dup;
+	sb.posL = ensureT<int32_t>(c, c->bPosL, nm);
+	sb.posR = ensureT<int32_t>(c, c->bPosR, nm);
 	HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-	HIPCHECK(launchMatchSortExact(ms, nMatches, endBit, temp, tempBytes, chainBlocks(c) / 2, c->stream));
+	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, temp, tempBytes, c->stream));
+	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, c->stream));
+	HIPCHECK(segSortReset(sb, c->stream));
+	HIPCHECK(segPushOffsets(sb, ms.qOff, nq, c->stream));
+	int64_t ties = 0;
+	runSegSort(c, sb, &ties);
 	HIPCHECK(hipEventRecord(c->ev[1], c->stream));
 	HIPCHECK(hipEventSynchronize(c->ev[1]));
 	float es = 0;
 	HIPCHECK(hipEventElapsedTime(&es, c->ev[0], c->ev[1]));
 	c->stats.ms_sort += es;
-	int32_t ties = 0;
-	download(c, &ties, ms.tieQueries, sizeof(int32_t));
-	sync(c);
 	c->stats.tie_queries += ties;
 }
 
@@ -782,6 +825,7 @@ int flyehip_open(int device, uint64_t memory_budget_bytes, flyehip_ctx** out_ctx
 		c->budget = memory_budget_bytes ? memory_budget_bytes : static_cast<uint64_t>(static_cast<double>(freeB) * 0.9);
 		HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 		for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
+		HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&c->hostSeg), fh::kSegCounters * sizeof(int64_t)));
 		loadLogTable(c);
 	}
 	catch (const FhError& e)
@@ -807,11 +851,14 @@ int flyehip_close(flyehip_ctx* c)
 					 &c->bPosL, &c->bPosR, &c->bHCur, &c->bHExt, &c->bGroupFlag, &c->bGroupOff,
 					 &c->bFiltered, &c->bQMatchOff, &c->bQFiltOff, &c->bQDesc, &c->bGroups, &c->bGroupCount,
 					 &c->bRowStart, &c->bRows, &c->bCScore, &c->bCBt, &c->bCProps, &c->bCOut, &c->bCHCur,
-					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bGroupPass, &c->bOrder, &c->bBins, &c->bSKey, &c->bAVal, &c->bTied, &c->bFromA, &c->bQTied, &c->bMsortTemp, &c->bQueue,
+					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bGroupPass, &c->bOrder, &c->bBins, &c->bSKey, &c->bAVal, &c->bRKey, &c->bDupFlag, &c->bDup, &c->bMsortTemp, &c->bQueue,
+					 &c->bSegBig0, &c->bSegBig1, &c->bSegLeaf, &c->bSegHeap, &c->bSegCounters, &c->bSegChunkOff,
+					 &c->bSegChunkRange, &c->bSegChunkCnt, &c->bSegChunkPre, &c->bSegRangeM, &c->bSegRangeLm, &c->bSegTemp,
 					 &c->bCounters, &c->bQRowCount, &c->bCurLen, &c->bDivOff, &c->bDivCount, &c->bDivOut,
 					 &c->bQueryIds, &c->bCand, &c->bExtWords, &c->bExtWordOff, &c->bExtLen, &c->bAln, &c->bAlnOff,
 					 &c->bRA};
 	for (DevBuf* b : all) release(c, *b);
+	if (c->hostSeg) hipHostFree(c->hostSeg);
 	for (auto& e : c->ev)
 		if (e) hipEventDestroy(e);
 	if (c->stream) hipStreamDestroy(c->stream);

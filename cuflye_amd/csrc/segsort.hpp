@@ -1,0 +1,164 @@
+// segsort.hpp - exact std::sort of many arrays at once (segsort.hip).
+//
+// Every array of a batch (a query's k-mer matches, a target group's hits) is
+// sorted exactly as libstdc++'s std::sort orders it (exsort.hpp).  Introsort
+// partitions disjoint ranges independently, so the ranges of all arrays are
+// processed level by level instead of array by array:
+//   * a "big" range (more than kSegTile elements) is partitioned by many
+//     waves at once: its elements are cut into chunks of kSegChunk, the
+//     ge / le counts of every chunk are scanned over the whole level, and the
+//     swap partners come from the scanned ranks (exsort.hpp's rank form of the
+//     Hoare partition), so one level costs a few bandwidth-bound passes over
+//     the data of all arrays, not a chain of dependent steps per array;
+//   * a range of at most kSegTile elements ("leaf") is finished by one wave in
+//     LDS with its remaining depth budget (waveTileSort);
+//   * a big range whose depth budget is spent is heap-sorted (one lane; only
+//     adversarial inputs get there).
+// Optionally a range is dropped when the sorted order already known for it is
+// final: with `dup` set (exclusive prefix counts over the stable sorted order
+// of the positions whose key is shared with a neighbour, "tied" positions), a
+// range [f, l) without a tied position holds keys that occur nowhere else in
+// the array, so its content and order are those of the stable order.  (An
+// equal pair inside the range is not the right test: a tie group split by a
+// partition leaves single members in two ranges.)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "exsort.hpp"
+#include "exsort_wave.hpp"
+
+namespace fh
+{
+
+constexpr int kSegTile = 2048;        // leaf size, elements (LDS tile of one wave)
+constexpr int kSegRows = 8;           // 64-element rows per chunk
+constexpr int kSegChunk = 64 * kSegRows;
+
+struct SegRange
+{
+	int32_t f;  // absolute positions in the key / value arrays
+	int32_t l;
+	int32_t d;  // remaining depth budget
+	int32_t pad;
+};
+
+// Counter words (int64) shared by the init kernels and the rounds.
+enum SegCounter
+{
+	kSegNBig = 0,     // ranges in the list being filled
+	kSegBigChunks,    // their chunk total
+	kSegNLeaf,
+	kSegNHeap,
+	kSegTieArrays,    // arrays with at least one tied position (pruned sorts)
+	kSegError,        // set when a partition produced an impossible cut or swap
+	kSegCounters = 8
+};
+
+struct SegSortBufs
+{
+	uint32_t* key;          // sorted in place (big ranges)
+	uint32_t* val;
+	uint32_t* outVal;       // leaves and heap ranges write their final values here
+	const int64_t* dup;     // tied-position prefix counts, n + 1 words (null: no pruning)
+	int32_t* posL;          // one int per element
+	int32_t* posR;
+	SegRange* big[2];       // ping-pong lists of big ranges
+	SegRange* leaf;
+	SegRange* heap;
+	int64_t* counters;      // kSegCounters words
+	int64_t* chunkOff;      // per big range of a round: chunk offsets (bigCap + 1)
+	int32_t* chunkRange;    // per chunk (chunkCap)
+	uint64_t* chunkCnt;     // per chunk: (ge << 32) | le  (chunkCap + 1)
+	uint64_t* chunkPre;     // exclusive scan of chunkCnt (max(chunkCap, bigCap) + 1)
+	int32_t* rangeM;        // swaps per range (bigCap)
+	int32_t* rangeLm;       // first ge element that is not swapped (bigCap)
+	void* scanTemp;
+	size_t scanTempBytes;
+	int64_t bigCap;
+	int64_t leafCap;
+	int64_t heapCap;
+	int64_t chunkCap;
+	int64_t* hostCounters;  // pinned, kSegCounters words
+};
+
+struct SegSortCaps
+{
+	int64_t bigCap;
+	int64_t leafCap;
+	int64_t heapCap;
+	int64_t chunkCap;
+	size_t scanTempBytes;
+};
+
+// Capacities for arrays totalling n elements in nArrays arrays.
+hipError_t segSortCaps(int64_t n, int64_t nArrays, SegSortCaps* caps);
+
+// Appends a range of [f, l) with depth d to the lists: leaf, heap or big
+// (bigList).  Every lane of the wave calls it; lanes with push == false add
+// nothing.  Ranges of fewer than 2 elements, and ranges pruned by `dup`, are
+// dropped.  One atomic per list and wave.
+__device__ inline void segPushWave(const SegSortBufs& b, SegRange* bigList, bool push, int32_t f, int32_t l, int32_t d)
+{
+	const int32_t n = l - f;
+	if (n < 2) push = false;
+	if (push && b.dup && b.dup[l] - b.dup[f] == 0) push = false;
+	const bool toLeaf = push && n <= kSegTile;
+	const bool toHeap = push && n > kSegTile && d == 0;
+	const bool toBig = push && n > kSegTile && d > 0;
+	const unsigned long long lt = laneMaskLt();
+	const int lane = laneId();
+	const SegRange r{f, l, d, 0};
+	unsigned long long m = __ballot(toLeaf);
+	if (m)
+	{
+		const int leader = __ffsll(static_cast<long long>(m)) - 1;
+		unsigned long long base = 0;
+		if (lane == leader)
+			base = atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegNLeaf]),
+							 static_cast<unsigned long long>(__popcll(m)));
+		base = __shfl(base, leader, FH_WAVE);
+		if (toLeaf) b.leaf[base + __popcll(m & lt)] = r;
+	}
+	m = __ballot(toHeap);
+	if (m)
+	{
+		const int leader = __ffsll(static_cast<long long>(m)) - 1;
+		unsigned long long base = 0;
+		if (lane == leader)
+			base = atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegNHeap]),
+							 static_cast<unsigned long long>(__popcll(m)));
+		base = __shfl(base, leader, FH_WAVE);
+		if (toHeap) b.heap[base + __popcll(m & lt)] = r;
+	}
+	m = __ballot(toBig);
+	if (m)
+	{
+		const int leader = __ffsll(static_cast<long long>(m)) - 1;
+		const int64_t chunks = toBig ? (static_cast<int64_t>(n) - 1 + kSegChunk - 1) / kSegChunk : 0;
+		const int64_t chunkSum = waveSum64(chunks);
+		unsigned long long base = 0;
+		if (lane == leader)
+		{
+			base = atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegNBig]),
+							 static_cast<unsigned long long>(__popcll(m)));
+			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegBigChunks]),
+					  static_cast<unsigned long long>(chunkSum));
+		}
+		base = __shfl(base, leader, FH_WAVE);
+		if (toBig) bigList[base + __popcll(m & lt)] = r;
+	}
+}
+
+// Zeroes the counters (before the init kernels push the first ranges).
+hipError_t segSortReset(const SegSortBufs& b, hipStream_t s);
+
+// Runs the rounds and the leaf / heap passes over what the init kernels
+// pushed.  Synchronises the stream once per round (range counts).
+hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays);
+
+// Pushes every array [off[i], off[i+1]) (i < nArrays), pruned by b.dup.
+hipError_t segPushOffsets(const SegSortBufs& b, const int64_t* off, int32_t nArrays, hipStream_t s);
+
+} // namespace fh

@@ -116,6 +116,25 @@ def test_overlaps_config2_small(gpu, oracle_mod):
     assert_rows_equal(rows, erows, "config2 read-to-graph shape")
 
 
+def test_candidates_sorted_tie_order_ont(gpu, oracle_mod):
+    # ONT-like errors and repeat families: long match lists (tens of
+    # thousands per query, partitioned across many waves) with equal
+    # (extId, curPos) keys throughout
+    r = S.config2(scale=0.03, seed=24)
+    orc = oracle_mod.Oracle(r.bases, r.offsets)
+    orc.build_index(15)
+    gpu.store_reads(r.bases, r.offsets)
+    gpu.build_index(15)
+    ids = both_strands(min(r.n_reads, 60))
+    m, moff = gpu.collect_sorted_candidates(ids)
+    st = gpu.stats()
+    assert st["tie_queries"] > 0
+    assert int(np.max(np.diff(moff))) > 4096
+    for i, q in enumerate(ids):
+        em, _ = orc.candidates(int(q), sorted_=True)
+        assert_rows_equal(m[moff[i]:moff[i + 1]], em, f"sorted q={q}")
+
+
 def test_chain_lists_config3_small(gpu, oracle_mod):
     h = S.config3(n_lists=4000, seed=23, max_hits=20000)
     print(f"[c3] lists={len(h.offsets) - 1} hits={len(h.cur)}", flush=True)
