@@ -20,6 +20,7 @@
 //           backtrack (lane 0, O(n)), overlapTest, divergence, selection.
 #include "chain.hpp"
 #include "exsort_wave.hpp"
+#include "segsort.hpp"
 
 namespace fh
 {
@@ -27,7 +28,6 @@ namespace fh
 namespace
 {
 
-constexpr int kTile = 2048;  // elements of a wave's LDS sort tile (32-bit keys)
 constexpr int kLaneMax = 48;
 constexpr int kRingTotal = 2048;  // LDS ring entries per wave (two rings), split over groups
 
@@ -106,71 +106,140 @@ __device__ inline int pullGroup(int32_t* queue)
 
 // ---- prep ------------------------------------------------------------------
 
-__global__ void __launch_bounds__(FH_WAVE) chainPrepKernel(ChainArgs a)
+// Prefilters (replay_overlap_chains.py:245-270).  Each wave takes 64
+// consecutive groups of the processing order: a group whose size is below the
+// unique-curPos bound (unique curPos <= n) fails without reading its hits; the
+// others are examined by the whole wave one after another.  Passing groups
+// with extLen > curLen get their extPos sort keys and are pushed to the
+// segmented sort.
+__global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, SegSortBufs sb)
 {
-	__shared__ WaveSortLds<uint32_t, kTile> lds;
 	const int lane = laneId();
 	const flyehip_overlap_params_t& p = a.params;
+	const float minUniq = p.min_kmer_survival_rate * static_cast<float>(p.minimum_overlap);
+	for (;;)
+	{
+		int base = 0;
+		if (lane == 0) base = atomicAdd(a.queue, FH_WAVE);
+		base = __shfl(base, 0, FH_WAVE);
+		if (base >= a.nGroups) break;
+		const int qi = base + lane;
+		int myG = -1;
+		bool cand = false;
+		if (qi < a.nGroups)
+		{
+			myG = a.order[qi];
+			const int n = a.groups[myG].n;
+			cand = n >= 2 && !(static_cast<float>(n) < minUniq);
+			if (!cand)
+			{
+				a.groupPass[myG] = 0;
+				a.groupCount[myG] = 0;
+			}
+		}
+		unsigned long long todo = __ballot(cand);
+		while (todo)
+		{
+			const int l = __ffsll(static_cast<long long>(todo)) - 1;
+			todo &= todo - 1;
+			const int g = __shfl(myG, l, FH_WAVE);
+			const GroupDesc gd = a.groups[g];
+			const QueryDesc qd = a.queries[gd.query];
+			const int n = gd.n;
+			const int64_t hb = gd.hitStart;
+			const int32_t* cur0 = a.cur + hb;
+			const int32_t* ext0 = a.ext + hb;
+			// unique curPos counted from prevPos = 0 on the (extId, curPos) order
+			int uniq = 0;
+			int mnE = 0x7fffffff, mxE = -0x7fffffff - 1;
+			for (int i = lane; i < n; i += FH_WAVE)
+			{
+				const int32_t c = cur0[i];
+				const int32_t prev = i == 0 ? 0 : cur0[i - 1];
+				uniq += (c != prev);
+				const int32_t e = ext0[i];
+				mnE = min(mnE, e);
+				mxE = max(mxE, e);
+			}
+			uniq = waveSum(uniq);
+			mnE = waveMin(mnE);
+			mxE = waveMax(mxE);
+			const int32_t mnC = cur0[0];
+			const int32_t mxC = cur0[n - 1];
+			bool pass = !(static_cast<float>(uniq) < minUniq);
+			if (pass && (mxC - mnC < p.minimum_overlap || mxE - mnE < p.minimum_overlap)) pass = false;
+			if (pass && p.check_overhang && !p.force_local)
+			{
+				if (min(mnC, mnE) > p.maximum_overhang) pass = false;
+				if (min(qd.curLen - mxC, gd.extLen - mxE) > p.maximum_overhang) pass = false;
+			}
+			const bool extSorted = pass && gd.extLen > qd.curLen;
+			if (extSorted)
+			{
+				uint32_t* sk = a.sortKey + hb;
+				uint32_t* sv = a.sortVal + hb;
+				for (int i = lane; i < n; i += FH_WAVE)
+				{
+					sk[i] = static_cast<uint32_t>(ext0[i]);
+					sv[i] = static_cast<uint32_t>(i);
+				}
+			}
+			if (lane == 0)
+			{
+				a.groupPass[g] = pass ? 1 : 0;
+				a.groupCount[g] = 0;
+			}
+			segPushWave(sb, sb.big[0], lane == 0 && extSorted, static_cast<int32_t>(hb), static_cast<int32_t>(hb + n),
+						exsort::depthLimit(static_cast<uint32_t>(n)), 0);
+		}
+	}
+}
+
+// Passing groups in processing order (stable compaction), and how many of
+// them hold at least kWalkSmall hits (a prefix: the order is size-descending
+// by floor(log2 n)).
+__global__ void passFlagKernel(const int32_t* order, const int32_t* groupPass, int32_t nGroups, int32_t* flag)
+{
+	const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+	if (qi < nGroups) flag[qi] = groupPass[order[qi]];
+}
+
+__global__ void passScatterKernel(const int32_t* order, const int32_t* flag, const int64_t* pos, int32_t nGroups,
+								  const GroupDesc* groups, int32_t bigMin, int32_t* passOrder, int64_t* nBig)
+{
+	const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+	bool big = false;
+	if (qi < nGroups && flag[qi])
+	{
+		const int g = order[qi];
+		passOrder[pos[qi]] = g;
+		big = groups[g].n >= bigMin;
+	}
+	const int c = __popcll(__ballot(big));
+	if (laneId() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(nBig), static_cast<unsigned long long>(c));
+}
+
+// Group-order hit copies for the DP: hits in (extId, curPos) order, or in the
+// extPos order of the sort when extLen > curLen.
+__global__ void __launch_bounds__(FH_WAVE) chainGatherKernel(ChainArgs a)
+{
+	const int lane = laneId();
 	for (;;)
 	{
 		const int qi = pullGroup(a.queue);
-		if (qi >= a.nGroups) break;
-		const int g = a.order ? a.order[qi] : qi;
+		if (qi >= a.nPass) break;
+		const int g = a.passOrder[qi];
 		const GroupDesc gd = a.groups[g];
 		const QueryDesc qd = a.queries[gd.query];
 		const int n = gd.n;
-		const int64_t base = gd.hitStart;
-		const int32_t* cur0 = a.cur + base;
-		const int32_t* ext0 = a.ext + base;
-
-		// prefilters (replay_overlap_chains.py:245-270); unique curPos counted
-		// from prevPos = 0 on the (extId, curPos) order
-		int uniq = 0;
-		int mnE = 0x7fffffff, mxE = -0x7fffffff - 1;
-		for (int i = lane; i < n; i += FH_WAVE)
+		const int64_t hb = gd.hitStart;
+		const int32_t* cur0 = a.cur + hb;
+		const int32_t* ext0 = a.ext + hb;
+		int32_t* hc = a.hcur + hb;
+		int32_t* he = a.hext + hb;
+		if (gd.extLen > qd.curLen)
 		{
-			const int32_t c = cur0[i];
-			const int32_t prev = i == 0 ? 0 : cur0[i - 1];
-			uniq += (c != prev);
-			const int32_t e = ext0[i];
-			mnE = min(mnE, e);
-			mxE = max(mxE, e);
-		}
-		uniq = waveSum(uniq);
-		mnE = waveMin(mnE);
-		mxE = waveMax(mxE);
-		const int32_t mnC = cur0[0];
-		const int32_t mxC = cur0[n - 1];
-		bool pass = !(static_cast<float>(uniq) < p.min_kmer_survival_rate * static_cast<float>(p.minimum_overlap));
-		if (pass && (mxC - mnC < p.minimum_overlap || mxE - mnE < p.minimum_overlap)) pass = false;
-		if (pass && p.check_overhang && !p.force_local)
-		{
-			if (min(mnC, mnE) > p.maximum_overhang) pass = false;
-			if (min(qd.curLen - mxC, gd.extLen - mxE) > p.maximum_overhang) pass = false;
-		}
-		if (n < 2) pass = false;
-		if (lane == 0)
-		{
-			a.groupPass[g] = pass ? 1 : 0;
-			a.groupCount[g] = 0;
-		}
-		if (!pass) continue;
-
-		int32_t* hc = a.hcur + base;
-		int32_t* he = a.hext + base;
-		const bool extSorted = gd.extLen > qd.curLen;
-		if (extSorted)
-		{
-			uint32_t* sk = a.sortKey + base;
-			uint32_t* sv = a.sortVal + base;
-			for (int i = lane; i < n; i += FH_WAVE)
-			{
-				sk[i] = static_cast<uint32_t>(ext0[i]);
-				sv[i] = static_cast<uint32_t>(i);
-			}
-			waveSync();
-			waveStdSort<uint32_t, kTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, n, a.posL + base, a.posR + base,
-												  lds);
+			const uint32_t* sv = a.sortVal + hb;
 			for (int i = lane; i < n; i += FH_WAVE)
 			{
 				const uint32_t src = sv[i];
@@ -186,7 +255,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrepKernel(ChainArgs a)
 				he[i] = ext0[i];
 			}
 		}
-		waveSync();
 	}
 }
 
@@ -239,13 +307,12 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			int qi = 0;
 			if (sl == 0) qi = atomicAdd(a.queue, 1);
 			qi = __shfl(qi, sub * G, FH_WAVE);
-			if (qi >= a.nGroups)
+			if (qi >= a.nPass)
 			{
 				done = true;
 				break;
 			}
-			const int gg = a.order ? a.order[qi] : qi;
-			if (!a.groupPass[gg]) continue;
+			const int gg = a.passOrder[qi];
 			const GroupDesc gd = a.groups[gg];
 			g = gg;
 			n = gd.n;
@@ -354,134 +421,190 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 
 // ---- finish ----------------------------------------------------------------------
 
-__global__ void __launch_bounds__(FH_WAVE) chainFinishKernel(ChainArgs a)
+// Score-order sort keys of every passing group (std::sort(indices,
+// score[a] > score[b]) as an ascending key), pushed to the segmented sort.
+__global__ void __launch_bounds__(FH_WAVE) chainScoreKeysKernel(ChainArgs a, SegSortBufs sb)
 {
-	__shared__ WaveSortLds<uint32_t, kTile> lds;
-	__shared__ int btS[kTile];
+	const int lane = laneId();
+	for (;;)
+	{
+		const int qi = pullGroup(a.queue);
+		if (qi >= a.nPass) break;
+		const GroupDesc gd = a.groups[a.passOrder[qi]];
+		const int n = gd.n;
+		const int64_t hb = gd.hitStart;
+		const int32_t* sc = a.score + hb;
+		uint32_t* sk = a.sortKey + hb;
+		uint32_t* sv = a.sortVal + hb;
+		for (int i = lane; i < n; i += FH_WAVE)
+		{
+			sk[i] = static_cast<uint32_t>(0x7fffffff - sc[i]);
+			sv[i] = static_cast<uint32_t>(i);
+		}
+		segPushWave(sb, sb.big[0], lane == 0, static_cast<int32_t>(hb), static_cast<int32_t>(hb + n),
+					exsort::depthLimit(static_cast<uint32_t>(n)), 0);
+	}
+}
+
+constexpr int kPropTile = 256;  // proposals sorted in LDS up to this count
+
+// Backtrack, overlapTest, divergence and selection of one group per wave,
+// groups passOrder[first, last).  The rows are visited in score order 64 at
+// a time: the lanes test which starts still have a backtrack pointer, lane 0
+// walks those in order through the LDS copy of the pointers (a walk clears
+// the pointers it passes, so lane 0 re-checks each start) and records
+// (start, first, length); then every lane evaluates one recorded walk
+// (positions, overlapTest, the filtered-position count, logf) and the
+// surviving proposals are appended in walk order.
+template <int kBt>
+__global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t first, int32_t last)
+{
+	__shared__ int btS[kBt];
+	__shared__ int recStart[FH_WAVE];
+	__shared__ int recFirst[FH_WAVE];
+	__shared__ int recLen[FH_WAVE];
 	__shared__ int red[2];
+	__shared__ WaveSortLds<uint32_t, kPropTile> lds;
 	const int lane = laneId();
 	const flyehip_overlap_params_t& p = a.params;
 	const int32_t k = p.kmer_size;
 	for (;;)
 	{
-		const int qi = pullGroup(a.queue);
-		if (qi >= a.nGroups) break;
-		const int g = a.order ? a.order[qi] : qi;
-		if (!a.groupPass[g]) continue;
+		const int qi = first + pullGroup(a.queue);
+		if (qi >= last) break;
+		const int g = a.passOrder[qi];
 		const GroupDesc gd = a.groups[g];
 		const QueryDesc qd = a.queries[gd.query];
 		const int n = gd.n;
-		const int64_t base = gd.hitStart;
-		const int32_t* hc = a.hcur + base;
-		const int32_t* he = a.hext + base;
-		const int32_t* sc = a.score + base;
-		uint32_t* sk = a.sortKey + base;
-		uint32_t* sv = a.sortVal + base;
-		int32_t* posL = a.posL + base;
-		int32_t* posR = a.posR + base;
-		// backtrack pointers in LDS when the group fits, else in place
-		const bool inLds = n <= kTile;
-		int32_t* bt = inLds ? btS : a.bt + base;
-		const uint32_t* order = inLds ? lds.v : sv;
-
-		// score order: std::sort(indices, score[a] > score[b])
-		for (int i = lane; i < n; i += FH_WAVE)
-		{
-			sk[i] = static_cast<uint32_t>(0x7fffffff - sc[i]);
-			sv[i] = static_cast<uint32_t>(i);
-			if (inLds) btS[i] = a.bt[base + i];
-		}
+		const int64_t hb = gd.hitStart;
+		const int32_t* hc = a.hcur + hb;
+		const int32_t* he = a.hext + hb;
+		const int32_t* sc = a.score + hb;
+		uint32_t* sk = a.sortKey + hb;
+		uint32_t* sv = a.sortVal + hb;
+		const bool inLds = n <= kBt;
+		int32_t* bt = inLds ? btS : a.bt + hb;
+		if (inLds)
+			for (int i = lane; i < n; i += FH_WAVE) btS[i] = a.bt[hb + i];
 		waveSync();
-		waveStdSort<uint32_t, kTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, n, posL, posR, lds);
 
-		// backtrack, overlapTest, divergence.  Chunks of 64 rows in score order:
-		// the lanes test which starts still have a backtrack pointer, lane 0
-		// walks those in order (a walk clears the pointers it passes, so lane 0
-		// re-checks each start before walking it).
-		Proposal* props = a.props + base;
-		Proposal* outp = a.outProps + base;
+		Proposal* props = a.props + hb;
+		Proposal* outp = a.outProps + hb;
 		const int32_t* filt = a.filtered + qd.filtStart;
 		const int filtN = qd.filtCount;
 		int np = 0;
-		int best = -1;
+		int bestScore = 0;
+		bool haveBest = false;
 		for (int q0 = 0; q0 < n; q0 += FH_WAVE)
 		{
 			const int q = q0 + lane;
-			const int myStart = q < n ? static_cast<int>(order[q]) : 0;
+			const int myStart = q < n ? static_cast<int>(sv[q]) : 0;
 			const bool live = q < n && bt[myStart] != -1;
-			unsigned long long m = __ballot(live);
+			const unsigned long long m = __ballot(live);
+			recStart[lane] = myStart;
 			waveSync();
 			if (lane == 0)
 			{
-				while (m)
+				int nrec = 0;
+				unsigned long long mm = m;
+				while (mm)
 				{
-					const int l = __ffsll(static_cast<long long>(m)) - 1;
-					m &= m - 1;
-					const int start = static_cast<int>(order[q0 + l]);
+					const int l = __ffsll(static_cast<long long>(mm)) - 1;
+					mm &= mm - 1;
+					const int start = recStart[l];
 					if (bt[start] == -1) continue;
-					int first = 0, len = 0, pos = start;
+					int f = 0, len = 0, pos = start;
 					while (pos != -1)
 					{
-						first = pos;
+						f = pos;
 						++len;
 						const int nxt = bt[pos];
 						bt[pos] = -1;
 						pos = nxt;
 					}
-					Proposal o;
-					o.curBegin = hc[first];
-					o.curEnd = hc[start] + k - 1;
-					o.extBegin = he[first];
-					o.extEnd = he[start] + k - 1;
-					o.score = sc[start] - sc[first] + k - 1;
-					o.chainLength = len;
-					o.div = 0.0f;
-					if (!overlapTestDev(o, qd.curId, gd.extId, qd.curLen, gd.extLen, p)) continue;
+					recFirst[nrec] = f;
+					recLen[nrec] = len;
+					recStart[nrec] = start;  // slots below l are no longer read
+					++nrec;
+				}
+				red[0] = nrec;
+			}
+			waveSync();
+			const int nrec = red[0];
+			bool ok = false;
+			Proposal o;
+			if (lane < nrec)
+			{
+				const int start = recStart[lane];
+				const int f = recFirst[lane];
+				const int len = recLen[lane];
+				o.curBegin = hc[f];
+				o.curEnd = hc[start] + k - 1;
+				o.extBegin = he[f];
+				o.extEnd = he[start] + k - 1;
+				o.score = sc[start] - sc[f] + k - 1;
+				o.chainLength = len;
+				o.div = 0.0f;
+				ok = overlapTestDev(o, qd.curId, gd.extId, qd.curLen, gd.extLen, p);
+				if (ok)
+				{
 					const int fc = upperBound(filt, filtN, o.curEnd) - lowerBound(filt, filtN, o.curBegin);
-					const float normLen = static_cast<float>(max(o.curEnd - o.curBegin, o.extEnd - o.extBegin) - fc);
-					if (!(normLen > 0.0f)) continue;
-					float rate = static_cast<float>(len) * p.sample_rate / normLen;
-					rate = fminf(rate, 1.0f);
-					if (!(rate > 0.0f)) continue;
-					const float inv = 1.0f / rate;
-					const float lg = flyeLogf(inv, a.logX, a.logY, a.logN);
-					o.div = lg / static_cast<float>(k);
-					if (p.only_max_ext)
+					const float normLen =
+						static_cast<float>(max(o.curEnd - o.curBegin, o.extEnd - o.extBegin) - fc);
+					ok = normLen > 0.0f;
+					if (ok)
 					{
-						if (best < 0 || o.score > outp[0].score)
+						float rate = static_cast<float>(len) * p.sample_rate / normLen;
+						rate = fminf(rate, 1.0f);
+						ok = rate > 0.0f;
+						if (ok)
 						{
-							outp[0] = o;
-							best = 0;
+							const float inv = 1.0f / rate;
+							o.div = flyeLogf(inv, a.logX, a.logY, a.logN) / static_cast<float>(k);
 						}
-					}
-					else
-					{
-						props[np++] = o;
 					}
 				}
 			}
+			const unsigned long long okm = __ballot(ok);
+			if (p.only_max_ext)
+			{
+				// first strict maximum in walk order
+				const int mx = waveMax(ok ? o.score : (-0x7fffffff - 1));
+				if (okm && (!haveBest || mx > bestScore))
+				{
+					const int l = __ffsll(static_cast<long long>(__ballot(ok && o.score == mx))) - 1;
+					if (lane == l) outp[0] = o;
+					bestScore = mx;
+					haveBest = true;
+				}
+			}
+			else if (ok)
+			{
+				props[np + __popcll(okm & laneMaskLt())] = o;
+			}
+			np += __popcll(okm);
 			waveSync();
 		}
-		if (lane == 0)
+		int count = 0;
+		if (p.only_max_ext)
 		{
-			red[0] = np;
-			red[1] = (p.only_max_ext && best >= 0 && outp[0].div < p.max_divergence) ? 1 : 0;
+			waveSync();
+			count = (haveBest && outp[0].div < p.max_divergence) ? 1 : 0;
 		}
-		waveSync();
-		if (!p.only_max_ext)
+		else
 		{
 			// std::sort(extOverlaps, score desc), containment filter, divergence filter
-			const int np = red[0];
 			for (int i = lane; i < np; i += FH_WAVE)
 			{
 				sk[i] = static_cast<uint32_t>(0x7fffffff - props[i].score);
 				sv[i] = static_cast<uint32_t>(i);
 			}
 			waveSync();
-			waveStdSort<uint32_t, kTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, np, posL, posR, lds);
+			waveStdSort<uint32_t, kPropTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, np, a.posL + hb, a.posR + hb,
+													   lds);
 			if (lane == 0)
 			{
-				int32_t* prim = posL;  // free again after the sort
+				int32_t* prim = a.posL + hb;  // free again after the sort
 				int nprim = 0;
 				for (int q = 0; q < np; ++q)
 				{
@@ -505,8 +628,9 @@ __global__ void __launch_bounds__(FH_WAVE) chainFinishKernel(ChainArgs a)
 				red[1] = w;
 			}
 			waveSync();
+			count = red[1];
 		}
-		if (lane == 0) a.groupCount[g] = red[1];
+		if (lane == 0) a.groupCount[g] = count;
 		waveSync();
 	}
 }
@@ -544,23 +668,36 @@ __global__ void writeRowsKernel(ChainArgs a, const int64_t* rowStart, flyehip_ov
 }
 
 // Size-descending processing order: groups binned by floor(log2 n) (the order
-// inside a bin is irrelevant to the results).
+// inside a bin is irrelevant to the results).  Block-local LDS histograms keep
+// the global atomics to one per bin and block.
 __global__ void binCountKernel(const GroupDesc* groups, int32_t nGroups, int32_t* binCount)
 {
+	__shared__ int h[33];
+	if (threadIdx.x < 33) h[threadIdx.x] = 0;
+	__syncthreads();
 	const int g = blockIdx.x * blockDim.x + threadIdx.x;
-	if (g >= nGroups) return;
-	const int n = max(1, groups[g].n);
-	const int bin = __clz(n);  // 31 - floor(log2 n): big groups first
-	atomicAdd(&binCount[bin], 1);
+	if (g < nGroups) atomicAdd(&h[__clz(max(1, groups[g].n))], 1);  // 31 - floor(log2 n): big groups first
+	__syncthreads();
+	if (threadIdx.x < 33 && h[threadIdx.x]) atomicAdd(&binCount[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void binScatterKernel(const GroupDesc* groups, int32_t nGroups, int32_t* binCursor, int32_t* order)
 {
+	__shared__ int h[33];
+	__shared__ int base[33];
+	if (threadIdx.x < 33) h[threadIdx.x] = 0;
+	__syncthreads();
 	const int g = blockIdx.x * blockDim.x + threadIdx.x;
-	if (g >= nGroups) return;
-	const int n = max(1, groups[g].n);
-	const int bin = __clz(n);
-	order[atomicAdd(&binCursor[bin], 1)] = g;
+	int bin = -1, local = 0;
+	if (g < nGroups)
+	{
+		bin = __clz(max(1, groups[g].n));
+		local = atomicAdd(&h[bin], 1);
+	}
+	__syncthreads();
+	if (threadIdx.x < 33 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&binCursor[threadIdx.x], h[threadIdx.x]);
+	__syncthreads();
+	if (bin >= 0) order[base[bin] + local] = g;
 }
 
 __global__ void binPrefixKernel(int32_t* binCount, int32_t* binCursor)
@@ -604,9 +741,38 @@ hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* b
 	return hipGetLastError();
 }
 
-hipError_t launchChainPrep(const ChainArgs& a, int blocks, hipStream_t stream)
+hipError_t launchChainPrefilter(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream)
 {
-	hipLaunchKernelGGL(chainPrepKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a);
+	hipLaunchKernelGGL(chainPrefilterKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a, sb);
+	return hipGetLastError();
+}
+
+hipError_t launchPassFlags(const ChainArgs& a, int32_t* flag, hipStream_t stream)
+{
+	if (a.nGroups == 0) return hipSuccess;
+	hipLaunchKernelGGL(passFlagKernel, dim3((a.nGroups + 255) / 256), dim3(256), 0, stream, a.order, a.groupPass,
+					   a.nGroups, flag);
+	return hipGetLastError();
+}
+
+hipError_t launchPassScatter(const ChainArgs& a, const int32_t* flag, const int64_t* pos, int32_t* passOrder,
+							 int64_t* nBig, hipStream_t stream)
+{
+	if (a.nGroups == 0) return hipSuccess;
+	hipLaunchKernelGGL(passScatterKernel, dim3((a.nGroups + 255) / 256), dim3(256), 0, stream, a.order, flag, pos,
+					   a.nGroups, a.groups, kWalkSmall, passOrder, nBig);
+	return hipGetLastError();
+}
+
+hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream)
+{
+	hipLaunchKernelGGL(chainGatherKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a);
+	return hipGetLastError();
+}
+
+hipError_t launchChainScoreKeys(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream)
+{
+	hipLaunchKernelGGL(chainScoreKeysKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a, sb);
 	return hipGetLastError();
 }
 
@@ -625,9 +791,12 @@ hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t 
 	return hipGetLastError();
 }
 
-hipError_t launchChainFinish(const ChainArgs& a, int blocks, hipStream_t stream)
+hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, bool big, int blocks,
+						   hipStream_t stream)
 {
-	hipLaunchKernelGGL(chainFinishKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a);
+	if (last <= first) return hipSuccess;
+	if (big) hipLaunchKernelGGL(chainWalkKernel<kWalkBig>, dim3(blocks), dim3(FH_WAVE), 0, stream, a, first, last);
+	else hipLaunchKernelGGL(chainWalkKernel<kWalkSmall>, dim3(blocks), dim3(FH_WAVE), 0, stream, a, first, last);
 	return hipGetLastError();
 }
 

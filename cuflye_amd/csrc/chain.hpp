@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "../../include/flyehip.h"
+#include "segsort.hpp"
 
 namespace fh
 {
@@ -47,15 +48,17 @@ struct ChainArgs
 	flyehip_overlap_params_t params;
 	const GroupDesc* groups;
 	const QueryDesc* queries;
-	const int32_t* order;      // processing order (may be null)
+	const int32_t* order;      // processing order: size-descending
 	int32_t nGroups;
+	const int32_t* passOrder;  // groups that pass the prefilters, in processing order
+	int32_t nPass;
 	int32_t logN;
 	const int32_t* cur;        // hits in (extId, curPos) order
 	const int32_t* ext;
 	const int32_t* filtered;   // per-query ascending filtered positions
 	// per-hit scratch (indexed like the hits)
 	uint32_t* sortKey;         // 32-bit sort keys (extPos, INT_MAX - score)
-	uint32_t* sortVal;
+	uint32_t* sortVal;         // index within the group; sorted in place with the keys
 	int32_t* posL;
 	int32_t* posR;
 	int32_t* hcur;             // group-order copies (after the extPos re-sort)
@@ -73,17 +76,30 @@ struct ChainArgs
 	const uint32_t* logY;
 };
 
-// Per-hit scratch bytes the chaining kernels need.
-constexpr size_t kChainScratchPerHit = 8 + 4 + 4 + 4 + 4 + 4 + 4 + 4 + 2 * sizeof(Proposal);
+// Walk kernel variants: backtrack pointers of groups up to kWalkSmall /
+// kWalkBig hits are staged in LDS (larger groups walk them in place).
+constexpr int kWalkSmall = 2048;
+constexpr int kWalkBig = 16384;
 
 // Flye's divergence logf (glibc-exact) for an array of arguments.
 hipError_t launchLogf(const float* x, int64_t n, float* out, const uint32_t* tabX, const uint32_t* tabY, int tabN,
 					  hipStream_t stream);
 hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* bins, int32_t* order,
 							hipStream_t stream);
-hipError_t launchChainPrep(const ChainArgs& a, int blocks, hipStream_t stream);
+// prefilters; pushes the extPos sorts to sb
+hipError_t launchChainPrefilter(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream);
+// passOrder = order filtered by groupPass (flags, scan by the caller, scatter);
+// *nBig counts passing groups of at least kWalkSmall hits
+hipError_t launchPassFlags(const ChainArgs& a, int32_t* flag, hipStream_t stream);
+hipError_t launchPassScatter(const ChainArgs& a, const int32_t* flag, const int64_t* pos, int32_t* passOrder,
+							 int64_t* nBig, hipStream_t stream);
+hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream);
 hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t stream);
-hipError_t launchChainFinish(const ChainArgs& a, int blocks, hipStream_t stream);
+// score-order keys; pushes the score sorts to sb
+hipError_t launchChainScoreKeys(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream);
+// backtrack .. selection for passOrder[first, last)
+hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, bool big, int blocks,
+						   hipStream_t stream);
 hipError_t launchWriteRows(const ChainArgs& a, const int64_t* rowStart, flyehip_overlap_t* rows,
 						   hipStream_t stream);
 

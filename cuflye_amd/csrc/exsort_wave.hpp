@@ -40,6 +40,7 @@ struct WaveSortLds
 	int posR[kTile / 2 + 1];
 	WSeg stack[40];
 	WSeg lanes[FH_WAVE];
+	WSeg gstack[40];  // ranges larger than the tile (waveStdSort)
 };
 
 __device__ inline void waveSync()
@@ -113,11 +114,11 @@ __device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int*
 
 // Sorts the range g[f, l) (l - f <= kTile) with remaining depth budget d,
 // through the wave's LDS tile; ends with the stable insertion pass.  The
-// sorted range goes back to g, or, with outV set, only its values go to
-// outV[f, l) (the keys are then left as they were).
+// sorted range goes back to g, or, with outV set, its values go to outV[f, l)
+// and its keys back to g only when keysBack is set.
 template <class K, int kTile, int kLaneMax>
 __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, WaveSortLds<K, kTile>& t,
-							 uint32_t* outV = nullptr)
+							 uint32_t* outV = nullptr, bool keysBack = false)
 {
 	const int lane = laneId();
 	const int n = l - f;
@@ -138,7 +139,11 @@ __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, Wave
 		if (lane < nl)
 		{
 			const WSeg w = t.lanes[lane];
-			exsort::introLoop<16>(s, w.f, w.l, w.d);
+			// a lane range has at most kLaneMax elements or no depth budget
+			// left (straight to heapsort): smaller part first keeps at most
+			// log2(kLaneMax / 16) + 1 entries on the stack
+			static_assert(kLaneMax <= 16 << 4, "lane stack bound");
+			exsort::introLoop<6>(s, w.f, w.l, w.d);
 			exsort::insertionSort(s, w.f, w.l);
 		}
 		nl = 0;
@@ -171,7 +176,11 @@ __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, Wave
 	if (nl) flush();
 	if (outV)
 	{
-		for (int i = lane; i < n; i += FH_WAVE) outV[f + i] = t.v[i];
+		for (int i = lane; i < n; i += FH_WAVE)
+		{
+			outV[f + i] = t.v[i];
+			if (keysBack) g.k[f + i] = t.k[i];
+		}
 	}
 	else
 	{
@@ -195,13 +204,17 @@ __device__ void waveStdSort(const exsort::Span<K>& g, int n, int* posL, int* pos
 		waveTileSort<K, kTile, kLaneMax>(g, 0, n, exsort::depthLimit(static_cast<uint32_t>(n)), t);
 		return;
 	}
-	// global ranges: stack kept in registers of every lane (uniform values)
-	WSeg st[40];
+	// global ranges: stack in LDS (a register array indexed at run time would
+	// live in scratch memory)
 	int top = 0;
-	st[top++] = WSeg{0, n, exsort::depthLimit(static_cast<uint32_t>(n))};
+	if (laneId() == 0) t.gstack[0] = WSeg{0, n, exsort::depthLimit(static_cast<uint32_t>(n))};
+	top = 1;
+	waveSync();
 	while (top > 0)
 	{
-		const WSeg w = st[--top];
+		--top;
+		const WSeg w = t.gstack[top];
+		waveSync();
 		if (w.l - w.f <= kTile)
 		{
 			waveTileSort<K, kTile, kLaneMax>(g, w.f, w.l, w.d, t);
@@ -214,10 +227,15 @@ __device__ void waveStdSort(const exsort::Span<K>& g, int n, int* posL, int* pos
 			continue;
 		}
 		const int cut = wavePartition(g, w.f, w.l, posL, posR);
-		const WSeg a{w.f, cut, w.d - 1}, b{cut, w.l, w.d - 1};
-		const bool leftSmaller = cut - w.f < w.l - cut;
-		st[top++] = leftSmaller ? b : a;
-		st[top++] = leftSmaller ? a : b;
+		if (laneId() == 0)
+		{
+			const WSeg a{w.f, cut, w.d - 1}, b{cut, w.l, w.d - 1};
+			const bool leftSmaller = cut - w.f < w.l - cut;
+			t.gstack[top] = leftSmaller ? b : a;
+			t.gstack[top + 1] = leftSmaller ? a : b;
+		}
+		top += 2;
+		waveSync();
 	}
 }
 

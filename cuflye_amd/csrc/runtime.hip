@@ -122,9 +122,9 @@ struct flyehip_ctx
 	fh::DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bHCur, bHExt, bGroupFlag, bGroupOff;
 	fh::DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
 	fh::DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
-	fh::DevBuf bGroupPass, bOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
+	fh::DevBuf bGroupPass, bOrder, bPassOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
 	fh::DevBuf bSegBig0, bSegBig1, bSegLeaf, bSegHeap, bSegCounters, bSegChunkOff, bSegChunkRange, bSegChunkCnt;
-	fh::DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp;
+	fh::DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp, bSegVerify, bSegShadowK, bSegShadowV;
 	int64_t* hostSeg = nullptr;  // pinned counter words of the segmented sort
 	fh::DevBuf bQueue, bCounters, bQRowCount, bCurLen, bDivOff, bDivCount, bDivOut, bQueryIds, bCand;
 	fh::DevBuf bExtWords, bExtWordOff, bExtLen;
@@ -460,6 +460,9 @@ SegSortBufs segBufs(flyehip_ctx* c, int64_t n, int64_t nArrays)
 	b.big[1] = ensureT<SegRange>(c, c->bSegBig1, static_cast<size_t>(caps.bigCap));
 	b.leaf = ensureT<SegRange>(c, c->bSegLeaf, static_cast<size_t>(caps.leafCap));
 	b.heap = ensureT<SegRange>(c, c->bSegHeap, static_cast<size_t>(caps.heapCap));
+	b.verify = ensureT<SegRange>(c, c->bSegVerify, static_cast<size_t>(caps.heapCap));
+	b.shadowKey = ensureT<uint32_t>(c, c->bSegShadowK, static_cast<size_t>(n) + 1);
+	b.shadowVal = ensureT<uint32_t>(c, c->bSegShadowV, static_cast<size_t>(n) + 1);
 	b.counters = ensureT<int64_t>(c, c->bSegCounters, kSegCounters);
 	b.chunkOff = ensureT<int64_t>(c, c->bSegChunkOff, static_cast<size_t>(caps.bigCap) + 1);
 	b.chunkRange = ensureT<int32_t>(c, c->bSegChunkRange, static_cast<size_t>(caps.chunkCap));
@@ -530,29 +533,65 @@ void runMatchSort(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches)
 	c->stats.tie_queries += ties;
 }
 
-// Order + prep + DP + finish over ca.nGroups groups; leaves groupCount set.
-void runChainKernels(flyehip_ctx* c, ChainArgs& ca)
+// Order, prefilters + extPos sort, DP, score sort + walks over ca.nGroups
+// groups whose hits total nHits; leaves groupCount set.
+void runChainKernels(flyehip_ctx* c, ChainArgs& ca, int64_t nHits)
 {
 	const int64_t ng = ca.nGroups;
 	zero32(c, ca.groupCount, ng + 1);
 	if (ng == 0) return;
+	if (nHits >= 0x7fffffffLL) fail(FLYEHIP_ERR_BUDGET, "batch of %lld hits exceeds 2^31 - 1", static_cast<long long>(nHits));
 	int32_t* bins = ensureT<int32_t>(c, c->bBins, 96);
 	int32_t* order = ensureT<int32_t>(c, c->bOrder, static_cast<size_t>(ng) + 1);
+	int32_t* passOrder = ensureT<int32_t>(c, c->bPassOrder, static_cast<size_t>(ng) + 1);
+	int32_t* flag = ensureT<int32_t>(c, c->bGroupFlag, static_cast<size_t>(ng) + 1);
+	int64_t* pos = ensureT<int64_t>(c, c->bGroupOff, static_cast<size_t>(ng) + 1);
+	int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(std::max(ng, nHits)) + 64);
 	int32_t* queues = ensureT<int32_t>(c, c->bQueue, 16);
+	int64_t* nBig = ensureT<int64_t>(c, c->bCounters, 16);
 	zero32(c, bins, 96);
 	zero32(c, queues, 16);
+	HIPCHECK(hipMemsetAsync(nBig, 0, sizeof(int64_t), c->stream));
+	SegSortBufs sb = segBufs(c, nHits, ng);
+	sb.key = ca.sortKey;
+	sb.val = ca.sortVal;
+	sb.outVal = ca.sortVal;
+	sb.dup = nullptr;
+	sb.posL = ca.posL;
+	sb.posR = ca.posR;
+	const int blocks = chainBlocks(c);
+
+	HIPCHECK(hipEventRecord(c->ev[2], c->stream));
 	HIPCHECK(launchGroupOrder(ca.groups, ca.nGroups, bins, order, c->stream));
 	ca.order = order;
-	const int blocks = chainBlocks(c);
-	HIPCHECK(hipEventRecord(c->ev[2], c->stream));
+	HIPCHECK(segSortReset(sb, c->stream));
 	ca.queue = queues + 0;
-	HIPCHECK(launchChainPrep(ca, blocks, c->stream));
-	HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+	HIPCHECK(launchChainPrefilter(ca, sb, blocks, c->stream));
+	runSegSort(c, sb, nullptr);
+	HIPCHECK(launchPassFlags(ca, flag, c->stream));
+	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, c->stream));
+	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, c->stream));
+	int64_t counts[2] = {0, 0};
+	download(c, &counts[0], pos + ng, sizeof(int64_t));
+	download(c, &counts[1], nBig, sizeof(int64_t));
+	sync(c);
+	ca.passOrder = passOrder;
+	ca.nPass = static_cast<int32_t>(counts[0]);
+	const int32_t nBigPass = static_cast<int32_t>(counts[1]);
 	ca.queue = queues + 1;
+	HIPCHECK(launchChainGather(ca, blocks, c->stream));
+	HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+	ca.queue = queues + 2;
 	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, c->stream));
 	HIPCHECK(hipEventRecord(c->ev[4], c->stream));
-	ca.queue = queues + 2;
-	HIPCHECK(launchChainFinish(ca, blocks, c->stream));
+	HIPCHECK(segSortReset(sb, c->stream));
+	ca.queue = queues + 3;
+	HIPCHECK(launchChainScoreKeys(ca, sb, blocks, c->stream));
+	runSegSort(c, sb, nullptr);
+	ca.queue = queues + 4;
+	HIPCHECK(launchChainWalk(ca, 0, nBigPass, true, blocks, c->stream));
+	ca.queue = queues + 5;
+	HIPCHECK(launchChainWalk(ca, nBigPass, ca.nPass, false, blocks, c->stream));
 	HIPCHECK(hipEventRecord(c->ev[5], c->stream));
 	HIPCHECK(hipEventSynchronize(c->ev[5]));
 	float a = 0, b = 0, d = 0;
@@ -627,7 +666,7 @@ BatchOut runChain(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches, const st
 	ca.groupPass = ensureT<int32_t>(c, c->bGroupPass, static_cast<size_t>(nGroups) + 1);
 	ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
 	ca.logY = static_cast<const uint32_t*>(c->dLogY.p);
-	runChainKernels(c, ca);
+	runChainKernels(c, ca, nMatches);
 	int64_t* rowStart = ensureT<int64_t>(c, c->bRowStart, static_cast<size_t>(nGroups) + 1);
 	HIPCHECK(exclusiveScan(ca.groupCount, rowStart, nGroups, tmp, c->stream));
 	int64_t nRows = 0;
@@ -851,9 +890,10 @@ int flyehip_close(flyehip_ctx* c)
 					 &c->bPosL, &c->bPosR, &c->bHCur, &c->bHExt, &c->bGroupFlag, &c->bGroupOff,
 					 &c->bFiltered, &c->bQMatchOff, &c->bQFiltOff, &c->bQDesc, &c->bGroups, &c->bGroupCount,
 					 &c->bRowStart, &c->bRows, &c->bCScore, &c->bCBt, &c->bCProps, &c->bCOut, &c->bCHCur,
-					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bGroupPass, &c->bOrder, &c->bBins, &c->bSKey, &c->bAVal, &c->bRKey, &c->bDupFlag, &c->bDup, &c->bMsortTemp, &c->bQueue,
+					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bGroupPass, &c->bOrder, &c->bPassOrder, &c->bBins, &c->bSKey, &c->bAVal, &c->bRKey, &c->bDupFlag, &c->bDup, &c->bMsortTemp, &c->bQueue,
 					 &c->bSegBig0, &c->bSegBig1, &c->bSegLeaf, &c->bSegHeap, &c->bSegCounters, &c->bSegChunkOff,
 					 &c->bSegChunkRange, &c->bSegChunkCnt, &c->bSegChunkPre, &c->bSegRangeM, &c->bSegRangeLm, &c->bSegTemp,
+					 &c->bSegVerify, &c->bSegShadowK, &c->bSegShadowV,
 					 &c->bCounters, &c->bQRowCount, &c->bCurLen, &c->bDivOff, &c->bDivCount, &c->bDivOut,
 					 &c->bQueryIds, &c->bCand, &c->bExtWords, &c->bExtWordOff, &c->bExtLen, &c->bAln, &c->bAlnOff,
 					 &c->bRA};
@@ -1169,7 +1209,7 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 		ca.groupPass = ensureT<int32_t>(c, c->bGroupPass, gs.size() + 1);
 		ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
 		ca.logY = static_cast<const uint32_t*>(c->dLogY.p);
-		runChainKernels(c, ca);
+		runChainKernels(c, ca, nHits);
 		int64_t* rowStart = ensureT<int64_t>(c, c->bRowStart, gs.size() + 1);
 		int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(static_cast<int64_t>(gs.size())) + 64);
 		HIPCHECK(exclusiveScan(ca.groupCount, rowStart, static_cast<int64_t>(gs.size()), tmp, c->stream));

@@ -18,6 +18,9 @@
 //           list, the leaf list or the heap list.
 #include <hipcub/hipcub.hpp>
 
+#include <cstdio>
+#include <cstdlib>
+
 #include "segsort.hpp"
 
 namespace fh
@@ -42,7 +45,26 @@ __global__ void segMedianKernel(SegSortBufs b, const SegRange* rs, int32_t nR)
 	}
 	const SegRange w = rs[r];
 	const exsort::Span<uint32_t> s{b.key, b.val};
-	exsort::medianToFirst(s, w.f, w.f + 1, w.f + (w.l - w.f) / 2, w.l - 1);
+	if (w.tag == 0)
+	{
+		exsort::medianToFirst(s, w.f, w.f + 1, w.f + (w.l - w.f) / 2, w.l - 1);
+	}
+	else
+	{
+		// free mode (any correct sort will do): median of three samples at a
+		// pseudo-random offset and a third of the range apart, which does not
+		// degenerate on the inputs that exhausted the exact budget.  The three
+		// positions are distinct, so one element >= and one <= the pivot stay
+		// in [f + 1, l) as the partition needs.
+		const uint32_t span = static_cast<uint32_t>(w.l - w.f - 1);
+		uint64_t h = (static_cast<uint64_t>(w.f) << 32) ^ static_cast<uint32_t>(w.l) ^ (static_cast<uint64_t>(w.d) << 56);
+		h = h * 6364136223846793005ULL + 1442695040888963407ULL;
+		const uint32_t o = static_cast<uint32_t>((h >> 33) % span);
+		const int32_t x0 = w.f + 1 + static_cast<int32_t>(o);
+		const int32_t x1 = w.f + 1 + static_cast<int32_t>((o + span / 3) % span);
+		const int32_t x2 = w.f + 1 + static_cast<int32_t>((o + 2 * (span / 3)) % span);
+		exsort::medianToFirst(s, w.f, x0, x1, x2);
+	}
 	b.chunkOff[r] = (static_cast<int64_t>(w.l - w.f) - 1 + kSegChunk - 1) / kSegChunk;
 	b.rangeM[r] = 0;
 	b.rangeLm[r] = 0x7fffffff;
@@ -217,8 +239,8 @@ __global__ void segSplitKernel(SegSortBufs b, const SegRange* rs, int32_t nR, Se
 			act = false;
 		}
 	}
-	segPushWave(b, next, act, w.f, cut, w.d - 1);
-	segPushWave(b, next, act, cut, w.l, w.d - 1);
+	segPushWave(b, next, act, w.f, cut, w.d - 1, w.tag);
+	segPushWave(b, next, act, cut, w.l, w.d - 1, w.tag);
 }
 
 __global__ void __launch_bounds__(FH_WAVE) segLeafKernel(SegSortBufs b, int64_t nLeaf)
@@ -227,17 +249,81 @@ __global__ void __launch_bounds__(FH_WAVE) segLeafKernel(SegSortBufs b, int64_t 
 	const int64_t i = blockIdx.x;
 	if (i >= nLeaf) return;
 	const SegRange w = b.leaf[i];
-	waveTileSort<uint32_t, kSegTile, 48>(exsort::Span<uint32_t>{b.key, b.val}, w.f, w.l, w.d, lds, b.outVal);
+	// free mode: the sorted keys stay in place for the verification
+	waveTileSort<uint32_t, kSegTile, 48>(exsort::Span<uint32_t>{b.key, b.val}, w.f, w.l, w.d, lds, b.outVal,
+										 w.tag != 0);
 }
 
-__global__ void segHeapKernel(SegSortBufs b, int64_t nHeap)
+// Heap sort, exactly libstdc++'s make_heap + sort_heap, of the ranges of a
+// list (only entries with d == 1 when onlyFlagged).  The sift steps are a
+// dependent chain, so one lane runs them; the range is staged in LDS as packed
+// (key << 32 | value) words so each step is one LDS round trip.  Ranges of
+// size in (minN, cap] are taken (dynamic LDS of cap words); larger ones go to
+// segHeapGlobalKernel.  Values go to dstV, keys to dstK when it is set.
+__global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list, int64_t nList, int onlyFlagged,
+															const uint32_t* srcK, const uint32_t* srcV, uint32_t* dstK,
+															uint32_t* dstV, int32_t minN, int32_t cap)
+{
+	extern __shared__ uint64_t hx[];
+	const int lane = laneId();
+	for (int64_t i = blockIdx.x; i < nList; i += gridDim.x)
+	{
+		const SegRange w = list[i];
+		const int n = w.l - w.f;
+		if (n > cap || n <= minN || (onlyFlagged && w.d != 1)) continue;
+		for (int t = lane; t < n; t += FH_WAVE)
+			hx[t] = (static_cast<uint64_t>(srcK[w.f + t]) << 32) | srcV[w.f + t];
+		waveSync();
+		if (lane == 0) exsort::heapSortPacked(hx, n);
+		waveSync();
+		for (int t = lane; t < n; t += FH_WAVE)
+		{
+			dstV[w.f + t] = static_cast<uint32_t>(hx[t]);
+			if (dstK) dstK[w.f + t] = static_cast<uint32_t>(hx[t] >> 32);
+		}
+		waveSync();
+	}
+}
+
+// The same in global memory, in place on (k, v), for ranges above minN; the
+// values are then copied to outV when it is a separate array.
+__global__ void segHeapGlobalKernel(const SegRange* list, int64_t nList, int onlyFlagged, uint32_t* k, uint32_t* v,
+									uint32_t* outV, int32_t minN)
 {
 	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (i >= nHeap) return;
-	const SegRange w = b.heap[i];
-	exsort::heapSort(exsort::Span<uint32_t>{b.key, b.val}, w.f, w.l);
-	if (b.outVal != b.val)
-		for (int32_t x = w.f; x < w.l; ++x) b.outVal[x] = b.val[x];
+	if (i >= nList) return;
+	const SegRange w = list[i];
+	if (w.l - w.f <= minN || (onlyFlagged && w.d != 1)) return;
+	exsort::heapSort(exsort::Span<uint32_t>{k, v}, w.f, w.l);
+	if (outV != v)
+		for (int32_t x = w.f; x < w.l; ++x) outV[x] = v[x];
+}
+
+// Shadow copies of the verify entries [v0, v1): their content at the moment
+// the depth budget ran out (one wave per entry).
+__global__ void segShadowKernel(SegSortBufs b, int64_t v0, int64_t v1)
+{
+	const int64_t i = v0 + waveIndex();
+	if (i >= v1) return;
+	const SegRange w = b.verify[i];
+	for (int32_t x = w.f + laneId(); x < w.l; x += FH_WAVE)
+	{
+		b.shadowKey[x] = b.key[x];
+		b.shadowVal[x] = b.val[x];
+	}
+}
+
+// A verify entry whose free-mode result has two equal neighbouring keys needs
+// the exact heap sort (d = 1); with all keys distinct every sort agrees.
+__global__ void segVerifyKernel(SegSortBufs b, int64_t nVerify)
+{
+	const int64_t i = waveIndex();
+	if (i >= nVerify) return;
+	const SegRange w = b.verify[i];
+	bool eq = false;
+	for (int32_t x = w.f + 1 + laneId(); x < w.l; x += FH_WAVE) eq |= b.key[x] == b.key[x - 1];
+	const bool any = __ballot(eq) != 0;
+	if (laneId() == 0) b.verify[i].d = any ? 1 : 0;
 }
 
 __global__ void segPushOffsetsKernel(SegSortBufs b, const int64_t* off, int32_t nArrays)
@@ -254,7 +340,7 @@ __global__ void segPushOffsetsKernel(SegSortBufs b, const int64_t* off, int32_t 
 	const int t = __popcll(__ballot(tied));
 	if (laneId() == 0 && t) atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegTieArrays]),
 									  static_cast<unsigned long long>(t));
-	segPushWave(b, b.big[0], act, f, l, act ? exsort::depthLimit(static_cast<uint32_t>(l - f)) : 0);
+	segPushWave(b, b.big[0], act, f, l, act ? exsort::depthLimit(static_cast<uint32_t>(l - f)) : 0, 0);
 }
 
 hipError_t download(const SegSortBufs& b, hipStream_t s)
@@ -262,6 +348,28 @@ hipError_t download(const SegSortBufs& b, hipStream_t s)
 	hipError_t e = hipMemcpyAsync(b.hostCounters, b.counters, kSegCounters * sizeof(int64_t), hipMemcpyDeviceToHost, s);
 	if (e != hipSuccess) return e;
 	return hipStreamSynchronize(s);
+}
+
+// Heap sorts of a list through the LDS tiers (32 KiB, 64 KiB, 128 KiB) and
+// the global fallback (in place on gk / gv).
+hipError_t heapTiers(const SegRange* list, int64_t n, int onlyFlagged, const uint32_t* srcK, const uint32_t* srcV,
+					 uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s)
+{
+	static const bool bigLds =
+		hipFuncSetAttribute(reinterpret_cast<const void*>(segHeapLdsKernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+							16384 * 8) == hipSuccess;
+	const unsigned grid = static_cast<unsigned>(n < 8192 ? n : 8192);
+	int32_t lo = 0;
+	for (int32_t cap : {4096, 8192, 16384})
+	{
+		if (cap == 16384 && !bigLds) break;
+		hipLaunchKernelGGL(segHeapLdsKernel, dim3(grid), dim3(FH_WAVE), static_cast<size_t>(cap) * 8, s, list, n,
+						   onlyFlagged, srcK, srcV, dstK, dstV, lo, cap);
+		lo = cap;
+	}
+	hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, s, list, n, onlyFlagged, gk, gv, dstV,
+					   lo);
+	return hipGetLastError();
 }
 
 hipError_t runLeaves(SegSortBufs& b, hipStream_t s)
@@ -313,9 +421,18 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 	if (e != hipSuccess) return e;
 	if (tieArrays) *tieArrays = b.hostCounters[kSegTieArrays];
 	int cur = 0;
+	int rounds = 0;
+	int64_t nVerified = 0;
 	const int tpb = FH_WAVE * kWavesPerBlock;
 	while (b.hostCounters[kSegNBig] > 0)
 	{
+		++rounds;
+		if (std::getenv("FLYEHIP_SEGSORT_TRACE"))
+			std::fprintf(stderr, "[segsort] round %d: big=%lld chunks=%lld leaves=%lld heap=%lld\n", rounds,
+						 static_cast<long long>(b.hostCounters[kSegNBig]),
+						 static_cast<long long>(b.hostCounters[kSegBigChunks]),
+						 static_cast<long long>(b.hostCounters[kSegNLeaf]),
+						 static_cast<long long>(b.hostCounters[kSegNHeap]));
 		const int64_t nR = b.hostCounters[kSegNBig];
 		const int64_t nChunks = b.hostCounters[kSegBigChunks];
 		if (nR > b.bigCap || nChunks > b.chunkCap) return hipErrorInvalidValue;
@@ -351,16 +468,34 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 		if (e != hipSuccess) return e;
 		e = download(b, s);
 		if (e != hipSuccess) return e;
+		const int64_t nv = b.hostCounters[kSegNVerify];
+		if (nv > nVerified)
+		{
+			hipLaunchKernelGGL(segShadowKernel, dim3(blocksFor(nv - nVerified, kWavesPerBlock)), dim3(tpb), 0, s, b,
+							   nVerified, nv);
+			nVerified = nv;
+		}
 		cur ^= 1;
 	}
 	if (b.hostCounters[kSegError]) return hipErrorIllegalState;
 	e = runLeaves(b, s);
 	if (e != hipSuccess) return e;
 	const int64_t nHeap = b.hostCounters[kSegNHeap];
+	if (std::getenv("FLYEHIP_SEGSORT_TRACE"))
+		std::fprintf(stderr, "[segsort] rounds=%d verify=%lld free-heap=%lld\n", rounds,
+					 static_cast<long long>(b.hostCounters[kSegNVerify]), static_cast<long long>(nHeap));
 	if (nHeap > 0)
 	{
-		hipLaunchKernelGGL(segHeapKernel, dim3(blocksFor(nHeap, 64)), dim3(64), 0, s, b, nHeap);
-		e = hipGetLastError();
+		// free-mode ranges out of budget: any sort will do; keys stay in place
+		e = heapTiers(b.heap, nHeap, 0, b.key, b.val, b.key, b.outVal, b.key, b.val, s);
+		if (e != hipSuccess) return e;
+	}
+	const int64_t nVerify = b.hostCounters[kSegNVerify];
+	if (nVerify > 0)
+	{
+		const int tpb = FH_WAVE * kWavesPerBlock;
+		hipLaunchKernelGGL(segVerifyKernel, dim3(blocksFor(nVerify, kWavesPerBlock)), dim3(tpb), 0, s, b, nVerify);
+		e = heapTiers(b.verify, nVerify, 1, b.shadowKey, b.shadowVal, nullptr, b.outVal, b.shadowKey, b.shadowVal, s);
 		if (e != hipSuccess) return e;
 	}
 	return hipSuccess;

@@ -12,8 +12,13 @@
 //     the data of all arrays, not a chain of dependent steps per array;
 //   * a range of at most kSegTile elements ("leaf") is finished by one wave in
 //     LDS with its remaining depth budget (waveTileSort);
-//   * a big range whose depth budget is spent is heap-sorted (one lane; only
-//     adversarial inputs get there).
+//   * a big range whose depth budget is spent would be heap-sorted by
+//     std::sort.  Heapsort is a chain of dependent sift steps, but its result
+//     differs from any other sort only in the order of equal keys; so the
+//     range is saved ("shadow" copy) and sorted on in free mode (partitions
+//     without a depth budget, children tagged with the range); afterwards its
+//     sorted keys are checked for an equal neighbour pair, and only a range
+//     that has one is heap-sorted from the shadow copy (one lane, in LDS).
 // Optionally a range is dropped when the sorted order already known for it is
 // final: with `dup` set (exclusive prefix counts over the stable sorted order
 // of the positions whose key is shared with a neighbour, "tied" positions), a
@@ -36,12 +41,14 @@ constexpr int kSegTile = 2048;        // leaf size, elements (LDS tile of one wa
 constexpr int kSegRows = 8;           // 64-element rows per chunk
 constexpr int kSegChunk = 64 * kSegRows;
 
+constexpr int kSegFreeDepth = 64;      // depth budget of a free-mode range
+
 struct SegRange
 {
-	int32_t f;  // absolute positions in the key / value arrays
+	int32_t f;    // absolute positions in the key / value arrays
 	int32_t l;
-	int32_t d;  // remaining depth budget
-	int32_t pad;
+	int32_t d;    // remaining depth budget (verify list: 1 = needs the exact heap sort)
+	int32_t tag;  // 0: exact mode; t > 0: free mode under verify entry t - 1
 };
 
 // Counter words (int64) shared by the init kernels and the rounds.
@@ -53,6 +60,7 @@ enum SegCounter
 	kSegNHeap,
 	kSegTieArrays,    // arrays with at least one tied position (pruned sorts)
 	kSegError,        // set when a partition produced an impossible cut or swap
+	kSegNVerify,      // exact-mode ranges whose depth budget ran out
 	kSegCounters = 8
 };
 
@@ -66,7 +74,10 @@ struct SegSortBufs
 	int32_t* posR;
 	SegRange* big[2];       // ping-pong lists of big ranges
 	SegRange* leaf;
-	SegRange* heap;
+	SegRange* heap;         // free-mode ranges whose (free) budget ran out
+	SegRange* verify;       // exact-mode ranges whose budget ran out
+	uint32_t* shadowKey;    // their content when the budget ran out (one per element)
+	uint32_t* shadowVal;
 	int64_t* counters;      // kSegCounters words
 	int64_t* chunkOff;      // per big range of a round: chunk offsets (bigCap + 1)
 	int32_t* chunkRange;    // per chunk (chunkCap)
@@ -78,7 +89,7 @@ struct SegSortBufs
 	size_t scanTempBytes;
 	int64_t bigCap;
 	int64_t leafCap;
-	int64_t heapCap;
+	int64_t heapCap;        // also the capacity of verify
 	int64_t chunkCap;
 	int64_t* hostCounters;  // pinned, kSegCounters words
 };
@@ -95,59 +106,67 @@ struct SegSortCaps
 // Capacities for arrays totalling n elements in nArrays arrays.
 hipError_t segSortCaps(int64_t n, int64_t nArrays, SegSortCaps* caps);
 
-// Appends a range of [f, l) with depth d to the lists: leaf, heap or big
-// (bigList).  Every lane of the wave calls it; lanes with push == false add
-// nothing.  Ranges of fewer than 2 elements, and ranges pruned by `dup`, are
-// dropped.  One atomic per list and wave.
-__device__ inline void segPushWave(const SegSortBufs& b, SegRange* bigList, bool push, int32_t f, int32_t l, int32_t d)
+__device__ inline unsigned long long segAppend(int64_t* counter, unsigned long long m)
+{
+	const int leader = __ffsll(static_cast<long long>(m)) - 1;
+	unsigned long long base = 0;
+	if (laneId() == leader)
+		base = atomicAdd(reinterpret_cast<unsigned long long*>(counter), static_cast<unsigned long long>(__popcll(m)));
+	base = __shfl(base, leader, FH_WAVE);
+	return base + __popcll(m & laneMaskLt());
+}
+
+// Appends a range [f, l) with depth d and tag to the lists: leaf, big
+// (bigList), verify + big (an exact-mode range out of budget), or heap (a
+// free-mode range out of budget).  Every lane of the wave calls it; lanes with
+// push == false add nothing.  Ranges pruned by `dup` (exact mode) are dropped;
+// a one-element range is final in place and only copies its value when the
+// output array is separate.  One atomic per list and wave.
+__device__ inline void segPushWave(const SegSortBufs& b, SegRange* bigList, bool push, int32_t f, int32_t l, int32_t d,
+								   int32_t tag)
 {
 	const int32_t n = l - f;
+	if (push && n == 1 && b.outVal != b.val && (tag != 0 || !b.dup || b.dup[l] - b.dup[f] > 0))
+		b.outVal[f] = b.val[f];
 	if (n < 2) push = false;
-	if (push && b.dup && b.dup[l] - b.dup[f] == 0) push = false;
+	if (push && tag == 0 && b.dup && b.dup[l] - b.dup[f] == 0) push = false;
 	const bool toLeaf = push && n <= kSegTile;
-	const bool toHeap = push && n > kSegTile && d == 0;
-	const bool toBig = push && n > kSegTile && d > 0;
-	const unsigned long long lt = laneMaskLt();
-	const int lane = laneId();
-	const SegRange r{f, l, d, 0};
+	const bool toVerify = push && n > kSegTile && d == 0 && tag == 0;
+	const bool toHeap = push && n > kSegTile && d == 0 && tag != 0;
 	unsigned long long m = __ballot(toLeaf);
 	if (m)
 	{
-		const int leader = __ffsll(static_cast<long long>(m)) - 1;
-		unsigned long long base = 0;
-		if (lane == leader)
-			base = atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegNLeaf]),
-							 static_cast<unsigned long long>(__popcll(m)));
-		base = __shfl(base, leader, FH_WAVE);
-		if (toLeaf) b.leaf[base + __popcll(m & lt)] = r;
+		const unsigned long long i = segAppend(&b.counters[kSegNLeaf], m);
+		if (toLeaf) b.leaf[i] = SegRange{f, l, d, tag};
 	}
 	m = __ballot(toHeap);
 	if (m)
 	{
-		const int leader = __ffsll(static_cast<long long>(m)) - 1;
-		unsigned long long base = 0;
-		if (lane == leader)
-			base = atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegNHeap]),
-							 static_cast<unsigned long long>(__popcll(m)));
-		base = __shfl(base, leader, FH_WAVE);
-		if (toHeap) b.heap[base + __popcll(m & lt)] = r;
+		const unsigned long long i = segAppend(&b.counters[kSegNHeap], m);
+		if (toHeap) b.heap[i] = SegRange{f, l, 0, tag};
 	}
+	m = __ballot(toVerify);
+	if (m)
+	{
+		const unsigned long long i = segAppend(&b.counters[kSegNVerify], m);
+		if (toVerify)
+		{
+			b.verify[i] = SegRange{f, l, 0, 0};
+			d = kSegFreeDepth;
+			tag = static_cast<int32_t>(i) + 1;
+		}
+	}
+	const bool toBig = push && n > kSegTile && d > 0;
 	m = __ballot(toBig);
 	if (m)
 	{
-		const int leader = __ffsll(static_cast<long long>(m)) - 1;
 		const int64_t chunks = toBig ? (static_cast<int64_t>(n) - 1 + kSegChunk - 1) / kSegChunk : 0;
 		const int64_t chunkSum = waveSum64(chunks);
-		unsigned long long base = 0;
-		if (lane == leader)
-		{
-			base = atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegNBig]),
-							 static_cast<unsigned long long>(__popcll(m)));
+		const unsigned long long i = segAppend(&b.counters[kSegNBig], m);
+		if (laneId() == __ffsll(static_cast<long long>(m)) - 1)
 			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegBigChunks]),
 					  static_cast<unsigned long long>(chunkSum));
-		}
-		base = __shfl(base, leader, FH_WAVE);
-		if (toBig) bigList[base + __popcll(m & lt)] = r;
+		if (toBig) bigList[i] = SegRange{f, l, d, tag};
 	}
 }
 

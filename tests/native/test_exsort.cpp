@@ -52,6 +52,28 @@ static void check(const std::vector<uint64_t>& keys, const char* what)
 	}
 }
 
+// Packed heap sort vs libstdc++'s partial_sort(first, last, last) (make_heap +
+// sort_heap) on 32-bit keys with many ties.
+static void checkHeapPacked(const std::vector<uint64_t>& keys)
+{
+	const size_t n = keys.size();
+	std::vector<uint64_t> a(n);
+	for (size_t i = 0; i < n; ++i) a[i] = (static_cast<uint64_t>(static_cast<uint32_t>(keys[i])) << 32) | i;
+	exsort::heapSortPacked(a.data(), static_cast<int>(n));
+	std::vector<uint32_t> idx(n);
+	for (size_t i = 0; i < n; ++i) idx[i] = static_cast<uint32_t>(i);
+	std::partial_sort(idx.begin(), idx.end(), idx.end(), [&](uint32_t x, uint32_t y) {
+		return static_cast<uint32_t>(keys[x]) < static_cast<uint32_t>(keys[y]);
+	});
+	for (size_t i = 0; i < n; ++i)
+		if (static_cast<uint32_t>(a[i]) != idx[i])
+		{
+			if (failures < 10) std::printf("MISMATCH heapPacked n=%zu at %zu\n", n, i);
+			++failures;
+			return;
+		}
+}
+
 // Rank partition vs scanning partition on the same median-moved array.
 static void checkPartition(const std::vector<uint64_t>& keys)
 {
@@ -116,6 +138,7 @@ int main()
 				for (auto& x : keys) x = rng() % static_cast<uint64_t>(distinct);
 				check(keys, "random");
 				checkPartition(keys);
+				checkHeapPacked(keys);
 				++cases;
 			}
 		}

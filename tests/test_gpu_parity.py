@@ -152,6 +152,48 @@ def test_chain_lists_config3_small(gpu, oracle_mod):
     assert_rows_equal(rows, erows, "config3 lists, containment selection")
 
 
+def _diagonal_lists(rng, sizes, twin):
+    """Long clean diagonals: DP scores grow along the list, so the score-order
+    keys are nearly reverse sorted, the input on which libstdc++'s introsort
+    exhausts its depth budget and heap-sorts.  With `twin` every list carries
+    a second, identically spaced diagonal, so every score occurs twice and
+    the heap sort's tie order decides the walk order."""
+    cur, ext, offs = [], [], [0]
+    for n in sizes:
+        m = n // 2 if twin else n
+        step = rng.integers(5, 9, size=m).cumsum().astype(np.int32)
+        c = 100 + step
+        e = 50 + step
+        if twin:
+            c = np.concatenate([c, c])
+            e = np.concatenate([e, e + 3 * int(step[-1]) // 2])
+            o = np.argsort(c, kind="stable")
+            c, e = c[o], e[o]
+        cur.append(c)
+        ext.append(e)
+        offs.append(offs[-1] + len(c))
+    return np.concatenate(cur), np.concatenate(ext), np.array(offs, np.int64)
+
+
+@pytest.mark.parametrize("twin", [False, True])
+def test_chain_lists_heap_fallback(gpu, oracle_mod, twin):
+    rng = np.random.default_rng(31 + int(twin))
+    sizes = [int(x) for x in rng.integers(2500, 24000, size=24)]
+    cur, ext, offs = _diagonal_lists(rng, sizes, twin)
+    n = len(offs) - 1
+    ci = np.arange(1, n + 1, dtype=np.int64)
+    ei = ci + 1000
+    cl = np.full(n, int(cur.max()) + 500, np.int32)
+    el = cl + 7  # extLen > curLen: the extPos sort runs too
+    for only_max in (True, False):
+        cfg = OverlapConfig(kmer_size=15, minimum_overlap=200, check_overhang=False, only_max_ext=only_max)
+        rows, roff = gpu.chain_lists(cur, ext, offs, ci, ei, cl, el, cfg)
+        erows, eroff = oracle_mod.chain_lists(cur, ext, offs, ci, ei, cl, el, cfg, threads=8)
+        assert np.array_equal(roff, eroff)
+        assert_rows_equal(rows, erows, f"diagonal lists twin={twin} only_max={only_max}")
+        assert len(rows) >= n
+
+
 def _random_alignments(rng, n_reads):
     rows = []
     offs = [0]
