@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--per-query-sample", type=int, default=200, help="query records on the per-query path")
     ap.add_argument("--batch", type=int, default=0, help="queries per GPU batch (0 = autotune)")
     ap.add_argument("--dp-lanes", type=int, default=0, help="DP lanes per group (0 = autotune)")
+    ap.add_argument("--flights", type=int, default=0, help="batches in flight (0 = autotune)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson request rate (1/s)")
     ap.add_argument("--serve-genome", type=int, default=200_000_000)
@@ -146,12 +147,14 @@ def main() -> int:
         f"distinct={info.distinct_kmers} build={index_s:.2f}s")
 
     tuning = None
-    if args.batch and args.dp_lanes:
-        g.set_tuning(dp_lanes_per_group=args.dp_lanes, queries_per_batch=args.batch)
+    if args.batch and args.dp_lanes and args.flights:
+        g.set_tuning(dp_lanes_per_group=args.dp_lanes, queries_per_batch=args.batch, batches_in_flight=args.flights)
     else:
-        sample = ids[: min(len(ids), 1024)]
+        # a sample of several batches, so that batches in flight can overlap
+        sample = ids[: min(len(ids), 4096)]
         tuning = autotune.tune(g, sample, cfg, batches=(args.batch,) if args.batch else (512, 1024),
-                               lanes=(args.dp_lanes,) if args.dp_lanes else autotune.DP_LANES, log=log)
+                               lanes=(args.dp_lanes,) if args.dp_lanes else (16, 32, 64),
+                               flights=(args.flights,) if args.flights else (1, 2, 3), log=log)
         log(f"autotune best: {tuning['best']}")
 
     def step():
@@ -238,7 +241,8 @@ def main() -> int:
         "config": {"workload": "5 Mb ONT-like read set, 40x, k=15, getSeqOverlaps for both strands of every read",
                    "genome_bp": reads.genome_len, "reads": int(n), "query_records": int(len(ids)),
                    "bases": int(len(reads.bases)), "minimum_overlap": cfg.minimum_overlap,
-                   "tuning": tuning["best"] if tuning else {"dp_lanes": args.dp_lanes, "batch": args.batch}},
+                   "tuning": tuning["best"] if tuning else {"dp_lanes": args.dp_lanes, "batch": args.batch,
+                                                                   "batches_in_flight": args.flights}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": dom, "algorithmic_bytes": stage_bytes[dom], "kernel_ms": stats[dom]},

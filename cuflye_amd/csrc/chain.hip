@@ -29,6 +29,7 @@ namespace
 {
 
 constexpr int kLaneMax = 48;
+constexpr uint32_t kWalkRelevant = 0x80000000u;  // score-order value bit: row has a backtrack pointer
 constexpr int kRingTotal = 2048;  // LDS ring entries per wave (two rings), split over groups
 
 __device__ inline float flyeLogf(float x, const uint32_t* tabX, const uint32_t* tabY, int tabN)
@@ -423,6 +424,9 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 
 // Score-order sort keys of every passing group (std::sort(indices,
 // score[a] > score[b]) as an ascending key), pushed to the segmented sort.
+// The walk skips every start without a backtrack pointer, so among equal
+// scores only the order of rows that have one can matter: those carry
+// kWalkRelevant in their value.
 __global__ void __launch_bounds__(FH_WAVE) chainScoreKeysKernel(ChainArgs a, SegSortBufs sb)
 {
 	const int lane = laneId();
@@ -434,12 +438,13 @@ __global__ void __launch_bounds__(FH_WAVE) chainScoreKeysKernel(ChainArgs a, Seg
 		const int n = gd.n;
 		const int64_t hb = gd.hitStart;
 		const int32_t* sc = a.score + hb;
+		const int32_t* bt = a.bt + hb;
 		uint32_t* sk = a.sortKey + hb;
 		uint32_t* sv = a.sortVal + hb;
 		for (int i = lane; i < n; i += FH_WAVE)
 		{
 			sk[i] = static_cast<uint32_t>(0x7fffffff - sc[i]);
-			sv[i] = static_cast<uint32_t>(i);
+			sv[i] = static_cast<uint32_t>(i) | (bt[i] != -1 ? kWalkRelevant : 0u);
 		}
 		segPushWave(sb, sb.big[0], lane == 0, static_cast<int32_t>(hb), static_cast<int32_t>(hb + n),
 					exsort::depthLimit(static_cast<uint32_t>(n)), 0);
@@ -498,7 +503,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		for (int q0 = 0; q0 < n; q0 += FH_WAVE)
 		{
 			const int q = q0 + lane;
-			const int myStart = q < n ? static_cast<int>(sv[q]) : 0;
+			const int myStart = q < n ? static_cast<int>(sv[q] & ~kWalkRelevant) : 0;
 			const bool live = q < n && bt[myStart] != -1;
 			const unsigned long long m = __ballot(live);
 			recStart[lane] = myStart;

@@ -58,7 +58,8 @@ struct ChainArgs
 	const int32_t* filtered;   // per-query ascending filtered positions
 	// per-hit scratch (indexed like the hits)
 	uint32_t* sortKey;         // 32-bit sort keys (extPos, INT_MAX - score)
-	uint32_t* sortVal;         // index within the group; sorted in place with the keys
+	uint32_t* sortVal;         // index within the group, sorted in place with the keys (score order:
+							   // bit 31 set on rows with a backtrack pointer)
 	int32_t* posL;
 	int32_t* posR;
 	int32_t* hcur;             // group-order copies (after the extPos re-sort)

@@ -246,6 +246,128 @@ EXS_HD void heapSortPacked(uint64_t* a, int n)
 	}
 }
 
+// ---- pipelined heap sort (exactly partial_sort(first, last, last)) -------------
+//
+// libstdc++'s __adjust_heap (hole down the larger-child path to a leaf, then
+// __push_heap of the value back up) leaves the same array as a top-down sift:
+// the larger child (the right one unless right < left) moves up while
+// !(child < value); the value settles where that fails.  The path depends only
+// on the children, and both forms stop at the deepest path node whose child
+// is below the value.
+//
+// make_heap sifts parents from the last one up to the root; parents of one
+// heap level have disjoint subtrees, so a level can be sifted in any order.
+// sort_heap pops in sequence, but pop t only reads a level after pop t-1 has
+// written it for the last time: started two steps apart, pops advance one
+// level per step in lock step, each on its own level.  A pop may start only
+// when the previous one has taken two steps, and when no running pop's hole
+// is an ancestor of the position the new pop reads its value from and writes
+// the popped root to (the last heap position).  heapSortPipelinedModel runs
+// exactly this schedule on the host; the GPU kernel runs the same steps with
+// one lane per running pop.
+
+EXS_HD bool isAncestorOrSelf(int h, int x)
+{
+	// 1-based heap numbering: the ancestor of X k levels up is X >> k
+	unsigned H = static_cast<unsigned>(h) + 1u, X = static_cast<unsigned>(x) + 1u;
+	if (H > X) return false;
+	int dh = 0, dx = 0;
+	for (unsigned t = H; t > 1; t >>= 1) ++dh;
+	for (unsigned t = X; t > 1; t >>= 1) ++dx;
+	return (X >> (dx - dh)) == H;
+}
+
+// One top-down sift step at hole h in a heap of m packed elements holding v;
+// returns false (and stores v) when the value has settled.
+EXS_HD bool siftStepPacked(uint64_t* a, int& h, int m, uint64_t v)
+{
+	const int c1 = 2 * h + 1;
+	if (c1 >= m)
+	{
+		a[h] = v;
+		return false;
+	}
+	int child = c1;
+	uint64_t cv = a[c1];
+	if (c1 + 1 < m)
+	{
+		const uint64_t c2v = a[c1 + 1];
+		if (!packedLess(c2v, cv))
+		{
+			child = c1 + 1;
+			cv = c2v;
+		}
+	}
+	if (packedLess(cv, v))
+	{
+		a[h] = v;
+		return false;
+	}
+	a[h] = cv;
+	h = child;
+	return true;
+}
+
+EXS_HD void siftPacked(uint64_t* a, int h, int m, uint64_t v)
+{
+	while (siftStepPacked(a, h, m, v))
+	{
+	}
+}
+
+// make_heap, level by level from the deepest parents up.
+EXS_HD void makeHeapByLevelsPacked(uint64_t* a, int n)
+{
+	if (n < 2) return;
+	const int lastParent = (n - 2) / 2;
+	int top = 0;
+	for (unsigned t = static_cast<unsigned>(lastParent) + 1u; t > 1; t >>= 1) ++top;
+	for (int lv = top; lv >= 0; --lv)
+	{
+		const int lo = (1 << lv) - 1;
+		const int hi = (lastParent < (2 << lv) - 2) ? lastParent : (2 << lv) - 2;
+		for (int p = hi; p >= lo; --p) siftPacked(a, p, n, a[p]);
+	}
+}
+
+// Host model of the GPU schedule (kSlots pop slots, pops two steps apart).
+template <int kSlots = 16>
+inline void heapSortPipelinedModel(uint64_t* a, int n)
+{
+	if (n < 2) return;
+	makeHeapByLevelsPacked(a, n);
+	bool act[kSlots] = {};
+	int hole[kSlots] = {}, size[kSlots] = {};
+	uint64_t val[kSlots] = {};
+	int L = n, lastStart = -1000, pops = 0;
+	for (int cyc = 0;; ++cyc)
+	{
+		if (L > 1)
+		{
+			const int slot = pops % kSlots;
+			bool ok = cyc - lastStart >= 2 && !act[slot];
+			for (int j = 0; j < kSlots; ++j)
+				if (act[j] && isAncestorOrSelf(hole[j], L - 1)) ok = false;
+			if (ok)
+			{
+				val[slot] = a[L - 1];
+				a[L - 1] = a[0];
+				hole[slot] = 0;
+				size[slot] = L - 1;
+				act[slot] = true;
+				lastStart = cyc;
+				--L;
+				++pops;
+			}
+		}
+		bool any = false;
+		for (int j = 0; j < kSlots; ++j) any |= act[j];
+		if (!any && L <= 1) break;
+		for (int j = 0; j < kSlots; ++j)
+			if (act[j]) act[j] = siftStepPacked(a, hole[j], size[j], val[j]);
+	}
+}
+
 // Stable insertion sort of [first, last): the final pass of std::sort.
 template <class K>
 EXS_HD void insertionSort(const Span<K>& s, int first, int last)

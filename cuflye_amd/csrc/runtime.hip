@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -18,8 +19,10 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <exception>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "chain.hpp"
@@ -79,6 +82,30 @@ struct DevBuf
 	size_t bytes = 0;
 };
 
+// One batch pipeline: a stream (plus side streams) and the grow-only batch
+// buffers.  Batches on different lanes run concurrently; they share the read
+// store, the index and the memory budget of the context.
+struct Lane
+{
+	flyehip_ctx* c = nullptr;
+	hipStream_t stream = nullptr;
+	hipStream_t aux[3] = {};    // side streams (default priority)
+	hipEvent_t auxEv[4] = {};   // timing disabled
+	hipEvent_t ev[8] = {};
+	int64_t* hostSeg = nullptr;  // pinned counter words of the segmented sort
+	flyehip_stats_t stats{};     // stage timings and counts of this lane's batches
+	DevBuf bQueries, bWinOff, bWinCount, bWinFilt, bWinInfo, bMatchOff, bFiltOff, bScanTmp;
+	DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bHCur, bHExt, bGroupFlag, bGroupOff;
+	DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
+	DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
+	DevBuf bGroupPass, bOrder, bPassOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
+	DevBuf bSegBig0, bSegBig1, bSegLeaf, bSegHeap, bSegCounters, bSegChunkOff, bSegChunkRange, bSegChunkCnt;
+	DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp, bSegVerify, bSegShadowK, bSegShadowV;
+	DevBuf bQueue, bCounters, bQRowCount, bCurLen, bDivOff, bDivCount, bDivOut, bQueryIds, bCand;
+	DevBuf bExtWords, bExtWordOff, bExtLen;
+	DevBuf bAln, bAlnOff, bRA;
+};
+
 } // namespace
 } // namespace fh
 
@@ -88,16 +115,15 @@ struct flyehip_ctx
 	uint64_t budget = 0;
 	uint64_t inUse = 0;
 	uint64_t peak = 0;
-	hipStream_t stream = nullptr;
-	hipEvent_t ev[8] = {};
+	std::mutex memMu;  // budget accounting (lanes allocate from their own threads)
 	std::string err;
 	std::mutex mu;
-	std::vector<fh::DevBuf*> owned;
+	std::vector<std::unique_ptr<fh::Lane>> lanes;  // lanes[0] serves every single-stream call
 
 	// tuning
 	int dpLanes = 64;
 	int queriesPerBatch = 2048;
-	int groupsPerPull = 1;
+	int batchesInFlight = 2;
 	int chainBlocks = 0;
 
 	// read store
@@ -117,19 +143,6 @@ struct flyehip_ctx
 	fh::DevBuf dLogX, dLogY;
 	int logN = 0;
 
-	// batch scratch (grow-only)
-	fh::DevBuf bQueries, bWinOff, bWinCount, bWinFilt, bWinInfo, bMatchOff, bFiltOff, bScanTmp;
-	fh::DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bHCur, bHExt, bGroupFlag, bGroupOff;
-	fh::DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
-	fh::DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
-	fh::DevBuf bGroupPass, bOrder, bPassOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
-	fh::DevBuf bSegBig0, bSegBig1, bSegLeaf, bSegHeap, bSegCounters, bSegChunkOff, bSegChunkRange, bSegChunkCnt;
-	fh::DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp, bSegVerify, bSegShadowK, bSegShadowV;
-	int64_t* hostSeg = nullptr;  // pinned counter words of the segmented sort
-	fh::DevBuf bQueue, bCounters, bQRowCount, bCurLen, bDivOff, bDivCount, bDivOut, bQueryIds, bCand;
-	fh::DevBuf bExtWords, bExtWordOff, bExtLen;
-	fh::DevBuf bAln, bAlnOff, bRA;
-
 	flyehip_stats_t stats{};
 };
 
@@ -142,7 +155,8 @@ void release(flyehip_ctx* c, DevBuf& b)
 {
 	if (b.p)
 	{
-		hipFree(b.p);
+		(void)hipFree(b.p);
+		std::lock_guard<std::mutex> lock(c->memMu);
 		c->inUse -= b.bytes;
 		b.p = nullptr;
 		b.bytes = 0;
@@ -156,14 +170,25 @@ void* ensure(flyehip_ctx* c, DevBuf& b, size_t bytes)
 	release(c, b);
 	size_t want = bytes + bytes / 8;
 	want = (want + 255) & ~static_cast<size_t>(255);
-	if (c->inUse + want > c->budget)
-		fail(FLYEHIP_ERR_BUDGET,
-			 "device memory budget exceeded: need %zu more bytes, %llu in use, budget %llu (device %d)", want,
-			 static_cast<unsigned long long>(c->inUse), static_cast<unsigned long long>(c->budget), c->device);
-	HIPCHECK(hipMalloc(&b.p, want));
+	{
+		// reserve under the lock, allocate outside it
+		std::lock_guard<std::mutex> lock(c->memMu);
+		if (c->inUse + want > c->budget)
+			fail(FLYEHIP_ERR_BUDGET,
+				 "device memory budget exceeded: need %zu more bytes, %llu in use, budget %llu (device %d)", want,
+				 static_cast<unsigned long long>(c->inUse), static_cast<unsigned long long>(c->budget), c->device);
+		c->inUse += want;
+		c->peak = std::max(c->peak, c->inUse);
+	}
+	const hipError_t e = hipMalloc(&b.p, want);
+	if (e != hipSuccess)
+	{
+		b.p = nullptr;
+		std::lock_guard<std::mutex> lock(c->memMu);
+		c->inUse -= want;
+	}
+	HIPCHECK(e);
 	b.bytes = want;
-	c->inUse += want;
-	c->peak = std::max(c->peak, c->inUse);
 	return b.p;
 }
 
@@ -173,15 +198,15 @@ T* ensureT(flyehip_ctx* c, DevBuf& b, size_t n)
 	return static_cast<T*>(ensure(c, b, n * sizeof(T)));
 }
 
-void upload(flyehip_ctx* c, void* dst, const void* src, size_t bytes)
+void upload(Lane* L, void* dst, const void* src, size_t bytes)
 {
-	if (bytes) HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+	if (bytes) HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, L->stream));
 }
-void download(flyehip_ctx* c, void* dst, const void* src, size_t bytes)
+void download(Lane* L, void* dst, const void* src, size_t bytes)
 {
-	if (bytes) HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+	if (bytes) HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, L->stream));
 }
-void sync(flyehip_ctx* c) { HIPCHECK(hipStreamSynchronize(c->stream)); }
+void sync(Lane* L) { HIPCHECK(hipStreamSynchronize(L->stream)); }
 
 int baseCode(uint8_t b)
 {
@@ -265,6 +290,7 @@ int autoRepMax(const std::vector<uint64_t>& hist, const uint64_t* acc, int solid
 
 void loadLogTable(flyehip_ctx* c)
 {
+	Lane* L = c->lanes[0].get();
 	const size_t bytes = static_cast<size_t>(fh_logf_table_end - fh_logf_table);
 	const size_t n = bytes / 8;
 	std::vector<uint32_t> xs(n), ys(n);
@@ -275,10 +301,10 @@ void loadLogTable(flyehip_ctx* c)
 		ys[i] = t[2 * i + 1];
 	}
 	if (n == 0) fail(FLYEHIP_ERR_STATE, "logf agreement table is empty (library built without it)");
-	upload(c, ensure(c, c->dLogX, n * 4), xs.data(), n * 4);
-	upload(c, ensure(c, c->dLogY, n * 4), ys.data(), n * 4);
+	upload(L, ensure(c, c->dLogX, n * 4), xs.data(), n * 4);
+	upload(L, ensure(c, c->dLogY, n * 4), ys.data(), n * 4);
 	c->logN = static_cast<int>(n);
-	sync(c);
+	sync(L);
 }
 
 __global__ void gatherAtKernel(const int64_t* src, const int64_t* idx, int64_t* dst, int32_t n)
@@ -322,17 +348,17 @@ __global__ void fillU64Kernel(uint64_t* p, int64_t n, uint64_t v)
 
 dim3 grid(int64_t n, int tpb) { return dim3(static_cast<unsigned>(std::max<int64_t>(1, (n + tpb - 1) / tpb))); }
 
-void zero32(flyehip_ctx* c, int32_t* p, int64_t n)
+void zero32(Lane* L, int32_t* p, int64_t n)
 {
 	if (n <= 0) return;
-	hipLaunchKernelGGL(zeroKernel, grid(n, 256), dim3(256), 0, c->stream, p, n);
+	hipLaunchKernelGGL(zeroKernel, grid(n, 256), dim3(256), 0, L->stream, p, n);
 	HIPCHECK(hipGetLastError());
 }
 
-void fill64(flyehip_ctx* c, uint64_t* p, int64_t n, uint64_t v)
+void fill64(Lane* L, uint64_t* p, int64_t n, uint64_t v)
 {
 	if (n <= 0) return;
-	hipLaunchKernelGGL(fillU64Kernel, grid(n, 256), dim3(256), 0, c->stream, p, n, v);
+	hipLaunchKernelGGL(fillU64Kernel, grid(n, 256), dim3(256), 0, L->stream, p, n, v);
 	HIPCHECK(hipGetLastError());
 }
 
@@ -384,17 +410,18 @@ struct BatchOut
 };
 
 // Join for a batch of queries; leaves matches (emission order) in bMKey/bMExt.
-BatchOut runJoin(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQuery>& qs, JoinArgs& ja)
+BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& qs, JoinArgs& ja)
 {
+	flyehip_ctx* c = L->c;
 	const int32_t nq = static_cast<int32_t>(qs.size());
 	const int k = c->k;
 	std::vector<int64_t> winOff(static_cast<size_t>(nq) + 1, 0);
 	for (int32_t i = 0; i < nq; ++i) winOff[static_cast<size_t>(i) + 1] = winOff[static_cast<size_t>(i)] + std::max(0, qs[static_cast<size_t>(i)].len - k + 1);
 	const int64_t nWin = winOff.back();
-	JoinQuery* dQ = ensureT<JoinQuery>(c, c->bQueries, qs.size());
-	int64_t* dWinOff = ensureT<int64_t>(c, c->bWinOff, winOff.size());
-	upload(c, dQ, qs.data(), qs.size() * sizeof(JoinQuery));
-	upload(c, dWinOff, winOff.data(), winOff.size() * sizeof(int64_t));
+	JoinQuery* dQ = ensureT<JoinQuery>(c, L->bQueries, qs.size());
+	int64_t* dWinOff = ensureT<int64_t>(c, L->bWinOff, winOff.size());
+	upload(L, dQ, qs.data(), qs.size() * sizeof(JoinQuery));
+	upload(L, dWinOff, winOff.data(), winOff.size() * sizeof(int64_t));
 	ja = JoinArgs{};
 	ja.seqs = seqs;
 	ja.queries = dQ;
@@ -405,45 +432,45 @@ BatchOut runJoin(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQ
 	ja.table = table(c);
 	ja.entries = static_cast<const uint64_t*>(c->dEntries.p);
 	ja.targetLen = static_cast<const int32_t*>(c->dReadLen.p);
-	ja.winCount = ensureT<int32_t>(c, c->bWinCount, static_cast<size_t>(nWin) + 1);
-	ja.winFilt = ensureT<int32_t>(c, c->bWinFilt, static_cast<size_t>(nWin) + 1);
-	ja.winInfo = ensureT<uint32_t>(c, c->bWinInfo, static_cast<size_t>(nWin) + 1);
-	int64_t* matchOff = ensureT<int64_t>(c, c->bMatchOff, static_cast<size_t>(nWin) + 1);
-	int64_t* filtOff = ensureT<int64_t>(c, c->bFiltOff, static_cast<size_t>(nWin) + 1);
-	int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(nWin) + 64);
+	ja.winCount = ensureT<int32_t>(c, L->bWinCount, static_cast<size_t>(nWin) + 1);
+	ja.winFilt = ensureT<int32_t>(c, L->bWinFilt, static_cast<size_t>(nWin) + 1);
+	ja.winInfo = ensureT<uint32_t>(c, L->bWinInfo, static_cast<size_t>(nWin) + 1);
+	int64_t* matchOff = ensureT<int64_t>(c, L->bMatchOff, static_cast<size_t>(nWin) + 1);
+	int64_t* filtOff = ensureT<int64_t>(c, L->bFiltOff, static_cast<size_t>(nWin) + 1);
+	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nWin) + 64);
 	ja.matchOff = matchOff;
 	ja.filtOff = filtOff;
-	HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-	HIPCHECK(launchJoinCount(ja, c->stream));
-	HIPCHECK(hipEventRecord(c->ev[1], c->stream));
-	HIPCHECK(exclusiveScan(ja.winCount, matchOff, nWin, tmp, c->stream));
-	HIPCHECK(exclusiveScan(ja.winFilt, filtOff, nWin, tmp, c->stream));
+	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
+	HIPCHECK(launchJoinCount(ja, L->stream));
+	HIPCHECK(hipEventRecord(L->ev[1], L->stream));
+	HIPCHECK(exclusiveScan(ja.winCount, matchOff, nWin, tmp, L->stream));
+	HIPCHECK(exclusiveScan(ja.winFilt, filtOff, nWin, tmp, L->stream));
 	// per-query offsets
-	int64_t* qMatchOff = ensureT<int64_t>(c, c->bQMatchOff, static_cast<size_t>(nq) + 1);
-	int64_t* qFiltOff = ensureT<int64_t>(c, c->bQFiltOff, static_cast<size_t>(nq) + 1);
-	hipLaunchKernelGGL(gatherAtKernel, grid(nq + 1, 256), dim3(256), 0, c->stream, matchOff, dWinOff, qMatchOff, nq);
-	hipLaunchKernelGGL(gatherAtKernel, grid(nq + 1, 256), dim3(256), 0, c->stream, filtOff, dWinOff, qFiltOff, nq);
+	int64_t* qMatchOff = ensureT<int64_t>(c, L->bQMatchOff, static_cast<size_t>(nq) + 1);
+	int64_t* qFiltOff = ensureT<int64_t>(c, L->bQFiltOff, static_cast<size_t>(nq) + 1);
+	hipLaunchKernelGGL(gatherAtKernel, grid(nq + 1, 256), dim3(256), 0, L->stream, matchOff, dWinOff, qMatchOff, nq);
+	hipLaunchKernelGGL(gatherAtKernel, grid(nq + 1, 256), dim3(256), 0, L->stream, filtOff, dWinOff, qFiltOff, nq);
 	HIPCHECK(hipGetLastError());
 	int64_t tot[2];
-	download(c, &tot[0], matchOff + nWin, sizeof(int64_t));
-	download(c, &tot[1], filtOff + nWin, sizeof(int64_t));
-	sync(c);
+	download(L, &tot[0], matchOff + nWin, sizeof(int64_t));
+	download(L, &tot[1], filtOff + nWin, sizeof(int64_t));
+	sync(L);
 	BatchOut bo;
 	bo.matches = tot[0];
 	bo.filtered = tot[1];
-	ja.mkey = ensureT<uint64_t>(c, c->bMKey, static_cast<size_t>(bo.matches) + 1);
-	ja.mext = ensureT<int32_t>(c, c->bMExt, static_cast<size_t>(bo.matches) + 1);
-	ja.filtered = ensureT<int32_t>(c, c->bFiltered, static_cast<size_t>(bo.filtered) + 1);
-	HIPCHECK(hipEventRecord(c->ev[6], c->stream));
-	HIPCHECK(launchJoinFill(ja, c->stream));
-	HIPCHECK(hipEventRecord(c->ev[7], c->stream));
-	HIPCHECK(hipEventSynchronize(c->ev[7]));
+	ja.mkey = ensureT<uint64_t>(c, L->bMKey, static_cast<size_t>(bo.matches) + 1);
+	ja.mext = ensureT<int32_t>(c, L->bMExt, static_cast<size_t>(bo.matches) + 1);
+	ja.filtered = ensureT<int32_t>(c, L->bFiltered, static_cast<size_t>(bo.filtered) + 1);
+	HIPCHECK(hipEventRecord(L->ev[6], L->stream));
+	HIPCHECK(launchJoinFill(ja, L->stream));
+	HIPCHECK(hipEventRecord(L->ev[7], L->stream));
+	HIPCHECK(hipEventSynchronize(L->ev[7]));
 	float e1 = 0, e2 = 0;
-	HIPCHECK(hipEventElapsedTime(&e1, c->ev[0], c->ev[1]));
-	HIPCHECK(hipEventElapsedTime(&e2, c->ev[6], c->ev[7]));
-	c->stats.ms_join += e1 + e2;
-	c->stats.windows += nWin;
-	c->stats.matches += bo.matches;
+	HIPCHECK(hipEventElapsedTime(&e1, L->ev[0], L->ev[1]));
+	HIPCHECK(hipEventElapsedTime(&e2, L->ev[6], L->ev[7]));
+	L->stats.ms_join += e1 + e2;
+	L->stats.windows += nWin;
+	L->stats.matches += bo.matches;
 	return bo;
 }
 
@@ -451,38 +478,42 @@ BatchOut runJoin(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQ
 // bSKey and the permutation (emission index within the query) in bVal.
 // Buffers of the segmented exact sort for n elements in nArrays arrays; the
 // caller sets key / val / outVal / dup / posL / posR.
-SegSortBufs segBufs(flyehip_ctx* c, int64_t n, int64_t nArrays)
+SegSortBufs segBufs(Lane* L, int64_t n, int64_t nArrays)
 {
+	flyehip_ctx* c = L->c;
 	SegSortCaps caps{};
 	HIPCHECK(segSortCaps(n, nArrays, &caps));
 	SegSortBufs b{};
-	b.big[0] = ensureT<SegRange>(c, c->bSegBig0, static_cast<size_t>(caps.bigCap));
-	b.big[1] = ensureT<SegRange>(c, c->bSegBig1, static_cast<size_t>(caps.bigCap));
-	b.leaf = ensureT<SegRange>(c, c->bSegLeaf, static_cast<size_t>(caps.leafCap));
-	b.heap = ensureT<SegRange>(c, c->bSegHeap, static_cast<size_t>(caps.heapCap));
-	b.verify = ensureT<SegRange>(c, c->bSegVerify, static_cast<size_t>(caps.heapCap));
-	b.shadowKey = ensureT<uint32_t>(c, c->bSegShadowK, static_cast<size_t>(n) + 1);
-	b.shadowVal = ensureT<uint32_t>(c, c->bSegShadowV, static_cast<size_t>(n) + 1);
-	b.counters = ensureT<int64_t>(c, c->bSegCounters, kSegCounters);
-	b.chunkOff = ensureT<int64_t>(c, c->bSegChunkOff, static_cast<size_t>(caps.bigCap) + 1);
-	b.chunkRange = ensureT<int32_t>(c, c->bSegChunkRange, static_cast<size_t>(caps.chunkCap));
-	b.chunkCnt = ensureT<uint64_t>(c, c->bSegChunkCnt, static_cast<size_t>(caps.chunkCap) + 1);
-	b.chunkPre = ensureT<uint64_t>(c, c->bSegChunkPre, static_cast<size_t>(std::max(caps.chunkCap, caps.bigCap)) + 1);
-	b.rangeM = ensureT<int32_t>(c, c->bSegRangeM, static_cast<size_t>(caps.bigCap));
-	b.rangeLm = ensureT<int32_t>(c, c->bSegRangeLm, static_cast<size_t>(caps.bigCap));
-	b.scanTemp = ensure(c, c->bSegTemp, caps.scanTempBytes + 256);
+	b.big[0] = ensureT<SegRange>(c, L->bSegBig0, static_cast<size_t>(caps.bigCap));
+	b.big[1] = ensureT<SegRange>(c, L->bSegBig1, static_cast<size_t>(caps.bigCap));
+	b.leaf = ensureT<SegRange>(c, L->bSegLeaf, static_cast<size_t>(caps.leafCap));
+	b.heap = ensureT<SegRange>(c, L->bSegHeap, static_cast<size_t>(caps.heapCap));
+	b.verify = ensureT<SegRange>(c, L->bSegVerify, static_cast<size_t>(caps.heapCap));
+	b.shadowKey = ensureT<uint32_t>(c, L->bSegShadowK, static_cast<size_t>(n) + 1);
+	b.shadowVal = ensureT<uint32_t>(c, L->bSegShadowV, static_cast<size_t>(n) + 1);
+	b.counters = ensureT<int64_t>(c, L->bSegCounters, kSegCounters);
+	b.chunkOff = ensureT<int64_t>(c, L->bSegChunkOff, static_cast<size_t>(caps.bigCap) + 1);
+	b.chunkRange = ensureT<int32_t>(c, L->bSegChunkRange, static_cast<size_t>(caps.chunkCap));
+	b.chunkCnt = ensureT<uint64_t>(c, L->bSegChunkCnt, static_cast<size_t>(caps.chunkCap) + 1);
+	b.chunkPre = ensureT<uint64_t>(c, L->bSegChunkPre, static_cast<size_t>(std::max(caps.chunkCap, caps.bigCap)) + 1);
+	b.rangeM = ensureT<int32_t>(c, L->bSegRangeM, static_cast<size_t>(caps.bigCap));
+	b.rangeLm = ensureT<int32_t>(c, L->bSegRangeLm, static_cast<size_t>(caps.bigCap));
+	b.scanTemp = ensure(c, L->bSegTemp, caps.scanTempBytes + 256);
 	b.scanTempBytes = caps.scanTempBytes;
 	b.bigCap = caps.bigCap;
 	b.leafCap = caps.leafCap;
 	b.heapCap = caps.heapCap;
 	b.chunkCap = caps.chunkCap;
-	b.hostCounters = c->hostSeg;
+	b.hostCounters = L->hostSeg;
+	for (int t = 0; t < 3; ++t) b.aux[t] = L->aux[t];
+	for (int t = 0; t < 4; ++t) b.auxEv[t] = L->auxEv[t];
 	return b;
 }
 
-void runSegSort(flyehip_ctx* c, SegSortBufs& b, int64_t* tieArrays)
+void runSegSort(Lane* L, SegSortBufs& b, int64_t* tieArrays)
 {
-	const hipError_t e = segSortRun(b, c->stream, tieArrays);
+	flyehip_ctx* c = L->c;
+	const hipError_t e = segSortRun(b, L->stream, tieArrays);
 	if (e == hipErrorIllegalState) fail(FLYEHIP_ERR_HIP, "segmented sort: inconsistent partition");
 	HIPCHECK(e);
 }
@@ -490,69 +521,71 @@ void runSegSort(flyehip_ctx* c, SegSortBufs& b, int64_t* tieArrays)
 // Exact std::sort of every query's matches: stable order, then the tie replay.
 // Leaves the sorted keys in bSKey and the permutation (emission index within
 // the query) in bVal.
-void runMatchSort(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches)
+void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 {
+	flyehip_ctx* c = L->c;
 	if (nMatches >= 0x7fffffffLL) fail(FLYEHIP_ERR_BUDGET, "batch of %lld matches exceeds 2^31 - 1", static_cast<long long>(nMatches));
 	const size_t nm = static_cast<size_t>(nMatches) + 1;
 	const int32_t nq = ja.nQueries;
 	MsortArgs ms{};
-	ms.qOff = static_cast<const int64_t*>(c->bQMatchOff.p);
+	ms.qOff = static_cast<const int64_t*>(L->bQMatchOff.p);
 	ms.nQueries = nq;
 	ms.akey = ja.mkey;
-	ms.aval = ensureT<uint32_t>(c, c->bAVal, nm);
-	ms.skey = ensureT<uint64_t>(c, c->bSKey, nm);
-	ms.sval = ensureT<uint32_t>(c, c->bVal, nm);
-	ms.rkey = ensureT<uint32_t>(c, c->bRKey, nm);
-	ms.dupFlag = ensureT<int32_t>(c, c->bDupFlag, nm);
-	int64_t* dup = ensureT<int64_t>(c, c->bDup, nm);
-	int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(nMatches) + 64);
+	ms.aval = ensureT<uint32_t>(c, L->bAVal, nm);
+	ms.skey = ensureT<uint64_t>(c, L->bSKey, nm);
+	ms.sval = ensureT<uint32_t>(c, L->bVal, nm);
+	ms.rkey = ensureT<uint32_t>(c, L->bRKey, nm);
+	ms.dupFlag = ensureT<int32_t>(c, L->bDupFlag, nm);
+	int64_t* dup = ensureT<int64_t>(c, L->bDup, nm);
+	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nMatches) + 64);
 	int bits = 1;
 	while ((int64_t{1} << bits) < 2 * static_cast<int64_t>(std::max(1, c->nReads))) ++bits;
 	size_t tempBytes = 0;
 	HIPCHECK(msortTempBytes(nMatches, nq, bits, &tempBytes));
-	void* temp = ensure(c, c->bMsortTemp, tempBytes + 256);
-	SegSortBufs sb = segBufs(c, nMatches, nq);
+	void* temp = ensure(c, L->bMsortTemp, tempBytes + 256);
+	SegSortBufs sb = segBufs(L, nMatches, nq);
 	sb.key = ms.rkey;
 	sb.val = ms.aval;
 	sb.outVal = ms.sval;
 	sb.dup = dup;
-	sb.posL = ensureT<int32_t>(c, c->bPosL, nm);
-	sb.posR = ensureT<int32_t>(c, c->bPosR, nm);
-	HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, temp, tempBytes, c->stream));
-	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, c->stream));
-	HIPCHECK(segSortReset(sb, c->stream));
-	HIPCHECK(segPushOffsets(sb, ms.qOff, nq, c->stream));
+	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
+	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
+	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
+	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, temp, tempBytes, L->stream));
+	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, L->stream));
+	HIPCHECK(segSortReset(sb, L->stream));
+	HIPCHECK(segPushOffsets(sb, ms.qOff, nq, L->stream));
 	int64_t ties = 0;
-	runSegSort(c, sb, &ties);
-	HIPCHECK(hipEventRecord(c->ev[1], c->stream));
-	HIPCHECK(hipEventSynchronize(c->ev[1]));
+	runSegSort(L, sb, &ties);
+	HIPCHECK(hipEventRecord(L->ev[1], L->stream));
+	HIPCHECK(hipEventSynchronize(L->ev[1]));
 	float es = 0;
-	HIPCHECK(hipEventElapsedTime(&es, c->ev[0], c->ev[1]));
-	c->stats.ms_sort += es;
-	c->stats.tie_queries += ties;
+	HIPCHECK(hipEventElapsedTime(&es, L->ev[0], L->ev[1]));
+	L->stats.ms_sort += es;
+	L->stats.tie_queries += ties;
 }
 
 // Order, prefilters + extPos sort, DP, score sort + walks over ca.nGroups
 // groups whose hits total nHits; leaves groupCount set.
-void runChainKernels(flyehip_ctx* c, ChainArgs& ca, int64_t nHits)
+void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 {
+	flyehip_ctx* c = L->c;
 	const int64_t ng = ca.nGroups;
-	zero32(c, ca.groupCount, ng + 1);
+	zero32(L, ca.groupCount, ng + 1);
 	if (ng == 0) return;
 	if (nHits >= 0x7fffffffLL) fail(FLYEHIP_ERR_BUDGET, "batch of %lld hits exceeds 2^31 - 1", static_cast<long long>(nHits));
-	int32_t* bins = ensureT<int32_t>(c, c->bBins, 96);
-	int32_t* order = ensureT<int32_t>(c, c->bOrder, static_cast<size_t>(ng) + 1);
-	int32_t* passOrder = ensureT<int32_t>(c, c->bPassOrder, static_cast<size_t>(ng) + 1);
-	int32_t* flag = ensureT<int32_t>(c, c->bGroupFlag, static_cast<size_t>(ng) + 1);
-	int64_t* pos = ensureT<int64_t>(c, c->bGroupOff, static_cast<size_t>(ng) + 1);
-	int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(std::max(ng, nHits)) + 64);
-	int32_t* queues = ensureT<int32_t>(c, c->bQueue, 16);
-	int64_t* nBig = ensureT<int64_t>(c, c->bCounters, 16);
-	zero32(c, bins, 96);
-	zero32(c, queues, 16);
-	HIPCHECK(hipMemsetAsync(nBig, 0, sizeof(int64_t), c->stream));
-	SegSortBufs sb = segBufs(c, nHits, ng);
+	int32_t* bins = ensureT<int32_t>(c, L->bBins, 96);
+	int32_t* order = ensureT<int32_t>(c, L->bOrder, static_cast<size_t>(ng) + 1);
+	int32_t* passOrder = ensureT<int32_t>(c, L->bPassOrder, static_cast<size_t>(ng) + 1);
+	int32_t* flag = ensureT<int32_t>(c, L->bGroupFlag, static_cast<size_t>(ng) + 1);
+	int64_t* pos = ensureT<int64_t>(c, L->bGroupOff, static_cast<size_t>(ng) + 1);
+	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(std::max(ng, nHits)) + 64);
+	int32_t* queues = ensureT<int32_t>(c, L->bQueue, 16);
+	int64_t* nBig = ensureT<int64_t>(c, L->bCounters, 16);
+	zero32(L, bins, 96);
+	zero32(L, queues, 16);
+	HIPCHECK(hipMemsetAsync(nBig, 0, sizeof(int64_t), L->stream));
+	SegSortBufs sb = segBufs(L, nHits, ng);
 	sb.key = ca.sortKey;
 	sb.val = ca.sortVal;
 	sb.outVal = ca.sortVal;
@@ -561,82 +594,84 @@ void runChainKernels(flyehip_ctx* c, ChainArgs& ca, int64_t nHits)
 	sb.posR = ca.posR;
 	const int blocks = chainBlocks(c);
 
-	HIPCHECK(hipEventRecord(c->ev[2], c->stream));
-	HIPCHECK(launchGroupOrder(ca.groups, ca.nGroups, bins, order, c->stream));
+	HIPCHECK(hipEventRecord(L->ev[2], L->stream));
+	HIPCHECK(launchGroupOrder(ca.groups, ca.nGroups, bins, order, L->stream));
 	ca.order = order;
-	HIPCHECK(segSortReset(sb, c->stream));
+	HIPCHECK(segSortReset(sb, L->stream));
 	ca.queue = queues + 0;
-	HIPCHECK(launchChainPrefilter(ca, sb, blocks, c->stream));
-	runSegSort(c, sb, nullptr);
-	HIPCHECK(launchPassFlags(ca, flag, c->stream));
-	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, c->stream));
-	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, c->stream));
+	HIPCHECK(launchChainPrefilter(ca, sb, blocks, L->stream));
+	runSegSort(L, sb, nullptr);
+	HIPCHECK(launchPassFlags(ca, flag, L->stream));
+	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
+	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream));
 	int64_t counts[2] = {0, 0};
-	download(c, &counts[0], pos + ng, sizeof(int64_t));
-	download(c, &counts[1], nBig, sizeof(int64_t));
-	sync(c);
+	download(L, &counts[0], pos + ng, sizeof(int64_t));
+	download(L, &counts[1], nBig, sizeof(int64_t));
+	sync(L);
 	ca.passOrder = passOrder;
 	ca.nPass = static_cast<int32_t>(counts[0]);
 	const int32_t nBigPass = static_cast<int32_t>(counts[1]);
 	ca.queue = queues + 1;
-	HIPCHECK(launchChainGather(ca, blocks, c->stream));
-	HIPCHECK(hipEventRecord(c->ev[3], c->stream));
+	HIPCHECK(launchChainGather(ca, blocks, L->stream));
+	HIPCHECK(hipEventRecord(L->ev[3], L->stream));
 	ca.queue = queues + 2;
-	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, c->stream));
-	HIPCHECK(hipEventRecord(c->ev[4], c->stream));
-	HIPCHECK(segSortReset(sb, c->stream));
+	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, L->stream));
+	HIPCHECK(hipEventRecord(L->ev[4], L->stream));
+	HIPCHECK(segSortReset(sb, L->stream));
+	sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
 	ca.queue = queues + 3;
-	HIPCHECK(launchChainScoreKeys(ca, sb, blocks, c->stream));
-	runSegSort(c, sb, nullptr);
+	HIPCHECK(launchChainScoreKeys(ca, sb, blocks, L->stream));
+	runSegSort(L, sb, nullptr);
 	ca.queue = queues + 4;
-	HIPCHECK(launchChainWalk(ca, 0, nBigPass, true, blocks, c->stream));
+	HIPCHECK(launchChainWalk(ca, 0, nBigPass, true, blocks, L->stream));
 	ca.queue = queues + 5;
-	HIPCHECK(launchChainWalk(ca, nBigPass, ca.nPass, false, blocks, c->stream));
-	HIPCHECK(hipEventRecord(c->ev[5], c->stream));
-	HIPCHECK(hipEventSynchronize(c->ev[5]));
+	HIPCHECK(launchChainWalk(ca, nBigPass, ca.nPass, false, blocks, L->stream));
+	HIPCHECK(hipEventRecord(L->ev[5], L->stream));
+	HIPCHECK(hipEventSynchronize(L->ev[5]));
 	float a = 0, b = 0, d = 0;
-	HIPCHECK(hipEventElapsedTime(&a, c->ev[2], c->ev[3]));
-	HIPCHECK(hipEventElapsedTime(&b, c->ev[3], c->ev[4]));
-	HIPCHECK(hipEventElapsedTime(&d, c->ev[4], c->ev[5]));
-	c->stats.ms_group += a;
-	c->stats.ms_chain += b;
-	c->stats.ms_extract += d;
+	HIPCHECK(hipEventElapsedTime(&a, L->ev[2], L->ev[3]));
+	HIPCHECK(hipEventElapsedTime(&b, L->ev[3], L->ev[4]));
+	HIPCHECK(hipEventElapsedTime(&d, L->ev[4], L->ev[5]));
+	L->stats.ms_group += a;
+	L->stats.ms_chain += b;
+	L->stats.ms_extract += d;
 }
 
 // Groups + chaining for a batch whose sorted matches sit in bMKey/bVal.
 // Returns rows on the device in bRows and per-query row counts in bQRowCount.
-BatchOut runChain(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches, const std::vector<int64_t>& qIds,
+BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vector<int64_t>& qIds,
 				  const std::vector<int32_t>& qLens, const flyehip_overlap_params_t& p, double* msGroup, double* msChain)
 {
+	flyehip_ctx* c = L->c;
 	BatchOut bo;
 	const int32_t nq = ja.nQueries;
 	Timer tg;
-	int32_t* hcur = ensureT<int32_t>(c, c->bHCur, static_cast<size_t>(nMatches) + 1);
-	int32_t* hext = ensureT<int32_t>(c, c->bHExt, static_cast<size_t>(nMatches) + 1);
-	int32_t* gflag = ensureT<int32_t>(c, c->bGroupFlag, static_cast<size_t>(nMatches) + 1);
-	int64_t* goff = ensureT<int64_t>(c, c->bGroupOff, static_cast<size_t>(nMatches) + 1);
-	int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(nMatches) + 64);
-	const int64_t* qMatchOff = static_cast<const int64_t*>(c->bQMatchOff.p);
-	const uint64_t* skey = static_cast<const uint64_t*>(c->bSKey.p);
-	HIPCHECK(launchGatherHits(qMatchOff, nq, skey, static_cast<const uint32_t*>(c->bVal.p), ja.mext, hcur, hext, gflag,
-							  nMatches, c->stream));
-	HIPCHECK(exclusiveScan(gflag, goff, nMatches, tmp, c->stream));
+	int32_t* hcur = ensureT<int32_t>(c, L->bHCur, static_cast<size_t>(nMatches) + 1);
+	int32_t* hext = ensureT<int32_t>(c, L->bHExt, static_cast<size_t>(nMatches) + 1);
+	int32_t* gflag = ensureT<int32_t>(c, L->bGroupFlag, static_cast<size_t>(nMatches) + 1);
+	int64_t* goff = ensureT<int64_t>(c, L->bGroupOff, static_cast<size_t>(nMatches) + 1);
+	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nMatches) + 64);
+	const int64_t* qMatchOff = static_cast<const int64_t*>(L->bQMatchOff.p);
+	const uint64_t* skey = static_cast<const uint64_t*>(L->bSKey.p);
+	HIPCHECK(launchGatherHits(qMatchOff, nq, skey, static_cast<const uint32_t*>(L->bVal.p), ja.mext, hcur, hext, gflag,
+							  nMatches, L->stream));
+	HIPCHECK(exclusiveScan(gflag, goff, nMatches, tmp, L->stream));
 	int64_t nGroups = 0;
-	download(c, &nGroups, goff + nMatches, sizeof(int64_t));
-	sync(c);
-	GroupDesc* groups = ensureT<GroupDesc>(c, c->bGroups, static_cast<size_t>(nGroups) + 1);
-	HIPCHECK(launchBuildGroups(qMatchOff, nq, skey, gflag, goff, ja.targetLen, groups, nMatches, c->stream));
-	HIPCHECK(launchGroupSizes(groups, nGroups, nMatches, c->stream));
+	download(L, &nGroups, goff + nMatches, sizeof(int64_t));
+	sync(L);
+	GroupDesc* groups = ensureT<GroupDesc>(c, L->bGroups, static_cast<size_t>(nGroups) + 1);
+	HIPCHECK(launchBuildGroups(qMatchOff, nq, skey, gflag, goff, ja.targetLen, groups, nMatches, L->stream));
+	HIPCHECK(launchGroupSizes(groups, nGroups, nMatches, L->stream));
 	// query descriptors
-	int64_t* dIds = ensureT<int64_t>(c, c->bQueryIds, static_cast<size_t>(nq) + 1);
-	int32_t* dLens = ensureT<int32_t>(c, c->bCurLen, static_cast<size_t>(nq) + 1);
-	upload(c, dIds, qIds.data(), qIds.size() * sizeof(int64_t));
-	upload(c, dLens, qLens.data(), qLens.size() * sizeof(int32_t));
-	QueryDesc* qd = ensureT<QueryDesc>(c, c->bQDesc, static_cast<size_t>(nq) + 1);
-	hipLaunchKernelGGL(makeQueryDescKernel, grid(nq, 256), dim3(256), 0, c->stream, dIds, dLens,
-					   static_cast<const int64_t*>(c->bQFiltOff.p), nq, qd);
+	int64_t* dIds = ensureT<int64_t>(c, L->bQueryIds, static_cast<size_t>(nq) + 1);
+	int32_t* dLens = ensureT<int32_t>(c, L->bCurLen, static_cast<size_t>(nq) + 1);
+	upload(L, dIds, qIds.data(), qIds.size() * sizeof(int64_t));
+	upload(L, dLens, qLens.data(), qLens.size() * sizeof(int32_t));
+	QueryDesc* qd = ensureT<QueryDesc>(c, L->bQDesc, static_cast<size_t>(nq) + 1);
+	hipLaunchKernelGGL(makeQueryDescKernel, grid(nq, 256), dim3(256), 0, L->stream, dIds, dLens,
+					   static_cast<const int64_t*>(L->bQFiltOff.p), nq, qd);
 	HIPCHECK(hipGetLastError());
-	sync(c);
+	sync(L);
 	(void)msGroup;
 	(void)tg;
 
@@ -652,49 +687,109 @@ BatchOut runChain(flyehip_ctx* c, const JoinArgs& ja, int64_t nMatches, const st
 	ca.ext = hext;
 	ca.filtered = ja.filtered;
 	const size_t nm = static_cast<size_t>(nMatches) + 1;
-	ca.sortKey = ensureT<uint32_t>(c, c->bCKey, nm);
-	ca.sortVal = ensureT<uint32_t>(c, c->bCVal, nm);
-	ca.posL = ensureT<int32_t>(c, c->bCPosL, nm);
-	ca.posR = ensureT<int32_t>(c, c->bCPosR, nm);
-	ca.hcur = ensureT<int32_t>(c, c->bCHCur, nm);
-	ca.hext = ensureT<int32_t>(c, c->bCHExt, nm);
-	ca.score = ensureT<int32_t>(c, c->bCScore, nm);
-	ca.bt = ensureT<int32_t>(c, c->bCBt, nm);
-	ca.props = ensureT<Proposal>(c, c->bCProps, nm);
-	ca.outProps = ensureT<Proposal>(c, c->bCOut, nm);
-	ca.groupCount = ensureT<int32_t>(c, c->bGroupCount, static_cast<size_t>(nGroups) + 1);
-	ca.groupPass = ensureT<int32_t>(c, c->bGroupPass, static_cast<size_t>(nGroups) + 1);
+	ca.sortKey = ensureT<uint32_t>(c, L->bCKey, nm);
+	ca.sortVal = ensureT<uint32_t>(c, L->bCVal, nm);
+	ca.posL = ensureT<int32_t>(c, L->bCPosL, nm);
+	ca.posR = ensureT<int32_t>(c, L->bCPosR, nm);
+	ca.hcur = ensureT<int32_t>(c, L->bCHCur, nm);
+	ca.hext = ensureT<int32_t>(c, L->bCHExt, nm);
+	ca.score = ensureT<int32_t>(c, L->bCScore, nm);
+	ca.bt = ensureT<int32_t>(c, L->bCBt, nm);
+	ca.props = ensureT<Proposal>(c, L->bCProps, nm);
+	ca.outProps = ensureT<Proposal>(c, L->bCOut, nm);
+	ca.groupCount = ensureT<int32_t>(c, L->bGroupCount, static_cast<size_t>(nGroups) + 1);
+	ca.groupPass = ensureT<int32_t>(c, L->bGroupPass, static_cast<size_t>(nGroups) + 1);
 	ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
 	ca.logY = static_cast<const uint32_t*>(c->dLogY.p);
-	runChainKernels(c, ca, nMatches);
-	int64_t* rowStart = ensureT<int64_t>(c, c->bRowStart, static_cast<size_t>(nGroups) + 1);
-	HIPCHECK(exclusiveScan(ca.groupCount, rowStart, nGroups, tmp, c->stream));
+	runChainKernels(L, ca, nMatches);
+	int64_t* rowStart = ensureT<int64_t>(c, L->bRowStart, static_cast<size_t>(nGroups) + 1);
+	HIPCHECK(exclusiveScan(ca.groupCount, rowStart, nGroups, tmp, L->stream));
 	int64_t nRows = 0;
-	download(c, &nRows, rowStart + nGroups, sizeof(int64_t));
-	sync(c);
-	flyehip_overlap_t* rows = ensureT<flyehip_overlap_t>(c, c->bRows, static_cast<size_t>(nRows) + 1);
-	HIPCHECK(launchWriteRows(ca, rowStart, rows, c->stream));
-	int32_t* qCount = ensureT<int32_t>(c, c->bQRowCount, static_cast<size_t>(nq) + 1);
-	zero32(c, qCount, nq + 1);
+	download(L, &nRows, rowStart + nGroups, sizeof(int64_t));
+	sync(L);
+	flyehip_overlap_t* rows = ensureT<flyehip_overlap_t>(c, L->bRows, static_cast<size_t>(nRows) + 1);
+	HIPCHECK(launchWriteRows(ca, rowStart, rows, L->stream));
+	int32_t* qCount = ensureT<int32_t>(c, L->bQRowCount, static_cast<size_t>(nq) + 1);
+	zero32(L, qCount, nq + 1);
 	if (nGroups > 0)
 	{
-		hipLaunchKernelGGL(queryRowCountKernel, grid(nGroups, 256), dim3(256), 0, c->stream, groups, ca.groupCount,
+		hipLaunchKernelGGL(queryRowCountKernel, grid(nGroups, 256), dim3(256), 0, L->stream, groups, ca.groupCount,
 						   static_cast<int32_t>(nGroups), qCount);
 		HIPCHECK(hipGetLastError());
 	}
-	sync(c);
+	sync(L);
 	(void)msChain;
 	(void)tc;
 	bo.groups = nGroups;
 	bo.rows = nRows;
-	c->stats.groups += nGroups;
-	c->stats.overlaps += nRows;
+	L->stats.groups += nGroups;
+	L->stats.overlaps += nRows;
 	return bo;
 }
 
 void resetStats(flyehip_ctx* c)
 {
 	c->stats = flyehip_stats_t{};
+	for (auto& l : c->lanes) l->stats = flyehip_stats_t{};
+}
+
+// Adds the lanes' stage timings and counts into the context's stats.
+void foldStats(flyehip_ctx* c)
+{
+	for (auto& l : c->lanes)
+	{
+		const flyehip_stats_t& s = l->stats;
+		c->stats.ms_join += s.ms_join;
+		c->stats.ms_sort += s.ms_sort;
+		c->stats.ms_group += s.ms_group;
+		c->stats.ms_chain += s.ms_chain;
+		c->stats.ms_extract += s.ms_extract;
+		c->stats.windows += s.windows;
+		c->stats.matches += s.matches;
+		c->stats.groups += s.groups;
+		c->stats.overlaps += s.overlaps;
+		c->stats.batches += s.batches;
+		c->stats.tie_queries += s.tie_queries;
+		l->stats = flyehip_stats_t{};
+	}
+	c->stats.device_bytes_peak = static_cast<int64_t>(c->peak);
+}
+
+void createLane(flyehip_ctx* c)
+{
+	auto l = std::make_unique<Lane>();
+	l->c = c;
+	HIPCHECK(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
+	for (auto& e : l->ev) HIPCHECK(hipEventCreate(&e));
+	for (auto& a : l->aux) HIPCHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+	for (auto& e : l->auxEv) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+	HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&l->hostSeg), kSegCounters * sizeof(int64_t)));
+	c->lanes.push_back(std::move(l));
+}
+
+void destroyLane(flyehip_ctx* c, Lane& l)
+{
+	if (l.stream) (void)hipStreamSynchronize(l.stream);
+	for (DevBuf* b : {&l.bQueries, &l.bWinOff, &l.bWinCount, &l.bWinFilt, &l.bWinInfo, &l.bMatchOff, &l.bFiltOff,
+					  &l.bScanTmp, &l.bMKey, &l.bMExt, &l.bVal, &l.bPosL, &l.bPosR, &l.bHCur, &l.bHExt, &l.bGroupFlag,
+					  &l.bGroupOff, &l.bFiltered, &l.bQMatchOff, &l.bQFiltOff, &l.bQDesc, &l.bGroups, &l.bGroupCount,
+					  &l.bRowStart, &l.bRows, &l.bCScore, &l.bCBt, &l.bCProps, &l.bCOut, &l.bCHCur, &l.bCHExt, &l.bCKey,
+					  &l.bCVal, &l.bCPosL, &l.bCPosR, &l.bGroupPass, &l.bOrder, &l.bPassOrder, &l.bBins, &l.bSKey,
+					  &l.bAVal, &l.bRKey, &l.bDupFlag, &l.bDup, &l.bMsortTemp, &l.bSegBig0, &l.bSegBig1, &l.bSegLeaf,
+					  &l.bSegHeap, &l.bSegCounters, &l.bSegChunkOff, &l.bSegChunkRange, &l.bSegChunkCnt,
+					  &l.bSegChunkPre, &l.bSegRangeM, &l.bSegRangeLm, &l.bSegTemp, &l.bSegVerify, &l.bSegShadowK,
+					  &l.bSegShadowV, &l.bQueue, &l.bCounters, &l.bQRowCount, &l.bCurLen, &l.bDivOff, &l.bDivCount,
+					  &l.bDivOut, &l.bQueryIds, &l.bCand, &l.bExtWords, &l.bExtWordOff, &l.bExtLen, &l.bAln,
+					  &l.bAlnOff, &l.bRA})
+		release(c, *b);
+	if (l.hostSeg) (void)hipHostFree(l.hostSeg);
+	for (auto& e : l.ev)
+		if (e) (void)hipEventDestroy(e);
+	for (auto& e : l.auxEv)
+		if (e) (void)hipEventDestroy(e);
+	for (auto& a : l.aux)
+		if (a) (void)hipStreamDestroy(a);
+	if (l.stream) (void)hipStreamDestroy(l.stream);
 }
 
 template <class F>
@@ -706,6 +801,7 @@ int guarded(flyehip_ctx* c, F f)
 	{
 		HIPCHECK(hipSetDevice(c->device));
 		f();
+		foldStats(c);
 		c->err.clear();
 		return FLYEHIP_OK;
 	}
@@ -721,95 +817,149 @@ int guarded(flyehip_ctx* c, F f)
 	}
 }
 
+// Host copy of one batch's results, in query order.
+struct BatchResult
+{
+	std::vector<flyehip_overlap_t> rows;
+	std::vector<int32_t> count;     // rows per query
+	std::vector<int32_t> divCount;  // divStats windows per query
+	std::vector<float> divs;
+};
+
+// One batch allQ[b0, b1) on a lane: join, exact match sort, chaining; rows
+// (and divStats windows, computed on the device) copied to the host.
+void runBatch(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& allQ, const int64_t* queryIds,
+			  const flyehip_overlap_params_t& p, int32_t b0, int32_t b1, bool withDiv, BatchResult& out)
+{
+	flyehip_ctx* c = L->c;
+	std::vector<JoinQuery> qs(allQ.begin() + b0, allQ.begin() + b1);
+	std::vector<int64_t> ids(queryIds + b0, queryIds + b1);
+	std::vector<int32_t> lens;
+	for (auto& q : qs) lens.push_back(q.len);
+	JoinArgs ja;
+	BatchOut jo = runJoin(L, seqs, qs, ja);
+	sync(L);
+	runMatchSort(L, ja, jo.matches);
+	BatchOut co = runChain(L, ja, jo.matches, ids, lens, p, nullptr, nullptr);
+	const int32_t nq = b1 - b0;
+	out.count.assign(static_cast<size_t>(nq), 0);
+	out.rows.resize(static_cast<size_t>(co.rows));
+	download(L, out.count.data(), L->bQRowCount.p, static_cast<size_t>(nq) * sizeof(int32_t));
+	if (co.rows) download(L, out.rows.data(), L->bRows.p, static_cast<size_t>(co.rows) * sizeof(flyehip_overlap_t));
+	sync(L);
+	if (withDiv)
+	{
+		// divStats windows, computed on the device from the batch rows
+		std::vector<int64_t> dOff(static_cast<size_t>(nq) + 1, 0);
+		std::vector<int64_t> qRowOff(static_cast<size_t>(nq) + 1, 0);
+		for (int32_t i = 0; i < nq; ++i)
+		{
+			dOff[static_cast<size_t>(i) + 1] = dOff[static_cast<size_t>(i)] + lens[static_cast<size_t>(i)] / 10000 + 1;
+			qRowOff[static_cast<size_t>(i) + 1] = qRowOff[static_cast<size_t>(i)] + out.count[static_cast<size_t>(i)];
+		}
+		int64_t* dDivOff = ensureT<int64_t>(c, L->bDivOff, dOff.size());
+		int64_t* dQRowOff = ensureT<int64_t>(c, L->bRowStart, std::max(qRowOff.size(), static_cast<size_t>(co.groups) + 1));
+		int32_t* dDivCount = ensureT<int32_t>(c, L->bDivCount, static_cast<size_t>(nq) + 1);
+		float* dDivOut = ensureT<float>(c, L->bDivOut, static_cast<size_t>(dOff.back()) + 1);
+		upload(L, dDivOff, dOff.data(), dOff.size() * sizeof(int64_t));
+		upload(L, dQRowOff, qRowOff.data(), qRowOff.size() * sizeof(int64_t));
+		HIPCHECK(launchDivStats(static_cast<const flyehip_overlap_t*>(L->bRows.p), dQRowOff,
+								static_cast<const int32_t*>(L->bCurLen.p), nq, dDivCount, dDivOut, dDivOff, L->stream));
+		out.divCount.assign(static_cast<size_t>(nq), 0);
+		std::vector<float> dv(static_cast<size_t>(dOff.back()) + 1);
+		download(L, out.divCount.data(), dDivCount, static_cast<size_t>(nq) * sizeof(int32_t));
+		download(L, dv.data(), dDivOut, static_cast<size_t>(dOff.back()) * sizeof(float));
+		sync(L);
+		for (int32_t i = 0; i < nq; ++i)
+			for (int32_t t = 0; t < out.divCount[static_cast<size_t>(i)]; ++t)
+				out.divs.push_back(dv[static_cast<size_t>(dOff[static_cast<size_t>(i)] + t)]);
+	}
+	L->stats.batches += 1;
+}
+
 // Batched overlap pipeline over JoinQuery lists (resident or external).
+// Batches are dealt to up to batchesInFlight lanes, each driven by its own
+// host thread on its own stream, so one batch's long tails (the last groups
+// of a kernel, a heap-sort fallback) overlap the next batch's work; results
+// are assembled in query order afterwards.
 void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQuery>& allQ,
 				  const int64_t* queryIds, const flyehip_overlap_params_t* p, flyehip_overlap_t* rows,
 				  int64_t rowCap, int64_t* rowOffsets, int64_t* rowsNeeded, float* divStats, int64_t divCap,
-				  int64_t* divOffsets, int batchQueries)
+				  int64_t* divOffsets, int batchQueries, int maxLanes)
 {
 	checkParams(p, c->k);
 	resetStats(c);
 	Timer total;
 	const int32_t n = static_cast<int32_t>(allQ.size());
-	int64_t written = 0;
-	int64_t divWritten = 0;
+	const int32_t nBatches = (n + batchQueries - 1) / batchQueries;
+	const int nl = std::max(1, std::min(std::min(c->batchesInFlight, maxLanes), nBatches));
+	while (static_cast<int>(c->lanes.size()) < nl) createLane(c);
+	std::vector<BatchResult> res(static_cast<size_t>(nBatches));
+	std::atomic<int32_t> next{0};
+	std::vector<std::exception_ptr> errs(static_cast<size_t>(nl));
+	auto worker = [&](int li) {
+		try
+		{
+			HIPCHECK(hipSetDevice(c->device));
+			Lane* L = c->lanes[static_cast<size_t>(li)].get();
+			for (;;)
+			{
+				const int32_t b = next.fetch_add(1);
+				if (b >= nBatches) break;
+				const int32_t b0 = b * batchQueries;
+				runBatch(L, seqs, allQ, queryIds, *p, b0, std::min(n, b0 + batchQueries), divOffsets != nullptr,
+						 res[static_cast<size_t>(b)]);
+			}
+		}
+		catch (...)
+		{
+			errs[static_cast<size_t>(li)] = std::current_exception();
+			next = nBatches;
+		}
+	};
+	std::vector<std::thread> th;
+	for (int li = 1; li < nl; ++li) th.emplace_back(worker, li);
+	worker(0);
+	for (auto& t : th) t.join();
+	for (auto& e : errs)
+		if (e) std::rethrow_exception(e);
+
+	int64_t written = 0, divWritten = 0;
+	bool capacityShort = false;
 	rowOffsets[0] = 0;
 	if (divOffsets) divOffsets[0] = 0;
-	bool capacityShort = false;
-	std::vector<flyehip_overlap_t> hostRows;
-	std::vector<int32_t> hostCount;
-	for (int32_t b0 = 0; b0 < n; b0 += batchQueries)
+	for (int32_t b = 0; b < nBatches; ++b)
 	{
-		const int32_t b1 = std::min(n, b0 + batchQueries);
-		std::vector<JoinQuery> qs(allQ.begin() + b0, allQ.begin() + b1);
-		std::vector<int64_t> ids(queryIds + b0, queryIds + b1);
-		std::vector<int32_t> lens;
-		for (auto& q : qs) lens.push_back(q.len);
-		JoinArgs ja;
-		BatchOut jo = runJoin(c, seqs, qs, ja);
-		sync(c);
-
-		runMatchSort(c, ja, jo.matches);
-
-		BatchOut co = runChain(c, ja, jo.matches, ids, lens, *p, &c->stats.ms_group, &c->stats.ms_chain);
-		Timer te;
-		const int32_t nq = b1 - b0;
-		hostCount.assign(static_cast<size_t>(nq) + 1, 0);
-		download(c, hostCount.data(), c->bQRowCount.p, static_cast<size_t>(nq) * sizeof(int32_t));
-		if (written + co.rows <= rowCap)
+		const BatchResult& r = res[static_cast<size_t>(b)];
+		const int32_t b0 = b * batchQueries;
+		const int64_t nr = static_cast<int64_t>(r.rows.size());
+		if (written + nr <= rowCap)
 		{
-			download(c, rows + written, c->bRows.p, static_cast<size_t>(co.rows) * sizeof(flyehip_overlap_t));
+			if (nr) std::memcpy(rows + written, r.rows.data(), static_cast<size_t>(nr) * sizeof(flyehip_overlap_t));
 		}
 		else
 		{
 			capacityShort = true;
 		}
-		if (divOffsets)
+		size_t d = 0;
+		for (size_t i = 0; i < r.count.size(); ++i)
 		{
-			// divStats windows, computed on the device from the batch rows
-			std::vector<int64_t> dOff(static_cast<size_t>(nq) + 1, 0);
-			std::vector<int64_t> qRowOff(static_cast<size_t>(nq) + 1, 0);
-			for (int32_t i = 0; i < nq; ++i)
+			written += r.count[i];
+			rowOffsets[b0 + static_cast<int64_t>(i) + 1] = written;
+			if (divOffsets)
 			{
-				dOff[static_cast<size_t>(i) + 1] = dOff[static_cast<size_t>(i)] + lens[static_cast<size_t>(i)] / 10000 + 1;
-			}
-			sync(c);
-			for (int32_t i = 0; i < nq; ++i) qRowOff[static_cast<size_t>(i) + 1] = qRowOff[static_cast<size_t>(i)] + hostCount[static_cast<size_t>(i)];
-			int64_t* dDivOff = ensureT<int64_t>(c, c->bDivOff, dOff.size());
-			int64_t* dQRowOff = ensureT<int64_t>(c, c->bRowStart, std::max(qRowOff.size(), static_cast<size_t>(co.groups) + 1));
-			int32_t* dDivCount = ensureT<int32_t>(c, c->bDivCount, static_cast<size_t>(nq) + 1);
-			float* dDivOut = ensureT<float>(c, c->bDivOut, static_cast<size_t>(dOff.back()) + 1);
-			upload(c, dDivOff, dOff.data(), dOff.size() * sizeof(int64_t));
-			upload(c, dQRowOff, qRowOff.data(), qRowOff.size() * sizeof(int64_t));
-			HIPCHECK(launchDivStats(static_cast<const flyehip_overlap_t*>(c->bRows.p), dQRowOff,
-									static_cast<const int32_t*>(c->bCurLen.p), nq, dDivCount, dDivOut, dDivOff, c->stream));
-			std::vector<int32_t> dc(static_cast<size_t>(nq));
-			std::vector<float> dv(static_cast<size_t>(dOff.back()) + 1);
-			download(c, dc.data(), dDivCount, static_cast<size_t>(nq) * sizeof(int32_t));
-			download(c, dv.data(), dDivOut, static_cast<size_t>(dOff.back()) * sizeof(float));
-			sync(c);
-			for (int32_t i = 0; i < nq; ++i)
-			{
-				for (int32_t t = 0; t < dc[static_cast<size_t>(i)]; ++t)
+				for (int32_t t = 0; t < r.divCount[i]; ++t, ++d)
 				{
-					if (divStats && divWritten < divCap) divStats[divWritten] = dv[static_cast<size_t>(dOff[static_cast<size_t>(i)] + t)];
+					if (divStats && divWritten < divCap) divStats[divWritten] = r.divs[d];
 					++divWritten;
 				}
-				divOffsets[b0 + i + 1] = divWritten;
+				divOffsets[b0 + static_cast<int64_t>(i) + 1] = divWritten;
 			}
 		}
-		sync(c);
-		for (int32_t i = 0; i < nq; ++i)
-		{
-			written += hostCount[static_cast<size_t>(i)];
-			rowOffsets[b0 + i + 1] = written;
-		}
-		(void)te;
-		c->stats.batches += 1;
 	}
+	foldStats(c);
 	c->stats.queries = n;
 	c->stats.ms_total = total.ms();
-	c->stats.device_bytes_peak = static_cast<int64_t>(c->peak);
 	if (rowsNeeded) *rowsNeeded = written;
 	if (capacityShort) fail(FLYEHIP_ERR_CAPACITY, "row capacity %lld too small: %lld rows needed",
 							static_cast<long long>(rowCap), static_cast<long long>(written));
@@ -862,9 +1012,7 @@ int flyehip_open(int device, uint64_t memory_budget_bytes, flyehip_ctx** out_ctx
 		size_t freeB = 0, totalB = 0;
 		HIPCHECK(hipMemGetInfo(&freeB, &totalB));
 		c->budget = memory_budget_bytes ? memory_budget_bytes : static_cast<uint64_t>(static_cast<double>(freeB) * 0.9);
-		HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-		for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
-		HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&c->hostSeg), fh::kSegCounters * sizeof(int64_t)));
+		createLane(c);
 		loadLogTable(c);
 	}
 	catch (const FhError& e)
@@ -882,26 +1030,11 @@ int flyehip_open(int device, uint64_t memory_budget_bytes, flyehip_ctx** out_ctx
 int flyehip_close(flyehip_ctx* c)
 {
 	if (!c) return FLYEHIP_OK;
-	hipSetDevice(c->device);
-	if (c->stream) hipStreamSynchronize(c->stream);
-	DevBuf* all[] = {&c->dWords, &c->dWordOff, &c->dReadLen, &c->dReadWinOff, &c->dKeys, &c->dVals, &c->dCursor,
-					 &c->dEntries, &c->dLogX, &c->dLogY, &c->bQueries, &c->bWinOff, &c->bWinCount, &c->bWinFilt,
-					 &c->bWinInfo, &c->bMatchOff, &c->bFiltOff, &c->bScanTmp, &c->bMKey, &c->bMExt, &c->bVal,
-					 &c->bPosL, &c->bPosR, &c->bHCur, &c->bHExt, &c->bGroupFlag, &c->bGroupOff,
-					 &c->bFiltered, &c->bQMatchOff, &c->bQFiltOff, &c->bQDesc, &c->bGroups, &c->bGroupCount,
-					 &c->bRowStart, &c->bRows, &c->bCScore, &c->bCBt, &c->bCProps, &c->bCOut, &c->bCHCur,
-					 &c->bCHExt, &c->bCKey, &c->bCVal, &c->bCPosL, &c->bCPosR, &c->bGroupPass, &c->bOrder, &c->bPassOrder, &c->bBins, &c->bSKey, &c->bAVal, &c->bRKey, &c->bDupFlag, &c->bDup, &c->bMsortTemp, &c->bQueue,
-					 &c->bSegBig0, &c->bSegBig1, &c->bSegLeaf, &c->bSegHeap, &c->bSegCounters, &c->bSegChunkOff,
-					 &c->bSegChunkRange, &c->bSegChunkCnt, &c->bSegChunkPre, &c->bSegRangeM, &c->bSegRangeLm, &c->bSegTemp,
-					 &c->bSegVerify, &c->bSegShadowK, &c->bSegShadowV,
-					 &c->bCounters, &c->bQRowCount, &c->bCurLen, &c->bDivOff, &c->bDivCount, &c->bDivOut,
-					 &c->bQueryIds, &c->bCand, &c->bExtWords, &c->bExtWordOff, &c->bExtLen, &c->bAln, &c->bAlnOff,
-					 &c->bRA};
-	for (DevBuf* b : all) release(c, *b);
-	if (c->hostSeg) hipHostFree(c->hostSeg);
-	for (auto& e : c->ev)
-		if (e) hipEventDestroy(e);
-	if (c->stream) hipStreamDestroy(c->stream);
+	(void)hipSetDevice(c->device);
+	for (auto& l : c->lanes) destroyLane(c, *l);
+	for (DevBuf* b : {&c->dWords, &c->dWordOff, &c->dReadLen, &c->dReadWinOff, &c->dKeys, &c->dVals, &c->dCursor,
+					  &c->dEntries, &c->dLogX, &c->dLogY})
+		release(c, *b);
 	delete c;
 	return FLYEHIP_OK;
 }
@@ -929,13 +1062,18 @@ int flyehip_set_tuning(flyehip_ctx* c, int32_t lanes, int32_t qpb, int32_t gpp)
 			if (qpb < 1) fail(FLYEHIP_ERR_ARG, "queries_per_batch must be positive");
 			c->queriesPerBatch = qpb;
 		}
-		if (gpp) c->groupsPerPull = gpp;
+		if (gpp)
+		{
+			if (gpp < 1 || gpp > 4) fail(FLYEHIP_ERR_ARG, "batches_in_flight %d not supported (1 .. 4)", gpp);
+			c->batchesInFlight = gpp;
+		}
 	});
 }
 
 int flyehip_store_reads(flyehip_ctx* c, const uint8_t* bases, const int64_t* offsets, int32_t n_reads)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		if (!bases || !offsets || n_reads <= 0) fail(FLYEHIP_ERR_ARG, "empty read store");
 		std::vector<uint32_t> words;
 		std::vector<int64_t> wordOff;
@@ -945,16 +1083,17 @@ int flyehip_store_reads(flyehip_ctx* c, const uint8_t* bases, const int64_t* off
 		c->nReads = n_reads;
 		c->readLen = lens;
 		c->readWordOff = wordOff;
-		upload(c, ensure(c, c->dWords, words.size() * 4), words.data(), words.size() * 4);
-		upload(c, ensure(c, c->dWordOff, wordOff.size() * 8), wordOff.data(), wordOff.size() * 8);
-		upload(c, ensure(c, c->dReadLen, lens.size() * 4), lens.data(), lens.size() * 4);
-		sync(c);
+		upload(L, ensure(c, c->dWords, words.size() * 4), words.data(), words.size() * 4);
+		upload(L, ensure(c, c->dWordOff, wordOff.size() * 8), wordOff.data(), wordOff.size() * 8);
+		upload(L, ensure(c, c->dReadLen, lens.size() * 4), lens.data(), lens.size() * 4);
+		sync(L);
 	});
 }
 
 int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehip_index_info_t* out)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		if (!ip) fail(FLYEHIP_ERR_ARG, "null index parameters");
 		if (c->nReads <= 0) fail(FLYEHIP_ERR_STATE, "no reads stored");
 		const int k = ip->kmer_size;
@@ -966,25 +1105,25 @@ int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehi
 			winOff[static_cast<size_t>(i) + 1] = winOff[static_cast<size_t>(i)] + std::max(0, c->readLen[static_cast<size_t>(i)] - k + 1);
 		const int64_t nWin = winOff.back();
 		int64_t* dWinOff = ensureT<int64_t>(c, c->dReadWinOff, winOff.size());
-		upload(c, dWinOff, winOff.data(), winOff.size() * 8);
+		upload(L, dWinOff, winOff.data(), winOff.size() * 8);
 		uint64_t slots = 1024;
 		while (slots < static_cast<uint64_t>(nWin) * 3 / 2 + 1024) slots <<= 1;
 		c->tableMask = slots - 1;
 		ensure(c, c->dKeys, slots * 8);
 		ensure(c, c->dVals, slots * 8);
 		ensure(c, c->dCursor, slots * 4);
-		fill64(c, static_cast<uint64_t*>(c->dKeys.p), static_cast<int64_t>(slots), kEmptyKey);
-		fill64(c, static_cast<uint64_t*>(c->dVals.p), static_cast<int64_t>(slots), 0);
+		fill64(L, static_cast<uint64_t*>(c->dKeys.p), static_cast<int64_t>(slots), kEmptyKey);
+		fill64(L, static_cast<uint64_t*>(c->dVals.p), static_cast<int64_t>(slots), 0);
 		const PackedSeqs rs = storeSeqs(c);
 		HashTable t = table(c);
-		HIPCHECK(launchCount(rs, dWinOff, nWin, k, t, c->stream));
+		HIPCHECK(launchCount(rs, dWinOff, nWin, k, t, L->stream));
 		const int hmax = 1 << 16;
-		uint64_t* dHist = ensureT<uint64_t>(c, c->bCKey, static_cast<size_t>(hmax) + 8);
-		fill64(c, dHist, hmax + 8, 0);
-		HIPCHECK(launchHistogram(t, dHist, hmax, dHist + hmax, c->stream));
+		uint64_t* dHist = ensureT<uint64_t>(c, L->bCKey, static_cast<size_t>(hmax) + 8);
+		fill64(L, dHist, hmax + 8, 0);
+		HIPCHECK(launchHistogram(t, dHist, hmax, dHist + hmax, L->stream));
 		std::vector<uint64_t> hist(static_cast<size_t>(hmax) + 8);
-		download(c, hist.data(), dHist, hist.size() * 8);
-		sync(c);
+		download(L, hist.data(), dHist, hist.size() * 8);
+		sync(L);
 		uint64_t acc[3] = {hist[static_cast<size_t>(hmax)], hist[static_cast<size_t>(hmax) + 1], hist[static_cast<size_t>(hmax) + 2]};
 		hist.resize(static_cast<size_t>(hmax));
 		// trim to the last non-zero count so the valley scan matches the oracle's histogram
@@ -995,24 +1134,24 @@ int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehi
 		const int solidMin = ip->solid_min > 0 ? ip->solid_min : autoSolidMin(hist);
 		const int repMax = ip->rep_max > 0 ? ip->rep_max : autoRepMax(hist, acc, solidMin, ip->repeat_kmer_rate);
 		if (repMax >= (1 << kCountBits)) fail(FLYEHIP_ERR_ARG, "rep_max %d exceeds the bucket size limit %d", repMax, (1 << kCountBits) - 1);
-		int32_t* sizes = ensureT<int32_t>(c, c->bWinCount, slots + 1);
-		uint64_t* counters = ensureT<uint64_t>(c, c->bCounters, 16);
-		fill64(c, counters, 16, 0);
-		HIPCHECK(launchClassify(t, solidMin, repMax, sizes, counters, c->stream));
-		int64_t* starts = ensureT<int64_t>(c, c->bMatchOff, slots + 1);
-		int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(static_cast<int64_t>(slots)) + 64);
-		HIPCHECK(exclusiveScan(sizes, starts, static_cast<int64_t>(slots), tmp, c->stream));
+		int32_t* sizes = ensureT<int32_t>(c, L->bWinCount, slots + 1);
+		uint64_t* counters = ensureT<uint64_t>(c, L->bCounters, 16);
+		fill64(L, counters, 16, 0);
+		HIPCHECK(launchClassify(t, solidMin, repMax, sizes, counters, L->stream));
+		int64_t* starts = ensureT<int64_t>(c, L->bMatchOff, slots + 1);
+		int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(static_cast<int64_t>(slots)) + 64);
+		HIPCHECK(exclusiveScan(sizes, starts, static_cast<int64_t>(slots), tmp, L->stream));
 		int64_t nEntries = 0;
 		uint64_t cnt[2];
-		download(c, &nEntries, starts + slots, 8);
-		download(c, cnt, counters, 16);
-		sync(c);
-		HIPCHECK(launchAssign(t, sizes, starts, c->stream));
-		uint64_t* ord = ensureT<uint64_t>(c, c->bMKey, static_cast<size_t>(nEntries) + 1);
+		download(L, &nEntries, starts + slots, 8);
+		download(L, cnt, counters, 16);
+		sync(L);
+		HIPCHECK(launchAssign(t, sizes, starts, L->stream));
+		uint64_t* ord = ensureT<uint64_t>(c, L->bMKey, static_cast<size_t>(nEntries) + 1);
 		ensure(c, c->dEntries, static_cast<size_t>(nEntries) * 8 + 8);
-		HIPCHECK(launchFill(rs, dWinOff, nWin, k, t, ord, c->stream));
-		HIPCHECK(launchBucketFinish(t, ord, static_cast<uint64_t*>(c->dEntries.p), static_cast<const int32_t*>(c->dReadLen.p), k, c->stream));
-		sync(c);
+		HIPCHECK(launchFill(rs, dWinOff, nWin, k, t, ord, L->stream));
+		HIPCHECK(launchBucketFinish(t, ord, static_cast<uint64_t*>(c->dEntries.p), static_cast<const int32_t*>(c->dReadLen.p), k, L->stream));
+		sync(L);
 		c->info = flyehip_index_info_t{};
 		c->info.kmer_size = k;
 		c->info.solid_min = solidMin;
@@ -1033,13 +1172,14 @@ int flyehip_collect_candidates(flyehip_ctx* c, const int64_t* query_ids, int32_t
 							   int64_t filtered_cap, int64_t* filtered_offsets, int64_t* filtered_needed)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		std::vector<JoinQuery> qs = residentQueries(c, query_ids, n);
 		JoinArgs ja;
-		BatchOut bo = runJoin(c, storeSeqs(c), qs, ja);
+		BatchOut bo = runJoin(L, storeSeqs(c), qs, ja);
 		std::vector<int64_t> qmo(static_cast<size_t>(n) + 1), qfo(static_cast<size_t>(n) + 1);
-		download(c, qmo.data(), c->bQMatchOff.p, qmo.size() * 8);
-		download(c, qfo.data(), c->bQFiltOff.p, qfo.size() * 8);
-		sync(c);
+		download(L, qmo.data(), L->bQMatchOff.p, qmo.size() * 8);
+		download(L, qfo.data(), L->bQFiltOff.p, qfo.size() * 8);
+		sync(L);
 		if (matches_needed) *matches_needed = bo.matches;
 		if (filtered_needed) *filtered_needed = bo.filtered;
 		for (int32_t i = 0; i <= n; ++i)
@@ -1050,13 +1190,13 @@ int flyehip_collect_candidates(flyehip_ctx* c, const int64_t* query_ids, int32_t
 		if (bo.matches > match_cap || bo.filtered > filtered_cap)
 			fail(FLYEHIP_ERR_CAPACITY, "capacity too small: %lld matches / %lld filtered needed",
 				 static_cast<long long>(bo.matches), static_cast<long long>(bo.filtered));
-		int64_t* dIds = ensureT<int64_t>(c, c->bQueryIds, static_cast<size_t>(n) + 1);
-		upload(c, dIds, query_ids, static_cast<size_t>(n) * 8);
-		flyehip_candidate_t* dc = ensureT<flyehip_candidate_t>(c, c->bCand, static_cast<size_t>(bo.matches) + 1);
-		HIPCHECK(launchWriteCandidates(ja, static_cast<const int64_t*>(c->bQMatchOff.p), dIds, nullptr, dc, bo.matches, c->stream));
-		download(c, matches, dc, static_cast<size_t>(bo.matches) * sizeof(flyehip_candidate_t));
-		download(c, filtered, ja.filtered, static_cast<size_t>(bo.filtered) * 4);
-		sync(c);
+		int64_t* dIds = ensureT<int64_t>(c, L->bQueryIds, static_cast<size_t>(n) + 1);
+		upload(L, dIds, query_ids, static_cast<size_t>(n) * 8);
+		flyehip_candidate_t* dc = ensureT<flyehip_candidate_t>(c, L->bCand, static_cast<size_t>(bo.matches) + 1);
+		HIPCHECK(launchWriteCandidates(ja, static_cast<const int64_t*>(L->bQMatchOff.p), dIds, nullptr, dc, bo.matches, L->stream));
+		download(L, matches, dc, static_cast<size_t>(bo.matches) * sizeof(flyehip_candidate_t));
+		download(L, filtered, ja.filtered, static_cast<size_t>(bo.filtered) * 4);
+		sync(L);
 	});
 }
 
@@ -1064,27 +1204,28 @@ int flyehip_collect_sorted_candidates(flyehip_ctx* c, const int64_t* query_ids, 
 									  int64_t match_cap, int64_t* match_offsets, int64_t* matches_needed)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		resetStats(c);
 		std::vector<JoinQuery> qs = residentQueries(c, query_ids, n);
 		JoinArgs ja;
-		BatchOut bo = runJoin(c, storeSeqs(c), qs, ja);
+		BatchOut bo = runJoin(L, storeSeqs(c), qs, ja);
 		std::vector<int64_t> qmo(static_cast<size_t>(n) + 1);
-		download(c, qmo.data(), c->bQMatchOff.p, qmo.size() * 8);
-		sync(c);
+		download(L, qmo.data(), L->bQMatchOff.p, qmo.size() * 8);
+		sync(L);
 		if (matches_needed) *matches_needed = bo.matches;
 		for (int32_t i = 0; i <= n; ++i) match_offsets[i] = qmo[static_cast<size_t>(i)];
 		if (bo.matches > match_cap)
 			fail(FLYEHIP_ERR_CAPACITY, "capacity too small: %lld matches needed", static_cast<long long>(bo.matches));
-		runMatchSort(c, ja, bo.matches);
-		int64_t* dIds = ensureT<int64_t>(c, c->bQueryIds, static_cast<size_t>(n) + 1);
-		upload(c, dIds, query_ids, static_cast<size_t>(n) * 8);
-		flyehip_candidate_t* dc = ensureT<flyehip_candidate_t>(c, c->bCand, static_cast<size_t>(bo.matches) + 1);
+		runMatchSort(L, ja, bo.matches);
+		int64_t* dIds = ensureT<int64_t>(c, L->bQueryIds, static_cast<size_t>(n) + 1);
+		upload(L, dIds, query_ids, static_cast<size_t>(n) * 8);
+		flyehip_candidate_t* dc = ensureT<flyehip_candidate_t>(c, L->bCand, static_cast<size_t>(bo.matches) + 1);
 		JoinArgs sorted = ja;
-		sorted.mkey = static_cast<uint64_t*>(c->bSKey.p);
-		HIPCHECK(launchWriteCandidates(sorted, static_cast<const int64_t*>(c->bQMatchOff.p), dIds,
-									   static_cast<const uint32_t*>(c->bVal.p), dc, bo.matches, c->stream));
-		download(c, matches, dc, static_cast<size_t>(bo.matches) * sizeof(flyehip_candidate_t));
-		sync(c);
+		sorted.mkey = static_cast<uint64_t*>(L->bSKey.p);
+		HIPCHECK(launchWriteCandidates(sorted, static_cast<const int64_t*>(L->bQMatchOff.p), dIds,
+									   static_cast<const uint32_t*>(L->bVal.p), dc, bo.matches, L->stream));
+		download(L, matches, dc, static_cast<size_t>(bo.matches) * sizeof(flyehip_candidate_t));
+		sync(L);
 	});
 }
 
@@ -1093,9 +1234,10 @@ int flyehip_overlaps(flyehip_ctx* c, const int64_t* query_ids, int32_t n, const 
 					 float* div_stats, int64_t div_cap, int64_t* div_offsets)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		std::vector<JoinQuery> qs = residentQueries(c, query_ids, n);
 		overlapsImpl(c, storeSeqs(c), qs, query_ids, p, rows, row_cap, row_offsets, rows_needed, div_stats, div_cap,
-					 div_offsets, c->queriesPerBatch);
+					 div_offsets, c->queriesPerBatch, 4);
 	});
 }
 
@@ -1103,8 +1245,9 @@ int flyehip_overlaps_per_query(flyehip_ctx* c, const int64_t* query_ids, int32_t
 							   flyehip_overlap_t* rows, int64_t row_cap, int64_t* row_offsets, int64_t* rows_needed)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		std::vector<JoinQuery> qs = residentQueries(c, query_ids, n);
-		overlapsImpl(c, storeSeqs(c), qs, query_ids, p, rows, row_cap, row_offsets, rows_needed, nullptr, 0, nullptr, 1);
+		overlapsImpl(c, storeSeqs(c), qs, query_ids, p, rows, row_cap, row_offsets, rows_needed, nullptr, 0, nullptr, 1, 1);
 	});
 }
 
@@ -1113,18 +1256,19 @@ int flyehip_overlaps_external(flyehip_ctx* c, const uint8_t* bases, const int64_
 							  int64_t* row_offsets, int64_t* rows_needed)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		if (!c->haveIndex) fail(FLYEHIP_ERR_STATE, "no index: call flyehip_store_reads and flyehip_build_index first");
 		std::vector<uint32_t> words;
 		std::vector<int64_t> wordOff;
 		std::vector<int32_t> lens;
 		packSeqs(bases, offsets, n, words, wordOff, lens);
-		upload(c, ensure(c, c->bExtWords, words.size() * 4), words.data(), words.size() * 4);
-		upload(c, ensure(c, c->bExtWordOff, wordOff.size() * 8), wordOff.data(), wordOff.size() * 8);
-		upload(c, ensure(c, c->bExtLen, lens.size() * 4 + 4), lens.data(), lens.size() * 4);
+		upload(L, ensure(c, L->bExtWords, words.size() * 4), words.data(), words.size() * 4);
+		upload(L, ensure(c, L->bExtWordOff, wordOff.size() * 8), wordOff.data(), wordOff.size() * 8);
+		upload(L, ensure(c, L->bExtLen, lens.size() * 4 + 4), lens.data(), lens.size() * 4);
 		PackedSeqs s;
-		s.words = static_cast<const uint32_t*>(c->bExtWords.p);
-		s.wordOff = static_cast<const int64_t*>(c->bExtWordOff.p);
-		s.len = static_cast<const int32_t*>(c->bExtLen.p);
+		s.words = static_cast<const uint32_t*>(L->bExtWords.p);
+		s.wordOff = static_cast<const int64_t*>(L->bExtWordOff.p);
+		s.len = static_cast<const int32_t*>(L->bExtLen.p);
 		s.n = n;
 		std::vector<JoinQuery> qs(static_cast<size_t>(n));
 		for (int32_t i = 0; i < n; ++i)
@@ -1141,7 +1285,7 @@ int flyehip_overlaps_external(flyehip_ctx* c, const uint8_t* bases, const int64_
 			q.reserved = 0;
 			qs[static_cast<size_t>(i)] = q;
 		}
-		overlapsImpl(c, s, qs, query_ids, p, rows, row_cap, row_offsets, rows_needed, nullptr, 0, nullptr, c->queriesPerBatch);
+		overlapsImpl(c, s, qs, query_ids, p, rows, row_cap, row_offsets, rows_needed, nullptr, 0, nullptr, c->queriesPerBatch, 4);
 	});
 }
 
@@ -1151,6 +1295,7 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 						int64_t row_cap, int64_t* row_offsets, int64_t* rows_needed)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		checkParams(p, 0);
 		resetStats(c);
 		if (!c->dLogX.p) fail(FLYEHIP_ERR_STATE, "context not initialised");
@@ -1176,15 +1321,15 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 			gs.push_back(g);
 		}
 		const size_t nm = static_cast<size_t>(nHits) + 1;
-		int32_t* dc = ensureT<int32_t>(c, c->bHCur, nm);
-		int32_t* de = ensureT<int32_t>(c, c->bHExt, nm);
-		upload(c, dc, cur_pos + h0, static_cast<size_t>(nHits) * 4);
-		upload(c, de, ext_pos + h0, static_cast<size_t>(nHits) * 4);
-		GroupDesc* dg = ensureT<GroupDesc>(c, c->bGroups, gs.size() + 1);
-		upload(c, dg, gs.data(), gs.size() * sizeof(GroupDesc));
-		QueryDesc* dq = ensureT<QueryDesc>(c, c->bQDesc, qd.size() + 1);
-		upload(c, dq, qd.data(), qd.size() * sizeof(QueryDesc));
-		int32_t* dFilt = ensureT<int32_t>(c, c->bFiltered, 4);
+		int32_t* dc = ensureT<int32_t>(c, L->bHCur, nm);
+		int32_t* de = ensureT<int32_t>(c, L->bHExt, nm);
+		upload(L, dc, cur_pos + h0, static_cast<size_t>(nHits) * 4);
+		upload(L, de, ext_pos + h0, static_cast<size_t>(nHits) * 4);
+		GroupDesc* dg = ensureT<GroupDesc>(c, L->bGroups, gs.size() + 1);
+		upload(L, dg, gs.data(), gs.size() * sizeof(GroupDesc));
+		QueryDesc* dq = ensureT<QueryDesc>(c, L->bQDesc, qd.size() + 1);
+		upload(L, dq, qd.data(), qd.size() * sizeof(QueryDesc));
+		int32_t* dFilt = ensureT<int32_t>(c, L->bFiltered, 4);
 		ChainArgs ca{};
 		ca.params = *p;
 		ca.groups = dg;
@@ -1195,33 +1340,33 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 		ca.cur = dc;
 		ca.ext = de;
 		ca.filtered = dFilt;
-		ca.sortKey = ensureT<uint32_t>(c, c->bCKey, nm);
-		ca.sortVal = ensureT<uint32_t>(c, c->bCVal, nm);
-		ca.posL = ensureT<int32_t>(c, c->bCPosL, nm);
-		ca.posR = ensureT<int32_t>(c, c->bCPosR, nm);
-		ca.hcur = ensureT<int32_t>(c, c->bCHCur, nm);
-		ca.hext = ensureT<int32_t>(c, c->bCHExt, nm);
-		ca.score = ensureT<int32_t>(c, c->bCScore, nm);
-		ca.bt = ensureT<int32_t>(c, c->bCBt, nm);
-		ca.props = ensureT<Proposal>(c, c->bCProps, nm);
-		ca.outProps = ensureT<Proposal>(c, c->bCOut, nm);
-		ca.groupCount = ensureT<int32_t>(c, c->bGroupCount, gs.size() + 1);
-		ca.groupPass = ensureT<int32_t>(c, c->bGroupPass, gs.size() + 1);
+		ca.sortKey = ensureT<uint32_t>(c, L->bCKey, nm);
+		ca.sortVal = ensureT<uint32_t>(c, L->bCVal, nm);
+		ca.posL = ensureT<int32_t>(c, L->bCPosL, nm);
+		ca.posR = ensureT<int32_t>(c, L->bCPosR, nm);
+		ca.hcur = ensureT<int32_t>(c, L->bCHCur, nm);
+		ca.hext = ensureT<int32_t>(c, L->bCHExt, nm);
+		ca.score = ensureT<int32_t>(c, L->bCScore, nm);
+		ca.bt = ensureT<int32_t>(c, L->bCBt, nm);
+		ca.props = ensureT<Proposal>(c, L->bCProps, nm);
+		ca.outProps = ensureT<Proposal>(c, L->bCOut, nm);
+		ca.groupCount = ensureT<int32_t>(c, L->bGroupCount, gs.size() + 1);
+		ca.groupPass = ensureT<int32_t>(c, L->bGroupPass, gs.size() + 1);
 		ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
 		ca.logY = static_cast<const uint32_t*>(c->dLogY.p);
-		runChainKernels(c, ca, nHits);
-		int64_t* rowStart = ensureT<int64_t>(c, c->bRowStart, gs.size() + 1);
-		int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(static_cast<int64_t>(gs.size())) + 64);
-		HIPCHECK(exclusiveScan(ca.groupCount, rowStart, static_cast<int64_t>(gs.size()), tmp, c->stream));
+		runChainKernels(L, ca, nHits);
+		int64_t* rowStart = ensureT<int64_t>(c, L->bRowStart, gs.size() + 1);
+		int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(static_cast<int64_t>(gs.size())) + 64);
+		HIPCHECK(exclusiveScan(ca.groupCount, rowStart, static_cast<int64_t>(gs.size()), tmp, L->stream));
 		int64_t nRows = 0;
-		download(c, &nRows, rowStart + gs.size(), 8);
-		sync(c);
-		flyehip_overlap_t* dr = ensureT<flyehip_overlap_t>(c, c->bRows, static_cast<size_t>(nRows) + 1);
-		HIPCHECK(launchWriteRows(ca, rowStart, dr, c->stream));
+		download(L, &nRows, rowStart + gs.size(), 8);
+		sync(L);
+		flyehip_overlap_t* dr = ensureT<flyehip_overlap_t>(c, L->bRows, static_cast<size_t>(nRows) + 1);
+		HIPCHECK(launchWriteRows(ca, rowStart, dr, L->stream));
 		std::vector<int32_t> gc(gs.size());
-		download(c, gc.data(), ca.groupCount, gs.size() * 4);
-		if (nRows <= row_cap) download(c, rows, dr, static_cast<size_t>(nRows) * sizeof(flyehip_overlap_t));
-		sync(c);
+		download(L, gc.data(), ca.groupCount, gs.size() * 4);
+		if (nRows <= row_cap) download(L, rows, dr, static_cast<size_t>(nRows) * sizeof(flyehip_overlap_t));
+		sync(L);
 		if (rows_needed) *rows_needed = nRows;
 		std::vector<int64_t> per(static_cast<size_t>(n_lists), 0);
 		for (size_t g = 0; g < gs.size(); ++g) per[static_cast<size_t>(gs[g].query)] += gc[g];
@@ -1239,18 +1384,19 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 int flyehip_divergence_logf(flyehip_ctx* c, const float* x, int64_t n, float* out)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		if (n < 0 || (n > 0 && (!x || !out))) fail(FLYEHIP_ERR_ARG, "invalid logf arguments");
 		const int64_t chunk = 1 << 26;
-		float* dx = ensureT<float>(c, c->bDivOut, static_cast<size_t>(std::min(n, chunk)) + 1);
-		float* dy = ensureT<float>(c, c->bCKey, static_cast<size_t>(std::min(n, chunk)) + 1);
+		float* dx = ensureT<float>(c, L->bDivOut, static_cast<size_t>(std::min(n, chunk)) + 1);
+		float* dy = ensureT<float>(c, L->bCKey, static_cast<size_t>(std::min(n, chunk)) + 1);
 		for (int64_t b = 0; b < n; b += chunk)
 		{
 			const int64_t m = std::min(chunk, n - b);
-			upload(c, dx, x + b, static_cast<size_t>(m) * 4);
+			upload(L, dx, x + b, static_cast<size_t>(m) * 4);
 			HIPCHECK(launchLogf(dx, m, dy, static_cast<const uint32_t*>(c->dLogX.p),
-								static_cast<const uint32_t*>(c->dLogY.p), c->logN, c->stream));
-			download(c, out + b, dy, static_cast<size_t>(m) * 4);
-			sync(c);
+								static_cast<const uint32_t*>(c->dLogY.p), c->logN, L->stream));
+			download(L, out + b, dy, static_cast<size_t>(m) * 4);
+			sync(L);
 		}
 	});
 }
@@ -1261,6 +1407,7 @@ int flyehip_read_align_chains(flyehip_ctx* c, const flyehip_edge_alignment_t* al
 							  int64_t segment_cap, int64_t* segment_offsets, int64_t* segments_needed)
 {
 	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
 		if (!params) fail(FLYEHIP_ERR_ARG, "null chain parameters");
 		resetStats(c);
 		const int64_t m = aln_offsets[n_reads] - aln_offsets[0];
@@ -1268,10 +1415,10 @@ int flyehip_read_align_chains(flyehip_ctx* c, const flyehip_edge_alignment_t* al
 		std::vector<int64_t> off(static_cast<size_t>(n_reads) + 1);
 		for (int32_t r = 0; r <= n_reads; ++r) off[static_cast<size_t>(r)] = aln_offsets[r] - a0;
 		const size_t nm = static_cast<size_t>(m) + 1;
-		flyehip_edge_alignment_t* dA = ensureT<flyehip_edge_alignment_t>(c, c->bAln, nm);
-		int64_t* dOff = ensureT<int64_t>(c, c->bAlnOff, off.size());
-		upload(c, dA, alignments + a0, static_cast<size_t>(m) * sizeof(flyehip_edge_alignment_t));
-		upload(c, dOff, off.data(), off.size() * 8);
+		flyehip_edge_alignment_t* dA = ensureT<flyehip_edge_alignment_t>(c, L->bAln, nm);
+		int64_t* dOff = ensureT<int64_t>(c, L->bAlnOff, off.size());
+		upload(L, dA, alignments + a0, static_cast<size_t>(m) * sizeof(flyehip_edge_alignment_t));
+		upload(L, dOff, off.data(), off.size() * 8);
 		ReadAlignArgs ra{};
 		ra.aln = dA;
 		ra.alnOff = dOff;
@@ -1279,37 +1426,37 @@ int flyehip_read_align_chains(flyehip_ctx* c, const flyehip_edge_alignment_t* al
 		ra.longEdge = long_edge;
 		ra.bigAln = big_aln;
 		ra.p = *params;
-		ra.kept = ensureT<int32_t>(c, c->bCPosL, nm);
-		ra.key = ensureT<uint64_t>(c, c->bCKey, nm);
-		ra.val = ensureT<uint32_t>(c, c->bCVal, nm);
-		ra.chainScore = ensureT<int32_t>(c, c->bCScore, nm);
-		ra.chainPrev = ensureT<int32_t>(c, c->bCBt, nm);
-		ra.chainLen = ensureT<int32_t>(c, c->bCPosR, nm);
-		ra.active = ensureT<int32_t>(c, c->bCHCur, nm);
-		ra.frozen = ensureT<int32_t>(c, c->bCHExt, nm);
-		ra.accepted = ensureT<int32_t>(c, c->bHCur, nm);
-		int32_t* counts = ensureT<int32_t>(c, c->bRA, 3 * static_cast<size_t>(n_reads) + 3);
+		ra.kept = ensureT<int32_t>(c, L->bCPosL, nm);
+		ra.key = ensureT<uint64_t>(c, L->bCKey, nm);
+		ra.val = ensureT<uint32_t>(c, L->bCVal, nm);
+		ra.chainScore = ensureT<int32_t>(c, L->bCScore, nm);
+		ra.chainPrev = ensureT<int32_t>(c, L->bCBt, nm);
+		ra.chainLen = ensureT<int32_t>(c, L->bCPosR, nm);
+		ra.active = ensureT<int32_t>(c, L->bCHCur, nm);
+		ra.frozen = ensureT<int32_t>(c, L->bCHExt, nm);
+		ra.accepted = ensureT<int32_t>(c, L->bHCur, nm);
+		int32_t* counts = ensureT<int32_t>(c, L->bRA, 3 * static_cast<size_t>(n_reads) + 3);
 		ra.keptCount = counts;
 		ra.acceptedCount = counts + n_reads + 1;
 		ra.segCount = counts + 2 * (n_reads + 1);
 		Timer tc;
-		HIPCHECK(launchReadAlignChain(ra, c->stream));
-		int64_t* segOff = ensureT<int64_t>(c, c->bRowStart, static_cast<size_t>(n_reads) + 1);
-		int64_t* tmp = ensureT<int64_t>(c, c->bScanTmp, scanTempWords(n_reads) + 64);
-		HIPCHECK(exclusiveScan(ra.segCount, segOff, n_reads, tmp, c->stream));
+		HIPCHECK(launchReadAlignChain(ra, L->stream));
+		int64_t* segOff = ensureT<int64_t>(c, L->bRowStart, static_cast<size_t>(n_reads) + 1);
+		int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(n_reads) + 64);
+		HIPCHECK(exclusiveScan(ra.segCount, segOff, n_reads, tmp, L->stream));
 		std::vector<int64_t> so(static_cast<size_t>(n_reads) + 1);
 		std::vector<int32_t> kc(static_cast<size_t>(n_reads));
-		download(c, so.data(), segOff, so.size() * 8);
-		download(c, kc.data(), ra.keptCount, kc.size() * 4);
-		sync(c);
+		download(L, so.data(), segOff, so.size() * 8);
+		download(L, kc.data(), ra.keptCount, kc.size() * 4);
+		sync(L);
 		const int64_t nSeg = so.back();
 		if (segments_needed) *segments_needed = nSeg;
-		flyehip_chain_segment_t* dS = ensureT<flyehip_chain_segment_t>(c, c->bRows, static_cast<size_t>(nSeg) + 1);
-		HIPCHECK(launchReadAlignWrite(ra, segOff, dS, c->stream));
+		flyehip_chain_segment_t* dS = ensureT<flyehip_chain_segment_t>(c, L->bRows, static_cast<size_t>(nSeg) + 1);
+		HIPCHECK(launchReadAlignWrite(ra, segOff, dS, L->stream));
 		std::vector<int32_t> keptAll(static_cast<size_t>(m) + 1);
-		download(c, keptAll.data(), ra.kept, static_cast<size_t>(m) * 4);
-		if (nSeg <= segment_cap) download(c, segments, dS, static_cast<size_t>(nSeg) * sizeof(flyehip_chain_segment_t));
-		sync(c);
+		download(L, keptAll.data(), ra.kept, static_cast<size_t>(m) * 4);
+		if (nSeg <= segment_cap) download(L, segments, dS, static_cast<size_t>(nSeg) * sizeof(flyehip_chain_segment_t));
+		sync(L);
 		c->stats.ms_chain = tc.ms();
 		int64_t kw = 0;
 		kept_offsets[0] = 0;

@@ -254,16 +254,26 @@ __global__ void __launch_bounds__(FH_WAVE) segLeafKernel(SegSortBufs b, int64_t 
 										 w.tag != 0);
 }
 
+__device__ inline bool heapAncestorOrSelf(int h, int x)
+{
+	const unsigned H = static_cast<unsigned>(h) + 1u, X = static_cast<unsigned>(x) + 1u;
+	if (H > X) return false;
+	return (X >> (__clz(H) - __clz(X))) == H;
+}
+
 // Heap sort, exactly libstdc++'s make_heap + sort_heap, of the ranges of a
-// list (only entries with d == 1 when onlyFlagged).  The sift steps are a
-// dependent chain, so one lane runs them; the range is staged in LDS as packed
-// (key << 32 | value) words so each step is one LDS round trip.  Ranges of
+// list (only entries with d == 1 when onlyFlagged), one wave per range staged
+// in LDS as packed (key << 32 | value) words; the schedule of
+// exsort::heapSortPipelinedModel: make_heap one heap level at a time with a
+// lane per parent, then sort_heap with up to kSlots pops in flight, one lane
+// each, started two steps apart and advancing one level per step.  Ranges of
 // size in (minN, cap] are taken (dynamic LDS of cap words); larger ones go to
 // segHeapGlobalKernel.  Values go to dstV, keys to dstK when it is set.
 __global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list, int64_t nList, int onlyFlagged,
 															const uint32_t* srcK, const uint32_t* srcV, uint32_t* dstK,
 															uint32_t* dstV, int32_t minN, int32_t cap)
 {
+	constexpr int kSlots = 16;
 	extern __shared__ uint64_t hx[];
 	const int lane = laneId();
 	for (int64_t i = blockIdx.x; i < nList; i += gridDim.x)
@@ -274,8 +284,55 @@ __global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list
 		for (int t = lane; t < n; t += FH_WAVE)
 			hx[t] = (static_cast<uint64_t>(srcK[w.f + t]) << 32) | srcV[w.f + t];
 		waveSync();
-		if (lane == 0) exsort::heapSortPacked(hx, n);
-		waveSync();
+		if (n >= 2)
+		{
+			// make_heap
+			const int lastParent = (n - 2) / 2;
+			const int top = 31 - __clz(static_cast<unsigned>(lastParent) + 1u);
+			for (int lv = top; lv >= 0; --lv)
+			{
+				const int lo = (1 << lv) - 1;
+				const int hi = min(lastParent, (2 << lv) - 2);
+				for (int p = hi - lane; p >= lo; p -= FH_WAVE) exsort::siftPacked(hx, p, n, hx[p]);
+				waveSync();
+			}
+			// sort_heap
+			bool act = false;
+			int h = 0, m = 0;
+			uint64_t v = 0;
+			int L = n, lastStart = -1000, pops = 0;
+			// LDS operations of one wave complete in program order, so a cycle's
+			// writes reach the next cycle's reads without waiting; the wave
+			// barrier only keeps the compiler from moving them across cycles
+			for (int cyc = 0;; ++cyc)
+			{
+				unsigned long long running = __ballot(act);
+				if (L > 1 && cyc - lastStart >= 2)
+				{
+					const int slot = pops % kSlots;
+					const bool blocked = __ballot(act && heapAncestorOrSelf(h, L - 1)) != 0;
+					if (((running >> slot) & 1ULL) == 0 && !blocked)
+					{
+						if (lane == slot)
+						{
+							v = hx[L - 1];
+							hx[L - 1] = hx[0];
+							h = 0;
+							m = L - 1;
+							act = true;
+						}
+						running |= 1ULL << slot;
+						lastStart = cyc;
+						--L;
+						++pops;
+					}
+				}
+				if (running == 0 && L <= 1) break;
+				if (act) act = exsort::siftStepPacked(hx, h, m, v);
+				__builtin_amdgcn_wave_barrier();
+			}
+			waveSync();
+		}
 		for (int t = lane; t < n; t += FH_WAVE)
 		{
 			dstV[w.f + t] = static_cast<uint32_t>(hx[t]);
@@ -314,15 +371,34 @@ __global__ void segShadowKernel(SegSortBufs b, int64_t v0, int64_t v1)
 }
 
 // A verify entry whose free-mode result has two equal neighbouring keys needs
-// the exact heap sort (d = 1); with all keys distinct every sort agrees.
+// the exact heap sort (d = 1); with all keys distinct every sort agrees.  With
+// relevantBit set, only a run of equal keys holding two marked values counts
+// (the order of the other elements of a run is immaterial to the caller).
 __global__ void segVerifyKernel(SegSortBufs b, int64_t nVerify)
 {
 	const int64_t i = waveIndex();
 	if (i >= nVerify) return;
 	const SegRange w = b.verify[i];
-	bool eq = false;
-	for (int32_t x = w.f + 1 + laneId(); x < w.l; x += FH_WAVE) eq |= b.key[x] == b.key[x - 1];
-	const bool any = __ballot(eq) != 0;
+	bool need = false;
+	for (int32_t x = w.f + laneId(); x < w.l; x += FH_WAVE)
+	{
+		// x starts a run of at least two equal keys
+		if ((x == w.f || b.key[x - 1] != b.key[x]) && x + 1 < w.l && b.key[x + 1] == b.key[x])
+		{
+			if (!b.relevantBit)
+			{
+				need = true;
+			}
+			else
+			{
+				int marked = 0;
+				for (int32_t y = x; y < w.l && b.key[y] == b.key[x] && marked < 2; ++y)
+					marked += (b.outVal[y] & b.relevantBit) ? 1 : 0;
+				need |= marked >= 2;
+			}
+		}
+	}
+	const bool any = __ballot(need) != 0;
 	if (laneId() == 0) b.verify[i].d = any ? 1 : 0;
 }
 
@@ -350,22 +426,31 @@ hipError_t download(const SegSortBufs& b, hipStream_t s)
 	return hipStreamSynchronize(s);
 }
 
-// Heap sorts of a list through the LDS tiers (32 KiB, 64 KiB, 128 KiB) and
-// the global fallback (in place on gk / gv).
-hipError_t heapTiers(const SegRange* list, int64_t n, int onlyFlagged, const uint32_t* srcK, const uint32_t* srcV,
-					 uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s)
+// Heap sorts of a list, in three size classes staged in LDS (up to 2048,
+// 8192 and 16384 elements: 16, 64 and 128 KiB per wave, so small ranges are
+// not held to one wave per CU by the largest), launched concurrently on the
+// side streams; ranges above 16384 elements sort in place on gk / gv in
+// global memory.  The calling stream waits for all of them.
+hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int onlyFlagged, const uint32_t* srcK,
+					 const uint32_t* srcV, uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s)
 {
 	static const bool bigLds =
 		hipFuncSetAttribute(reinterpret_cast<const void*>(segHeapLdsKernel), hipFuncAttributeMaxDynamicSharedMemorySize,
 							16384 * 8) == hipSuccess;
+	const int32_t caps[3] = {2048, 8192, bigLds ? 16384 : 8192};
 	const unsigned grid = static_cast<unsigned>(n < 8192 ? n : 8192);
+	hipError_t e = hipEventRecord(b.auxEv[3], s);
+	if (e != hipSuccess) return e;
 	int32_t lo = 0;
-	for (int32_t cap : {4096, 8192, 16384})
+	for (int t = 0; t < 3; ++t)
 	{
-		if (cap == 16384 && !bigLds) break;
-		hipLaunchKernelGGL(segHeapLdsKernel, dim3(grid), dim3(FH_WAVE), static_cast<size_t>(cap) * 8, s, list, n,
-						   onlyFlagged, srcK, srcV, dstK, dstV, lo, cap);
-		lo = cap;
+		if (caps[t] <= lo) break;
+		if ((e = hipStreamWaitEvent(b.aux[t], b.auxEv[3], 0)) != hipSuccess) return e;
+		hipLaunchKernelGGL(segHeapLdsKernel, dim3(grid), dim3(FH_WAVE), static_cast<size_t>(caps[t]) * 8, b.aux[t], list,
+						   n, onlyFlagged, srcK, srcV, dstK, dstV, lo, caps[t]);
+		if ((e = hipEventRecord(b.auxEv[t], b.aux[t])) != hipSuccess) return e;
+		if ((e = hipStreamWaitEvent(s, b.auxEv[t], 0)) != hipSuccess) return e;
+		lo = caps[t];
 	}
 	hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, s, list, n, onlyFlagged, gk, gv, dstV,
 					   lo);
@@ -487,7 +572,7 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 	if (nHeap > 0)
 	{
 		// free-mode ranges out of budget: any sort will do; keys stay in place
-		e = heapTiers(b.heap, nHeap, 0, b.key, b.val, b.key, b.outVal, b.key, b.val, s);
+		e = heapTiers(b, b.heap, nHeap, 0, b.key, b.val, b.key, b.outVal, b.key, b.val, s);
 		if (e != hipSuccess) return e;
 	}
 	const int64_t nVerify = b.hostCounters[kSegNVerify];
@@ -495,8 +580,23 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 	{
 		const int tpb = FH_WAVE * kWavesPerBlock;
 		hipLaunchKernelGGL(segVerifyKernel, dim3(blocksFor(nVerify, kWavesPerBlock)), dim3(tpb), 0, s, b, nVerify);
-		e = heapTiers(b.verify, nVerify, 1, b.shadowKey, b.shadowVal, nullptr, b.outVal, b.shadowKey, b.shadowVal, s);
+		e = heapTiers(b, b.verify, nVerify, 1, b.shadowKey, b.shadowVal, nullptr, b.outVal, b.shadowKey, b.shadowVal, s);
 		if (e != hipSuccess) return e;
+		if (std::getenv("FLYEHIP_SEGSORT_TRACE"))
+		{
+			SegRange* hv = new SegRange[static_cast<size_t>(nVerify)];
+			(void)hipMemcpyAsync(hv, b.verify, static_cast<size_t>(nVerify) * sizeof(SegRange), hipMemcpyDeviceToHost, s);
+			(void)hipStreamSynchronize(s);
+			long long exact = 0, elems = 0;
+			for (int64_t i = 0; i < nVerify; ++i)
+				if (hv[i].d == 1)
+				{
+					++exact;
+					elems += hv[i].l - hv[i].f;
+				}
+			std::fprintf(stderr, "[segsort] exact heap sorts=%lld elements=%lld\n", exact, elems);
+			delete[] hv;
+		}
 	}
 	return hipSuccess;
 }

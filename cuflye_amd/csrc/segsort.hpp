@@ -37,7 +37,10 @@
 namespace fh
 {
 
-constexpr int kSegTile = 2048;        // leaf size, elements (LDS tile of one wave)
+#ifndef FH_SEG_TILE
+#define FH_SEG_TILE 1024
+#endif
+constexpr int kSegTile = FH_SEG_TILE;  // leaf size, elements (LDS tile of one wave)
 constexpr int kSegRows = 8;           // 64-element rows per chunk
 constexpr int kSegChunk = 64 * kSegRows;
 
@@ -69,6 +72,9 @@ struct SegSortBufs
 	uint32_t* key;          // sorted in place (big ranges)
 	uint32_t* val;
 	uint32_t* outVal;       // leaves and heap ranges write their final values here
+	uint32_t relevantBit;   // 0, or a value bit marking the elements whose mutual order
+							// matters; a free-mode result is then accepted unless a run of
+							// equal keys holds two marked elements (outVal == val only)
 	const int64_t* dup;     // tied-position prefix counts, n + 1 words (null: no pruning)
 	int32_t* posL;          // one int per element
 	int32_t* posR;
@@ -92,6 +98,8 @@ struct SegSortBufs
 	int64_t heapCap;        // also the capacity of verify
 	int64_t chunkCap;
 	int64_t* hostCounters;  // pinned, kSegCounters words
+	hipStream_t aux[3];     // default-priority side streams for the heap-sort size classes
+	hipEvent_t auxEv[4];    // (timing disabled)
 };
 
 struct SegSortCaps

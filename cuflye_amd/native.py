@@ -7,6 +7,7 @@ structured dtypes so results come back as arrays without copies per row.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -101,8 +102,9 @@ def _ptr(a: np.ndarray | None):
 
 
 def load_flyehip(path: Path | None = None) -> C.CDLL:
-    """Loads libflyehip.so; raises when it is not built (no CPU fallback)."""
-    path = path or (LIB_DIR / "libflyehip.so")
+    """Loads libflyehip.so (or the build named by FLYEHIP_LIBRARY); raises when
+    it is not built (no CPU fallback)."""
+    path = path or Path(os.environ.get("FLYEHIP_LIBRARY") or (LIB_DIR / "libflyehip.so"))
     if not path.exists():
         raise RuntimeError(f"{path} is not built: run __graft_entry__.build() "
                            "(python -c 'import __graft_entry__ as g; g.build()')")

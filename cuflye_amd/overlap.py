@@ -138,9 +138,9 @@ class FlyeHip:
         return s.as_dict()
 
     def set_tuning(self, dp_lanes_per_group: int = 0, queries_per_batch: int = 0,
-                   groups_per_pull: int = 0) -> None:
+                   batches_in_flight: int = 0) -> None:
         self._check(self.lib.flyehip_set_tuning(self.ctx, dp_lanes_per_group, queries_per_batch,
-                                                groups_per_pull))
+                                                batches_in_flight))
 
     # -- read store + VertexIndex --------------------------------------------
     def store_reads(self, bases: np.ndarray, offsets: np.ndarray) -> None:

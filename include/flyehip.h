@@ -192,10 +192,12 @@ int flyehip_abi_version(void);
 int flyehip_get_stats(const flyehip_ctx* ctx, flyehip_stats_t* out);
 
 /* Tuning knobs picked by the autotuner: lanes that chain one target group
- * (1, 4, 8, 16, 32 or 64), the query batch size, and the groups one DP wave
- * takes from the work queue at a time.  0 keeps the current value. */
+ * (1, 4, 8, 16, 32 or 64), the query batch size, and the batches in flight
+ * (1 .. 4): flyehip_overlaps* deal batches to that many streams, each driven
+ * by its own host thread, so one batch's tail overlaps the next batch (the
+ * per-query path always uses one).  0 keeps the current value. */
 int flyehip_set_tuning(flyehip_ctx* ctx, int32_t dp_lanes_per_group,
-                       int32_t queries_per_batch, int32_t groups_per_pull);
+                       int32_t queries_per_batch, int32_t batches_in_flight);
 
 /* ---- resident read store and k-mer index (VertexIndex) ---------------- */
 

@@ -59,7 +59,14 @@ static void checkHeapPacked(const std::vector<uint64_t>& keys)
 	const size_t n = keys.size();
 	std::vector<uint64_t> a(n);
 	for (size_t i = 0; i < n; ++i) a[i] = (static_cast<uint64_t>(static_cast<uint32_t>(keys[i])) << 32) | i;
+	std::vector<uint64_t> b = a;
 	exsort::heapSortPacked(a.data(), static_cast<int>(n));
+	exsort::heapSortPipelinedModel(b.data(), static_cast<int>(n));
+	if (a != b)
+	{
+		if (failures < 10) std::printf("MISMATCH pipelined heap n=%zu\n", n);
+		++failures;
+	}
 	std::vector<uint32_t> idx(n);
 	for (size_t i = 0; i < n; ++i) idx[i] = static_cast<uint32_t>(i);
 	std::partial_sort(idx.begin(), idx.end(), idx.end(), [&](uint32_t x, uint32_t y) {

@@ -109,6 +109,15 @@ def test_overlaps_config2_small(gpu, oracle_mod):
     erows, eroff = orc.overlaps(ids, cfg, threads=8)
     assert np.array_equal(roff, eroff)
     assert_rows_equal(rows, erows, "config2 overlaps")
+    # batches dealt to 1 .. 3 concurrent streams: scheduling only
+    try:
+        for fl in (1, 3):
+            gpu.set_tuning(queries_per_batch=64, batches_in_flight=fl)
+            frows, froff = gpu.get_seq_overlaps(ids, cfg)
+            assert np.array_equal(froff, eroff)
+            assert_rows_equal(frows, erows, f"config2 overlaps, {fl} batches in flight")
+    finally:
+        gpu.set_tuning(queries_per_batch=2048, batches_in_flight=2)
     # read-to-graph detector shape on the same data: containment filter path
     cfg2 = OverlapConfig.read_to_graph(15, 1000)
     rows, roff = gpu.get_seq_overlaps(ids[:60], cfg2)
