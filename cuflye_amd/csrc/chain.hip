@@ -196,9 +196,9 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 	}
 }
 
-// Passing groups in processing order (stable compaction), and how many of
-// them hold at least kWalkSmall hits (a prefix: the order is size-descending
-// by floor(log2 n)).
+// Passing groups in processing order (stable compaction), how many of them
+// hold at least kWalkSmall hits (a prefix: the order is size-descending by
+// floor(log2 n)), and their hits in total.
 __global__ void passFlagKernel(const int32_t* order, const int32_t* groupPass, int32_t nGroups, int32_t* flag)
 {
 	const int qi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -210,14 +210,21 @@ __global__ void passScatterKernel(const int32_t* order, const int32_t* flag, con
 {
 	const int qi = blockIdx.x * blockDim.x + threadIdx.x;
 	bool big = false;
+	long long hits = 0;
 	if (qi < nGroups && flag[qi])
 	{
 		const int g = order[qi];
 		passOrder[pos[qi]] = g;
-		big = groups[g].n >= bigMin;
+		hits = groups[g].n;
+		big = hits >= bigMin;
 	}
 	const int c = __popcll(__ballot(big));
-	if (laneId() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(nBig), static_cast<unsigned long long>(c));
+	hits = waveSum64(hits);
+	if (laneId() == 0)
+	{
+		if (c) atomicAdd(reinterpret_cast<unsigned long long*>(nBig), static_cast<unsigned long long>(c));
+		if (hits) atomicAdd(reinterpret_cast<unsigned long long*>(nBig + 1), static_cast<unsigned long long>(hits));
+	}
 }
 
 // Group-order hit copies for the DP: hits in (extId, curPos) order, or in the

@@ -90,7 +90,8 @@ hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* b
 // prefilters; pushes the extPos sorts to sb
 hipError_t launchChainPrefilter(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream);
 // passOrder = order filtered by groupPass (flags, scan by the caller, scatter);
-// *nBig counts passing groups of at least kWalkSmall hits
+// nBig[0] counts passing groups of at least kWalkSmall hits, nBig[1] their hits
+// in total (both zeroed by the caller)
 hipError_t launchPassFlags(const ChainArgs& a, int32_t* flag, hipStream_t stream);
 hipError_t launchPassScatter(const ChainArgs& a, const int32_t* flag, const int64_t* pos, int32_t* passOrder,
 							 int64_t* nBig, hipStream_t stream);

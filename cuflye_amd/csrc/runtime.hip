@@ -30,6 +30,7 @@
 #include "msort.hpp"
 #include "pipeline.hpp"
 #include "readalign.hpp"
+#include "ktimer.hpp"
 #include "segsort.hpp"
 
 extern "C" const unsigned char fh_logf_table[];
@@ -94,6 +95,7 @@ struct Lane
 	hipEvent_t ev[8] = {};
 	int64_t* hostSeg = nullptr;  // pinned counter words of the segmented sort
 	flyehip_stats_t stats{};     // stage timings and counts of this lane's batches
+	KTimer timer;                // kernel-family intervals of the batch being enqueued
 	DevBuf bQueries, bWinOff, bWinCount, bWinFilt, bWinInfo, bMatchOff, bFiltOff, bScanTmp;
 	DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bHCur, bHExt, bGroupFlag, bGroupOff;
 	DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
@@ -440,8 +442,15 @@ BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& 
 	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nWin) + 64);
 	ja.matchOff = matchOff;
 	ja.filtOff = filtOff;
+	// join bytes: 2-bit read (L / 4 per query) + one 8-byte slot probe per
+	// window; the fill adds 8 bytes per bucket entry read and 12 per match
+	// written (u64 key, i32 extPos)
+	double seqBytes = 0.0;
+	for (const auto& q : qs) seqBytes += static_cast<double>((q.len + 3) / 4);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
+	const int tj = L->timer.begin(FLYEHIP_K_JOIN, L->stream, seqBytes + 8.0 * static_cast<double>(nWin));
 	HIPCHECK(launchJoinCount(ja, L->stream));
+	L->timer.end(tj, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[1], L->stream));
 	HIPCHECK(exclusiveScan(ja.winCount, matchOff, nWin, tmp, L->stream));
 	HIPCHECK(exclusiveScan(ja.winFilt, filtOff, nWin, tmp, L->stream));
@@ -462,7 +471,10 @@ BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& 
 	ja.mext = ensureT<int32_t>(c, L->bMExt, static_cast<size_t>(bo.matches) + 1);
 	ja.filtered = ensureT<int32_t>(c, L->bFiltered, static_cast<size_t>(bo.filtered) + 1);
 	HIPCHECK(hipEventRecord(L->ev[6], L->stream));
+	const int tf = L->timer.begin(FLYEHIP_K_JOIN, L->stream,
+								  seqBytes + 8.0 * static_cast<double>(nWin) + 20.0 * static_cast<double>(bo.matches));
 	HIPCHECK(launchJoinFill(ja, L->stream));
+	L->timer.end(tf, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[7], L->stream));
 	HIPCHECK(hipEventSynchronize(L->ev[7]));
 	float e1 = 0, e2 = 0;
@@ -507,6 +519,7 @@ SegSortBufs segBufs(Lane* L, int64_t n, int64_t nArrays)
 	b.hostCounters = L->hostSeg;
 	for (int t = 0; t < 3; ++t) b.aux[t] = L->aux[t];
 	for (int t = 0; t < 4; ++t) b.auxEv[t] = L->auxEv[t];
+	b.timer = &L->timer;
 	return b;
 }
 
@@ -551,7 +564,11 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
 	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
+	// stable order: 4 (index payload) + 2 radix passes x 24 (key, value read
+	// and written) + 20 (rank keys and tie flags) bytes per match
+	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, 72.0 * static_cast<double>(nMatches));
 	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, temp, tempBytes, L->stream));
+	L->timer.end(tr, L->stream);
 	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, L->stream));
 	HIPCHECK(segSortReset(sb, L->stream));
 	HIPCHECK(segPushOffsets(sb, ms.qOff, nq, L->stream));
@@ -584,7 +601,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	int64_t* nBig = ensureT<int64_t>(c, L->bCounters, 16);
 	zero32(L, bins, 96);
 	zero32(L, queues, 16);
-	HIPCHECK(hipMemsetAsync(nBig, 0, sizeof(int64_t), L->stream));
+	HIPCHECK(hipMemsetAsync(nBig, 0, 2 * sizeof(int64_t), L->stream));
 	SegSortBufs sb = segBufs(L, nHits, ng);
 	sb.key = ca.sortKey;
 	sb.val = ca.sortVal;
@@ -599,23 +616,29 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	ca.order = order;
 	HIPCHECK(segSortReset(sb, L->stream));
 	ca.queue = queues + 0;
+	const int tp = L->timer.begin(FLYEHIP_K_PREFILTER, L->stream, 8.0 * static_cast<double>(nHits));
 	HIPCHECK(launchChainPrefilter(ca, sb, blocks, L->stream));
+	L->timer.end(tp, L->stream);
 	runSegSort(L, sb, nullptr);
 	HIPCHECK(launchPassFlags(ca, flag, L->stream));
 	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
 	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream));
-	int64_t counts[2] = {0, 0};
+	int64_t counts[3] = {0, 0, 0};
 	download(L, &counts[0], pos + ng, sizeof(int64_t));
-	download(L, &counts[1], nBig, sizeof(int64_t));
+	download(L, &counts[1], nBig, 2 * sizeof(int64_t));
 	sync(L);
 	ca.passOrder = passOrder;
 	ca.nPass = static_cast<int32_t>(counts[0]);
 	const int32_t nBigPass = static_cast<int32_t>(counts[1]);
+	const double passHits = static_cast<double>(counts[2]);
 	ca.queue = queues + 1;
 	HIPCHECK(launchChainGather(ca, blocks, L->stream));
 	HIPCHECK(hipEventRecord(L->ev[3], L->stream));
 	ca.queue = queues + 2;
+	// DP: hits read (hcur, hext) and score / backtrack written, 16 bytes per hit
+	const int td = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * passHits);
 	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, L->stream));
+	L->timer.end(td, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[4], L->stream));
 	HIPCHECK(segSortReset(sb, L->stream));
 	sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
@@ -623,9 +646,12 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	HIPCHECK(launchChainScoreKeys(ca, sb, blocks, L->stream));
 	runSegSort(L, sb, nullptr);
 	ca.queue = queues + 4;
+	// walk: score order and backtrack pointers read, 8 bytes per hit
+	const int tw = L->timer.begin(FLYEHIP_K_WALK, L->stream, 8.0 * passHits);
 	HIPCHECK(launchChainWalk(ca, 0, nBigPass, true, blocks, L->stream));
 	ca.queue = queues + 5;
 	HIPCHECK(launchChainWalk(ca, nBigPass, ca.nPass, false, blocks, L->stream));
+	L->timer.end(tw, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[5], L->stream));
 	HIPCHECK(hipEventSynchronize(L->ev[5]));
 	float a = 0, b = 0, d = 0;
@@ -738,6 +764,7 @@ void foldStats(flyehip_ctx* c)
 {
 	for (auto& l : c->lanes)
 	{
+		l->timer.collect(l->stats);  // every call ends with its streams synchronised
 		const flyehip_stats_t& s = l->stats;
 		c->stats.ms_join += s.ms_join;
 		c->stats.ms_sort += s.ms_sort;
@@ -750,6 +777,12 @@ void foldStats(flyehip_ctx* c)
 		c->stats.overlaps += s.overlaps;
 		c->stats.batches += s.batches;
 		c->stats.tie_queries += s.tie_queries;
+		for (int f = 0; f < 8; ++f)
+		{
+			c->stats.kernel_ms[f] += s.kernel_ms[f];
+			c->stats.kernel_launches[f] += s.kernel_launches[f];
+			c->stats.kernel_bytes[f] += s.kernel_bytes[f];
+		}
 		l->stats = flyehip_stats_t{};
 	}
 	c->stats.device_bytes_peak = static_cast<int64_t>(c->peak);
@@ -764,6 +797,7 @@ void createLane(flyehip_ctx* c)
 	for (auto& a : l->aux) HIPCHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
 	for (auto& e : l->auxEv) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 	HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&l->hostSeg), kSegCounters * sizeof(int64_t)));
+	HIPCHECK(l->timer.create());
 	c->lanes.push_back(std::move(l));
 }
 
@@ -783,6 +817,7 @@ void destroyLane(flyehip_ctx* c, Lane& l)
 					  &l.bAlnOff, &l.bRA})
 		release(c, *b);
 	if (l.hostSeg) (void)hipHostFree(l.hostSeg);
+	l.timer.destroy();
 	for (auto& e : l.ev)
 		if (e) (void)hipEventDestroy(e);
 	for (auto& e : l.auxEv)
@@ -874,6 +909,7 @@ void runBatch(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& all
 			for (int32_t t = 0; t < out.divCount[static_cast<size_t>(i)]; ++t)
 				out.divs.push_back(dv[static_cast<size_t>(dOff[static_cast<size_t>(i)] + t)]);
 	}
+	L->timer.collect(L->stats);
 	L->stats.batches += 1;
 }
 

@@ -254,6 +254,22 @@ __global__ void __launch_bounds__(FH_WAVE) segLeafKernel(SegSortBufs b, int64_t 
 										 w.tag != 0);
 }
 
+// exsort::siftStepPacked without branches: both children are loaded
+// together (indices clamped to the range; the clamped load is not used), keys
+// compared on their 32-bit halves.  Same result: the right child unless it is
+// below the left one, moved up unless it is below the value.
+__device__ inline bool heapStep(uint64_t* hx, int& h, int m, int n, uint64_t v)
+{
+	const int c1 = 2 * h + 1;
+	const uint64_t x1 = hx[min(c1, n - 1)], x2 = hx[min(c1 + 1, n - 1)];
+	const bool right = c1 + 1 < m && !(static_cast<uint32_t>(x2 >> 32) < static_cast<uint32_t>(x1 >> 32));
+	const uint64_t cv = right ? x2 : x1;
+	const bool up = c1 < m && !(static_cast<uint32_t>(cv >> 32) < static_cast<uint32_t>(v >> 32));
+	hx[h] = up ? cv : v;
+	h = up ? (right ? c1 + 1 : c1) : h;
+	return up;
+}
+
 __device__ inline bool heapAncestorOrSelf(int h, int x)
 {
 	const unsigned H = static_cast<unsigned>(h) + 1u, X = static_cast<unsigned>(x) + 1u;
@@ -293,7 +309,14 @@ __global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list
 			{
 				const int lo = (1 << lv) - 1;
 				const int hi = min(lastParent, (2 << lv) - 2);
-				for (int p = hi - lane; p >= lo; p -= FH_WAVE) exsort::siftPacked(hx, p, n, hx[p]);
+				for (int p = hi - lane; p >= lo; p -= FH_WAVE)
+				{
+					int hole = p;
+					const uint64_t pv = hx[p];
+					while (heapStep(hx, hole, n, n, pv))
+					{
+					}
+				}
 				waveSync();
 			}
 			// sort_heap
@@ -328,7 +351,7 @@ __global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list
 					}
 				}
 				if (running == 0 && L <= 1) break;
-				if (act) act = exsort::siftStepPacked(hx, h, m, v);
+				if (act) act = heapStep(hx, h, m, n, v);
 				__builtin_amdgcn_wave_barrier();
 			}
 			waveSync();
@@ -399,7 +422,13 @@ __global__ void segVerifyKernel(SegSortBufs b, int64_t nVerify)
 		}
 	}
 	const bool any = __ballot(need) != 0;
-	if (laneId() == 0) b.verify[i].d = any ? 1 : 0;
+	if (laneId() == 0)
+	{
+		b.verify[i].d = any ? 1 : 0;
+		if (any)
+			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegExactElems]),
+					  static_cast<unsigned long long>(w.l - w.f));
+	}
 }
 
 __global__ void segPushOffsetsKernel(SegSortBufs b, const int64_t* off, int32_t nArrays)
@@ -432,8 +461,10 @@ hipError_t download(const SegSortBufs& b, hipStream_t s)
 // side streams; ranges above 16384 elements sort in place on gk / gv in
 // global memory.  The calling stream waits for all of them.
 hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int onlyFlagged, const uint32_t* srcK,
-					 const uint32_t* srcV, uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s)
+					 const uint32_t* srcV, uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s,
+					 int* firstSlot)
 {
+	*firstSlot = -1;
 	static const bool bigLds =
 		hipFuncSetAttribute(reinterpret_cast<const void*>(segHeapLdsKernel), hipFuncAttributeMaxDynamicSharedMemorySize,
 							16384 * 8) == hipSuccess;
@@ -446,8 +477,11 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 	{
 		if (caps[t] <= lo) break;
 		if ((e = hipStreamWaitEvent(b.aux[t], b.auxEv[3], 0)) != hipSuccess) return e;
+		const int slot = b.timer ? b.timer->begin(FLYEHIP_K_HEAP, b.aux[t]) : -1;
+		if (t == 0) *firstSlot = slot;
 		hipLaunchKernelGGL(segHeapLdsKernel, dim3(grid), dim3(FH_WAVE), static_cast<size_t>(caps[t]) * 8, b.aux[t], list,
 						   n, onlyFlagged, srcK, srcV, dstK, dstV, lo, caps[t]);
+		if (b.timer) b.timer->end(slot, b.aux[t]);
 		if ((e = hipEventRecord(b.auxEv[t], b.aux[t])) != hipSuccess) return e;
 		if ((e = hipStreamWaitEvent(s, b.auxEv[t], 0)) != hipSuccess) return e;
 		lo = caps[t];
@@ -457,17 +491,21 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 	return hipGetLastError();
 }
 
+// Leaf bytes: key and value read, value written (12 per element).
 hipError_t runLeaves(SegSortBufs& b, hipStream_t s)
 {
 	const int64_t nLeaf = b.hostCounters[kSegNLeaf];
 	if (nLeaf > 0)
 	{
+		const int t = b.timer ? b.timer->begin(FLYEHIP_K_LEAF, s, 12.0 * static_cast<double>(b.hostCounters[kSegLeafElems])) : -1;
 		hipLaunchKernelGGL(segLeafKernel, dim3(static_cast<unsigned>(nLeaf)), dim3(FH_WAVE), 0, s, b, nLeaf);
+		if (b.timer) b.timer->end(t, s);
 		hipError_t e = hipGetLastError();
 		if (e != hipSuccess) return e;
 	}
 	b.hostCounters[kSegNLeaf] = 0;
-	return hipMemsetAsync(b.counters + kSegNLeaf, 0, sizeof(int64_t), s);
+	b.hostCounters[kSegLeafElems] = 0;
+	return hipMemsetAsync(b.counters + kSegNLeaf, 0, 2 * sizeof(int64_t), s);
 }
 
 } // namespace
@@ -528,6 +566,9 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 		}
 		const SegRange* rs = b.big[cur];
 		const int32_t r32 = static_cast<int32_t>(nR);
+		// round bytes: keys read by the count and the rank passes, swapped
+		// pairs read and written (at most half the range): 16 per element
+		const int tround = b.timer ? b.timer->begin(FLYEHIP_K_ROUNDS, s, 16.0 * static_cast<double>(b.hostCounters[kSegBigElems])) : -1;
 		hipLaunchKernelGGL(segMedianKernel, dim3(blocksFor(nR + 1, 256)), dim3(256), 0, s, b, rs, r32);
 		size_t tb = b.scanTempBytes;
 		// chunk counts were written to chunkOff; scan them into chunkPre, then
@@ -546,9 +587,10 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 		if (e != hipSuccess) return e;
 		hipLaunchKernelGGL(segRankKernel, dim3(blocksFor(nChunks, kWavesPerBlock)), dim3(tpb), 0, s, b, rs, nChunks);
 		hipLaunchKernelGGL(segSwapKernel, dim3(blocksFor(nChunks, kWavesPerBlock)), dim3(tpb), 0, s, b, rs, nChunks);
-		e = hipMemsetAsync(b.counters + kSegNBig, 0, 2 * sizeof(int64_t), s);
+		e = hipMemsetAsync(b.counters + kSegNBig, 0, 3 * sizeof(int64_t), s);
 		if (e != hipSuccess) return e;
 		hipLaunchKernelGGL(segSplitKernel, dim3(blocksFor(nR, 256)), dim3(256), 0, s, b, rs, r32, b.big[cur ^ 1]);
+		if (b.timer) b.timer->end(tround, s);
 		e = hipGetLastError();
 		if (e != hipSuccess) return e;
 		e = download(b, s);
@@ -569,19 +611,30 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 	if (std::getenv("FLYEHIP_SEGSORT_TRACE"))
 		std::fprintf(stderr, "[segsort] rounds=%d verify=%lld free-heap=%lld\n", rounds,
 					 static_cast<long long>(b.hostCounters[kSegNVerify]), static_cast<long long>(nHeap));
+	// heap bytes: key and value read, value written (12 per element)
 	if (nHeap > 0)
 	{
 		// free-mode ranges out of budget: any sort will do; keys stay in place
-		e = heapTiers(b, b.heap, nHeap, 0, b.key, b.val, b.key, b.outVal, b.key, b.val, s);
+		int slot = -1;
+		e = heapTiers(b, b.heap, nHeap, 0, b.key, b.val, b.key, b.outVal, b.key, b.val, s, &slot);
 		if (e != hipSuccess) return e;
+		if (b.timer) b.timer->addBytes(slot, 12.0 * static_cast<double>(b.hostCounters[kSegHeapElems]));
 	}
 	const int64_t nVerify = b.hostCounters[kSegNVerify];
 	if (nVerify > 0)
 	{
 		const int tpb = FH_WAVE * kWavesPerBlock;
 		hipLaunchKernelGGL(segVerifyKernel, dim3(blocksFor(nVerify, kWavesPerBlock)), dim3(tpb), 0, s, b, nVerify);
-		e = heapTiers(b, b.verify, nVerify, 1, b.shadowKey, b.shadowVal, nullptr, b.outVal, b.shadowKey, b.shadowVal, s);
+		int slot = -1;
+		e = heapTiers(b, b.verify, nVerify, 1, b.shadowKey, b.shadowVal, nullptr, b.outVal, b.shadowKey, b.shadowVal, s,
+					  &slot);
 		if (e != hipSuccess) return e;
+		if (b.timer)
+		{
+			// the exact element count is known once the verification has run
+			if ((e = download(b, s)) != hipSuccess) return e;
+			b.timer->addBytes(slot, 12.0 * static_cast<double>(b.hostCounters[kSegExactElems]));
+		}
 		if (std::getenv("FLYEHIP_SEGSORT_TRACE"))
 		{
 			SegRange* hv = new SegRange[static_cast<size_t>(nVerify)];

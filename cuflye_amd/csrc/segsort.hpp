@@ -33,6 +33,7 @@
 
 #include "exsort.hpp"
 #include "exsort_wave.hpp"
+#include "ktimer.hpp"
 
 namespace fh
 {
@@ -59,12 +60,16 @@ enum SegCounter
 {
 	kSegNBig = 0,     // ranges in the list being filled
 	kSegBigChunks,    // their chunk total
+	kSegBigElems,     // their element total
 	kSegNLeaf,
-	kSegNHeap,
+	kSegLeafElems,
+	kSegNHeap,        // free-mode ranges out of budget
+	kSegHeapElems,
 	kSegTieArrays,    // arrays with at least one tied position (pruned sorts)
 	kSegError,        // set when a partition produced an impossible cut or swap
 	kSegNVerify,      // exact-mode ranges whose depth budget ran out
-	kSegCounters = 8
+	kSegExactElems,   // elements of the verify entries that needed the exact heap sort
+	kSegCounters = 12
 };
 
 struct SegSortBufs
@@ -100,6 +105,7 @@ struct SegSortBufs
 	int64_t* hostCounters;  // pinned, kSegCounters words
 	hipStream_t aux[3];     // default-priority side streams for the heap-sort size classes
 	hipEvent_t auxEv[4];    // (timing disabled)
+	KTimer* timer;          // kernel-family intervals (may be null)
 };
 
 struct SegSortCaps
@@ -146,12 +152,18 @@ __device__ inline void segPushWave(const SegSortBufs& b, SegRange* bigList, bool
 	{
 		const unsigned long long i = segAppend(&b.counters[kSegNLeaf], m);
 		if (toLeaf) b.leaf[i] = SegRange{f, l, d, tag};
+		const long long e = waveSum64(toLeaf ? n : 0);
+		if (laneId() == __ffsll(static_cast<long long>(m)) - 1)
+			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegLeafElems]), static_cast<unsigned long long>(e));
 	}
 	m = __ballot(toHeap);
 	if (m)
 	{
 		const unsigned long long i = segAppend(&b.counters[kSegNHeap], m);
 		if (toHeap) b.heap[i] = SegRange{f, l, 0, tag};
+		const long long e = waveSum64(toHeap ? n : 0);
+		if (laneId() == __ffsll(static_cast<long long>(m)) - 1)
+			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegHeapElems]), static_cast<unsigned long long>(e));
 	}
 	m = __ballot(toVerify);
 	if (m)
@@ -170,10 +182,15 @@ __device__ inline void segPushWave(const SegSortBufs& b, SegRange* bigList, bool
 	{
 		const int64_t chunks = toBig ? (static_cast<int64_t>(n) - 1 + kSegChunk - 1) / kSegChunk : 0;
 		const int64_t chunkSum = waveSum64(chunks);
+		const int64_t elemSum = waveSum64(toBig ? n : 0);
 		const unsigned long long i = segAppend(&b.counters[kSegNBig], m);
 		if (laneId() == __ffsll(static_cast<long long>(m)) - 1)
+		{
 			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegBigChunks]),
 					  static_cast<unsigned long long>(chunkSum));
+			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegBigElems]),
+					  static_cast<unsigned long long>(elemSum));
+		}
 		if (toBig) bigList[i] = SegRange{f, l, d, tag};
 	}
 }

@@ -68,10 +68,19 @@ class Stats(C.Structure):
                 ("ms_group", C.c_double), ("ms_chain", C.c_double), ("ms_extract", C.c_double),
                 ("queries", C.c_int64), ("windows", C.c_int64), ("matches", C.c_int64),
                 ("groups", C.c_int64), ("overlaps", C.c_int64), ("batches", C.c_int64),
-                ("device_bytes_peak", C.c_int64), ("tie_queries", C.c_int64)]
+                ("device_bytes_peak", C.c_int64), ("tie_queries", C.c_int64),
+                ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_int64 * 8),
+                ("kernel_bytes", C.c_double * 8)]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if name.startswith("kernel_") else v
+        return out
+
+
+KERNEL_FAMILIES = ("join", "radix", "leaf", "heap", "prefilter", "dp", "walk", "rounds")
 
 
 CANDIDATE_DTYPE = np.dtype([("query_id", "<i8"), ("target_id", "<i8"), ("kmer", "<u8"),

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FLYEHIP_ABI_VERSION 1
+#define FLYEHIP_ABI_VERSION 2
 
 /* Status codes. */
 #define FLYEHIP_OK 0
@@ -178,7 +178,24 @@ typedef struct {
     int64_t batches;
     int64_t device_bytes_peak;
     int64_t tie_queries;     /* queries whose match sort needed tie emulation */
+    /* Kernel families (FLYEHIP_K_*): GPU time summed over the launches (HIP
+     * events on the stream each launch runs on; launches on different
+     * streams overlap, so the sum can exceed ms_total), launch count, and
+     * algorithmic bytes (the per-unit figures documented with the roofline
+     * times the units the launches processed). */
+    double kernel_ms[8];
+    int64_t kernel_launches[8];
+    double kernel_bytes[8];
 } flyehip_stats_t;
+
+#define FLYEHIP_K_JOIN 0       /* window count + match fill                 */
+#define FLYEHIP_K_RADIX 1      /* stable segmented radix sort of the matches */
+#define FLYEHIP_K_LEAF 2       /* exact leaf sorts (LDS tiles), all sorts    */
+#define FLYEHIP_K_HEAP 3       /* heap-sort fallbacks, all sorts             */
+#define FLYEHIP_K_PREFILTER 4  /* group prefilters                           */
+#define FLYEHIP_K_DP 5         /* chaining DP                                */
+#define FLYEHIP_K_WALK 6       /* backtrack, overlapTest, selection          */
+#define FLYEHIP_K_ROUNDS 7     /* partition rounds of the segmented sorts    */
 
 /* ---- context ---------------------------------------------------------- */
 
