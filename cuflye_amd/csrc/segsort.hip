@@ -540,6 +540,7 @@ hipError_t segPushOffsets(const SegSortBufs& b, const int64_t* off, int32_t nArr
 
 hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 {
+	if (b.dup && b.outVal == b.val) return hipErrorInvalidValue;
 	hipError_t e = download(b, s);
 	if (e != hipSuccess) return e;
 	if (tieArrays) *tieArrays = b.hostCounters[kSegTieArrays];

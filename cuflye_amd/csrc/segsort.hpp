@@ -80,7 +80,8 @@ struct SegSortBufs
 	uint32_t relevantBit;   // 0, or a value bit marking the elements whose mutual order
 							// matters; a free-mode result is then accepted unless a run of
 							// equal keys holds two marked elements (outVal == val only)
-	const int64_t* dup;     // tied-position prefix counts, n + 1 words (null: no pruning)
+	const int64_t* dup;     // tied-position prefix counts, n + 1 words (null: no pruning;
+							// pruning needs a separate outVal holding the stable order)
 	int32_t* posL;          // one int per element
 	int32_t* posR;
 	SegRange* big[2];       // ping-pong lists of big ranges
