@@ -161,11 +161,11 @@ def main() -> int:
     if args.batch and args.dp_lanes and args.flights:
         g.set_tuning(dp_lanes_per_group=args.dp_lanes, queries_per_batch=args.batch, batches_in_flight=args.flights)
     else:
-        # a sample of several batches, so that batches in flight can overlap
-        sample = ids[: min(len(ids), 4096)]
-        tuning = autotune.tune(g, sample, cfg, batches=(args.batch,) if args.batch else (512, 1024),
+        # the whole workload: batch sizes are only comparable at the real
+        # number of batches (a sample of a few batches hides the overlap)
+        tuning = autotune.tune(g, ids, cfg, batches=(args.batch,) if args.batch else (1024, 2048, 3072, 4096),
                                lanes=(args.dp_lanes,) if args.dp_lanes else (16, 32, 64),
-                               flights=(args.flights,) if args.flights else (1, 2, 3), log=log)
+                               flights=(args.flights,) if args.flights else (2, 3), log=log)
         log(f"autotune best: {tuning['best']}")
 
     def step():

@@ -227,6 +227,36 @@ __global__ void passScatterKernel(const int32_t* order, const int32_t* flag, con
 	}
 }
 
+// A verify range lies inside one group's hits; groups are stored in hit
+// order, so the group is the last one starting at or before the range.
+__global__ void deferGroupsKernel(const SegRange* verify, int64_t nVerify, const GroupDesc* groups, int32_t nGroups,
+								  int32_t* deferred)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= nVerify) return;
+	const SegRange w = verify[i];
+	if (w.d != 1) return;
+	int lo = 0, hi = nGroups - 1;
+	while (lo < hi)
+	{
+		const int mid = lo + (hi - lo + 1) / 2;
+		if (groups[mid].hitStart <= w.f) lo = mid;
+		else hi = mid - 1;
+	}
+	deferred[lo] = 1;
+}
+
+__global__ void splitPassKernel(const int32_t* order, const int32_t* groupPass, const int32_t* deferred,
+								int32_t nGroups, int32_t* flagReady, int32_t* flagDefer)
+{
+	const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+	if (qi >= nGroups) return;
+	const int g = order[qi];
+	const bool pass = groupPass[g] != 0, later = deferred[g] != 0;
+	flagReady[qi] = pass && !later;
+	flagDefer[qi] = pass && later;
+}
+
 // Group-order hit copies for the DP: hits in (extId, curPos) order, or in the
 // extPos order of the sort when extLen > curLen.
 __global__ void __launch_bounds__(FH_WAVE) chainGatherKernel(ChainArgs a)
@@ -773,6 +803,24 @@ hipError_t launchPassScatter(const ChainArgs& a, const int32_t* flag, const int6
 	if (a.nGroups == 0) return hipSuccess;
 	hipLaunchKernelGGL(passScatterKernel, dim3((a.nGroups + 255) / 256), dim3(256), 0, stream, a.order, flag, pos,
 					   a.nGroups, a.groups, kWalkSmall, passOrder, nBig);
+	return hipGetLastError();
+}
+
+hipError_t launchDeferGroups(const SegRange* verify, int64_t nVerify, const ChainArgs& a, int32_t* deferred,
+							 hipStream_t stream)
+{
+	if (nVerify == 0 || a.nGroups == 0) return hipSuccess;
+	hipLaunchKernelGGL(deferGroupsKernel, dim3(static_cast<unsigned>((nVerify + 255) / 256)), dim3(256), 0, stream,
+					   verify, nVerify, a.groups, a.nGroups, deferred);
+	return hipGetLastError();
+}
+
+hipError_t launchSplitPass(const ChainArgs& a, const int32_t* deferred, int32_t* flagReady, int32_t* flagDefer,
+						   hipStream_t stream)
+{
+	if (a.nGroups == 0) return hipSuccess;
+	hipLaunchKernelGGL(splitPassKernel, dim3((a.nGroups + 255) / 256), dim3(256), 0, stream, a.order, a.groupPass,
+					   deferred, a.nGroups, flagReady, flagDefer);
 	return hipGetLastError();
 }
 

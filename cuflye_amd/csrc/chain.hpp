@@ -95,6 +95,14 @@ hipError_t launchChainPrefilter(const ChainArgs& a, const SegSortBufs& sb, int b
 hipError_t launchPassFlags(const ChainArgs& a, int32_t* flag, hipStream_t stream);
 hipError_t launchPassScatter(const ChainArgs& a, const int32_t* flag, const int64_t* pos, int32_t* passOrder,
 							 int64_t* nBig, hipStream_t stream);
+// Groups holding an extPos-sort range whose exact heap sort is still running
+// (verify entries with d == 1) get deferred[g] = 1 (deferred zeroed by the
+// caller); the split flags then divide the passing groups of the processing
+// order into those whose DP can run meanwhile and the deferred ones.
+hipError_t launchDeferGroups(const SegRange* verify, int64_t nVerify, const ChainArgs& a, int32_t* deferred,
+							 hipStream_t stream);
+hipError_t launchSplitPass(const ChainArgs& a, const int32_t* deferred, int32_t* flagReady, int32_t* flagDefer,
+						   hipStream_t stream);
 hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream);
 hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t stream);
 // score-order keys; pushes the score sorts to sb

@@ -91,7 +91,7 @@ struct Lane
 	flyehip_ctx* c = nullptr;
 	hipStream_t stream = nullptr;
 	hipStream_t aux[3] = {};    // side streams (default priority)
-	hipEvent_t auxEv[4] = {};   // timing disabled
+	hipEvent_t auxEv[5] = {};   // timing disabled
 	hipEvent_t ev[8] = {};
 	int64_t* hostSeg = nullptr;  // pinned counter words of the segmented sort
 	flyehip_stats_t stats{};     // stage timings and counts of this lane's batches
@@ -100,6 +100,7 @@ struct Lane
 	DevBuf bMKey, bMExt, bVal, bPosL, bPosR, bHCur, bHExt, bGroupFlag, bGroupOff;
 	DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
 	DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
+	DevBuf bGroupDefer, bFlagReady, bFlagDefer, bPosReady, bPosDefer, bReadyOrder, bDeferOrder;
 	DevBuf bGroupPass, bOrder, bPassOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
 	DevBuf bSegBig0, bSegBig1, bSegLeaf, bSegHeap, bSegCounters, bSegChunkOff, bSegChunkRange, bSegChunkCnt;
 	DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp, bSegVerify, bSegShadowK, bSegShadowV;
@@ -518,15 +519,14 @@ SegSortBufs segBufs(Lane* L, int64_t n, int64_t nArrays)
 	b.chunkCap = caps.chunkCap;
 	b.hostCounters = L->hostSeg;
 	for (int t = 0; t < 3; ++t) b.aux[t] = L->aux[t];
-	for (int t = 0; t < 4; ++t) b.auxEv[t] = L->auxEv[t];
+	for (int t = 0; t < 5; ++t) b.auxEv[t] = L->auxEv[t];
 	b.timer = &L->timer;
 	return b;
 }
 
-void runSegSort(Lane* L, SegSortBufs& b, int64_t* tieArrays)
+void runSegSort(Lane* L, SegSortBufs& b, int64_t* tieArrays, bool* exactPending = nullptr)
 {
-	flyehip_ctx* c = L->c;
-	const hipError_t e = segSortRun(b, L->stream, tieArrays);
+	const hipError_t e = segSortRun(b, L->stream, tieArrays, exactPending);
 	if (e == hipErrorIllegalState) fail(FLYEHIP_ERR_HIP, "segmented sort: inconsistent partition");
 	HIPCHECK(e);
 }
@@ -601,7 +601,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	int64_t* nBig = ensureT<int64_t>(c, L->bCounters, 16);
 	zero32(L, bins, 96);
 	zero32(L, queues, 16);
-	HIPCHECK(hipMemsetAsync(nBig, 0, 2 * sizeof(int64_t), L->stream));
+	HIPCHECK(hipMemsetAsync(nBig, 0, 6 * sizeof(int64_t), L->stream));
 	SegSortBufs sb = segBufs(L, nHits, ng);
 	sb.key = ca.sortKey;
 	sb.val = ca.sortVal;
@@ -619,26 +619,88 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	const int tp = L->timer.begin(FLYEHIP_K_PREFILTER, L->stream, 8.0 * static_cast<double>(nHits));
 	HIPCHECK(launchChainPrefilter(ca, sb, blocks, L->stream));
 	L->timer.end(tp, L->stream);
-	runSegSort(L, sb, nullptr);
+	// the extPos sort leaves its exact heap sorts (libstdc++'s heapsort
+	// fallback, a dependent chain per range) running on the side streams: the
+	// groups that hold such a range are deferred, and the gather + DP of all
+	// other passing groups run meanwhile
+	bool exactPending = false;
+	// should anything below throw, the lane stream still waits for the
+	// pending heap sorts before its next batch reuses their buffers
+	struct JoinOnExit
+	{
+		hipStream_t s;
+		hipEvent_t e;
+		bool* pending;
+		~JoinOnExit()
+		{
+			if (*pending) (void)hipStreamWaitEvent(s, e, 0);
+		}
+	} joinOnExit{L->stream, L->auxEv[4], &exactPending};
+	runSegSort(L, sb, nullptr, &exactPending);
 	HIPCHECK(launchPassFlags(ca, flag, L->stream));
 	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
 	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream));
+	int32_t* readyOrder = passOrder;
+	int32_t* deferOrder = nullptr;
+	int64_t* posReady = nullptr;
+	if (exactPending)
+	{
+		int32_t* deferred = ensureT<int32_t>(c, L->bGroupDefer, static_cast<size_t>(ng) + 1);
+		int32_t* flagReady = ensureT<int32_t>(c, L->bFlagReady, static_cast<size_t>(ng) + 1);
+		int32_t* flagDefer = ensureT<int32_t>(c, L->bFlagDefer, static_cast<size_t>(ng) + 1);
+		int64_t* posDefer = ensureT<int64_t>(c, L->bPosDefer, static_cast<size_t>(ng) + 1);
+		posReady = ensureT<int64_t>(c, L->bPosReady, static_cast<size_t>(ng) + 1);
+		readyOrder = ensureT<int32_t>(c, L->bReadyOrder, static_cast<size_t>(ng) + 1);
+		deferOrder = ensureT<int32_t>(c, L->bDeferOrder, static_cast<size_t>(ng) + 1);
+		zero32(L, deferred, ng);
+		HIPCHECK(launchDeferGroups(sb.verify, sb.hostCounters[kSegNVerify], ca, deferred, L->stream));
+		HIPCHECK(launchSplitPass(ca, deferred, flagReady, flagDefer, L->stream));
+		HIPCHECK(exclusiveScan(flagReady, posReady, ng, tmp, L->stream));
+		HIPCHECK(launchPassScatter(ca, flagReady, posReady, readyOrder, nBig + 2, L->stream));
+		HIPCHECK(exclusiveScan(flagDefer, posDefer, ng, tmp, L->stream));
+		HIPCHECK(launchPassScatter(ca, flagDefer, posDefer, deferOrder, nBig + 4, L->stream));
+	}
 	int64_t counts[3] = {0, 0, 0};
+	int64_t split[5] = {0, 0, 0, 0, 0};  // ready (big groups, hits), deferred (big groups, hits), ready groups
 	download(L, &counts[0], pos + ng, sizeof(int64_t));
 	download(L, &counts[1], nBig, 2 * sizeof(int64_t));
+	if (exactPending)
+	{
+		download(L, split, nBig + 2, 4 * sizeof(int64_t));
+		download(L, &split[4], posReady + ng, sizeof(int64_t));
+	}
 	sync(L);
-	ca.passOrder = passOrder;
-	ca.nPass = static_cast<int32_t>(counts[0]);
+	const int32_t nPassAll = static_cast<int32_t>(counts[0]);
 	const int32_t nBigPass = static_cast<int32_t>(counts[1]);
 	const double passHits = static_cast<double>(counts[2]);
+	const int32_t nReady = exactPending ? static_cast<int32_t>(split[4]) : nPassAll;
+	const double readyHits = exactPending ? static_cast<double>(split[1]) : passHits;
+	const double deferHits = static_cast<double>(split[3]);
+	ca.passOrder = readyOrder;
+	ca.nPass = nReady;
 	ca.queue = queues + 1;
 	HIPCHECK(launchChainGather(ca, blocks, L->stream));
 	HIPCHECK(hipEventRecord(L->ev[3], L->stream));
 	ca.queue = queues + 2;
 	// DP: hits read (hcur, hext) and score / backtrack written, 16 bytes per hit
-	const int td = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * passHits);
+	const int td = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * readyHits);
 	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, L->stream));
 	L->timer.end(td, L->stream);
+	if (deferOrder)
+	{
+		exactPending = false;
+		HIPCHECK(hipStreamWaitEvent(L->stream, L->auxEv[4], 0));
+		ca.passOrder = deferOrder;
+		ca.nPass = nPassAll - nReady;
+		ca.queue = queues + 6;
+		HIPCHECK(launchChainGather(ca, blocks, L->stream));
+		ca.queue = queues + 7;
+		const int td2 = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * deferHits);
+		HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, L->stream));
+		L->timer.end(td2, L->stream);
+		ca.passOrder = passOrder;
+		ca.nPass = nPassAll;
+	}
 	HIPCHECK(hipEventRecord(L->ev[4], L->stream));
 	HIPCHECK(segSortReset(sb, L->stream));
 	sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
@@ -801,7 +863,8 @@ void createLane(flyehip_ctx* c)
 	c->lanes.push_back(std::move(l));
 }
 
-void destroyLane(flyehip_ctx* c, Lane& l)
+// Releases a lane's per-batch device buffers; the next call sizes them again.
+void releaseBatchBuffers(flyehip_ctx* c, Lane& l)
 {
 	if (l.stream) (void)hipStreamSynchronize(l.stream);
 	for (DevBuf* b : {&l.bQueries, &l.bWinOff, &l.bWinCount, &l.bWinFilt, &l.bWinInfo, &l.bMatchOff, &l.bFiltOff,
@@ -814,8 +877,14 @@ void destroyLane(flyehip_ctx* c, Lane& l)
 					  &l.bSegChunkPre, &l.bSegRangeM, &l.bSegRangeLm, &l.bSegTemp, &l.bSegVerify, &l.bSegShadowK,
 					  &l.bSegShadowV, &l.bQueue, &l.bCounters, &l.bQRowCount, &l.bCurLen, &l.bDivOff, &l.bDivCount,
 					  &l.bDivOut, &l.bQueryIds, &l.bCand, &l.bExtWords, &l.bExtWordOff, &l.bExtLen, &l.bAln,
-					  &l.bAlnOff, &l.bRA})
+					  &l.bAlnOff, &l.bRA, &l.bGroupDefer, &l.bFlagReady, &l.bFlagDefer, &l.bPosReady, &l.bPosDefer,
+					  &l.bReadyOrder, &l.bDeferOrder})
 		release(c, *b);
+}
+
+void destroyLane(flyehip_ctx* c, Lane& l)
+{
+	releaseBatchBuffers(c, l);
 	if (l.hostSeg) (void)hipHostFree(l.hostSeg);
 	l.timer.destroy();
 	for (auto& e : l.ev)
@@ -1096,13 +1165,18 @@ int flyehip_set_tuning(flyehip_ctx* c, int32_t lanes, int32_t qpb, int32_t gpp)
 		if (qpb)
 		{
 			if (qpb < 1) fail(FLYEHIP_ERR_ARG, "queries_per_batch must be positive");
-			c->queriesPerBatch = qpb;
 		}
 		if (gpp)
 		{
 			if (gpp < 1 || gpp > 4) fail(FLYEHIP_ERR_ARG, "batches_in_flight %d not supported (1 .. 4)", gpp);
-			c->batchesInFlight = gpp;
 		}
+		// buffers sized for another batch size or number of batches in flight
+		// would hold device memory the new setting may need
+		const bool resize = (qpb && qpb != c->queriesPerBatch) || (gpp && gpp != c->batchesInFlight);
+		if (qpb) c->queriesPerBatch = qpb;
+		if (gpp) c->batchesInFlight = gpp;
+		if (resize)
+			for (auto& l : c->lanes) releaseBatchBuffers(c, *l);
 	});
 }
 
@@ -1270,7 +1344,6 @@ int flyehip_overlaps(flyehip_ctx* c, const int64_t* query_ids, int32_t n, const 
 					 float* div_stats, int64_t div_cap, int64_t* div_offsets)
 {
 	return guarded(c, [&]() {
-		Lane* L = c->lanes[0].get();
 		std::vector<JoinQuery> qs = residentQueries(c, query_ids, n);
 		overlapsImpl(c, storeSeqs(c), qs, query_ids, p, rows, row_cap, row_offsets, rows_needed, div_stats, div_cap,
 					 div_offsets, c->queriesPerBatch, 4);
@@ -1281,7 +1354,6 @@ int flyehip_overlaps_per_query(flyehip_ctx* c, const int64_t* query_ids, int32_t
 							   flyehip_overlap_t* rows, int64_t row_cap, int64_t* row_offsets, int64_t* rows_needed)
 {
 	return guarded(c, [&]() {
-		Lane* L = c->lanes[0].get();
 		std::vector<JoinQuery> qs = residentQueries(c, query_ids, n);
 		overlapsImpl(c, storeSeqs(c), qs, query_ids, p, rows, row_cap, row_offsets, rows_needed, nullptr, 0, nullptr, 1, 1);
 	});

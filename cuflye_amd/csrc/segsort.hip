@@ -459,10 +459,12 @@ hipError_t download(const SegSortBufs& b, hipStream_t s)
 // 8192 and 16384 elements: 16, 64 and 128 KiB per wave, so small ranges are
 // not held to one wave per CU by the largest), launched concurrently on the
 // side streams; ranges above 16384 elements sort in place on gk / gv in
-// global memory.  The calling stream waits for all of them.
+// global memory.  With join the calling stream waits for all of them;
+// otherwise they are joined on the first side stream, where auxEv[4] is
+// recorded after the last of them.
 hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int onlyFlagged, const uint32_t* srcK,
 					 const uint32_t* srcV, uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s,
-					 int* firstSlot)
+					 int* firstSlot, bool join = true)
 {
 	*firstSlot = -1;
 	static const bool bigLds =
@@ -473,6 +475,7 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 	hipError_t e = hipEventRecord(b.auxEv[3], s);
 	if (e != hipSuccess) return e;
 	int32_t lo = 0;
+	int last = 0;
 	for (int t = 0; t < 3; ++t)
 	{
 		if (caps[t] <= lo) break;
@@ -482,13 +485,29 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 		hipLaunchKernelGGL(segHeapLdsKernel, dim3(grid), dim3(FH_WAVE), static_cast<size_t>(caps[t]) * 8, b.aux[t], list,
 						   n, onlyFlagged, srcK, srcV, dstK, dstV, lo, caps[t]);
 		if (b.timer) b.timer->end(slot, b.aux[t]);
-		if ((e = hipEventRecord(b.auxEv[t], b.aux[t])) != hipSuccess) return e;
-		if ((e = hipStreamWaitEvent(s, b.auxEv[t], 0)) != hipSuccess) return e;
+		if (join)
+		{
+			if ((e = hipEventRecord(b.auxEv[t], b.aux[t])) != hipSuccess) return e;
+			if ((e = hipStreamWaitEvent(s, b.auxEv[t], 0)) != hipSuccess) return e;
+		}
 		lo = caps[t];
+		last = t;
 	}
-	hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, s, list, n, onlyFlagged, gk, gv, dstV,
-					   lo);
-	return hipGetLastError();
+	if (join)
+	{
+		hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, s, list, n, onlyFlagged, gk, gv,
+						   dstV, lo);
+		return hipGetLastError();
+	}
+	hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, b.aux[last], list, n, onlyFlagged, gk,
+					   gv, dstV, lo);
+	if ((e = hipGetLastError()) != hipSuccess) return e;
+	for (int t = 1; t <= last; ++t)
+	{
+		if ((e = hipEventRecord(b.auxEv[t], b.aux[t])) != hipSuccess) return e;
+		if ((e = hipStreamWaitEvent(b.aux[0], b.auxEv[t], 0)) != hipSuccess) return e;
+	}
+	return hipEventRecord(b.auxEv[4], b.aux[0]);
 }
 
 // Leaf bytes: key and value read, value written (12 per element).
@@ -538,8 +557,9 @@ hipError_t segPushOffsets(const SegSortBufs& b, const int64_t* off, int32_t nArr
 	return hipGetLastError();
 }
 
-hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
+hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* exactPending)
 {
+	if (exactPending) *exactPending = false;
 	if (b.dup && b.outVal == b.val) return hipErrorInvalidValue;
 	hipError_t e = download(b, s);
 	if (e != hipSuccess) return e;
@@ -628,8 +648,9 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays)
 		hipLaunchKernelGGL(segVerifyKernel, dim3(blocksFor(nVerify, kWavesPerBlock)), dim3(tpb), 0, s, b, nVerify);
 		int slot = -1;
 		e = heapTiers(b, b.verify, nVerify, 1, b.shadowKey, b.shadowVal, nullptr, b.outVal, b.shadowKey, b.shadowVal, s,
-					  &slot);
+					  &slot, exactPending == nullptr);
 		if (e != hipSuccess) return e;
+		if (exactPending) *exactPending = true;
 		if (b.timer)
 		{
 			// the exact element count is known once the verification has run

@@ -105,7 +105,8 @@ struct SegSortBufs
 	int64_t chunkCap;
 	int64_t* hostCounters;  // pinned, kSegCounters words
 	hipStream_t aux[3];     // default-priority side streams for the heap-sort size classes
-	hipEvent_t auxEv[4];    // (timing disabled)
+	hipEvent_t auxEv[5];    // (timing disabled) [0..2] size classes done, [3] fork, [4] deferred
+							// exact heap sorts done
 	KTimer* timer;          // kernel-family intervals (may be null)
 };
 
@@ -200,8 +201,13 @@ __device__ inline void segPushWave(const SegSortBufs& b, SegRange* bigList, bool
 hipError_t segSortReset(const SegSortBufs& b, hipStream_t s);
 
 // Runs the rounds and the leaf / heap passes over what the init kernels
-// pushed.  Synchronises the stream once per round (range counts).
-hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays);
+// pushed.  Synchronises the stream once per round (range counts).  With
+// exactPending set, the exact heap sorts of the verify entries that need one
+// (d == 1 after the verification on s) are left running on the side streams:
+// *exactPending tells whether any were launched, and b.auxEv[4] completes
+// with them; the caller makes its stream wait on that event before it reads
+// those ranges of outVal (or the side streams' timing intervals).
+hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* exactPending = nullptr);
 
 // Pushes every array [off[i], off[i+1]) (i < nArrays), pruned by b.dup.
 hipError_t segPushOffsets(const SegSortBufs& b, const int64_t* off, int32_t nArrays, hipStream_t s);

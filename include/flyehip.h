@@ -212,7 +212,9 @@ int flyehip_get_stats(const flyehip_ctx* ctx, flyehip_stats_t* out);
  * (1, 4, 8, 16, 32 or 64), the query batch size, and the batches in flight
  * (1 .. 4): flyehip_overlaps* deal batches to that many streams, each driven
  * by its own host thread, so one batch's tail overlaps the next batch (the
- * per-query path always uses one).  0 keeps the current value. */
+ * per-query path always uses one).  0 keeps the current value.  A new batch
+ * size or number of batches in flight releases the per-batch device buffers
+ * (each batch in flight holds its own; the next call sizes them again). */
 int flyehip_set_tuning(flyehip_ctx* ctx, int32_t dp_lanes_per_group,
                        int32_t queries_per_batch, int32_t batches_in_flight);
 
