@@ -56,12 +56,32 @@ __device__ inline unsigned long long laneMaskLt()
 }
 
 // Partition s[f, l) around the median moved to f; all 64 lanes call it.
-// posL / posR need (l - f) / 2 + 1 entries.
+// posL / posR need (l - f) / 2 + 1 entries.  freeSeed 0: std::sort's median
+// of (f + 1, mid, l - 1); otherwise ("free mode", where any correct sort will
+// do) the median of three distinct samples at a pseudo-random offset a third
+// of the range apart, which does not degenerate on the inputs that exhausted
+// std::sort's depth budget.
 template <class K>
-__device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int* posL, int* posR)
+__device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int* posL, int* posR,
+									uint32_t freeSeed = 0)
 {
 	const int lane = laneId();
-	if (lane == 0) exsort::medianToFirst(s, f, f + 1, f + (l - f) / 2, l - 1);
+	if (lane == 0)
+	{
+		if (freeSeed == 0)
+		{
+			exsort::medianToFirst(s, f, f + 1, f + (l - f) / 2, l - 1);
+		}
+		else
+		{
+			const uint32_t span = static_cast<uint32_t>(l - f - 1);
+			uint64_t h = (static_cast<uint64_t>(freeSeed) << 32) ^ (static_cast<uint64_t>(f) << 16) ^ static_cast<uint32_t>(l);
+			h = h * 6364136223846793005ULL + 1442695040888963407ULL;
+			const uint32_t o = static_cast<uint32_t>((h >> 33) % span);
+			exsort::medianToFirst(s, f, f + 1 + static_cast<int>(o), f + 1 + static_cast<int>((o + span / 3) % span),
+								  f + 1 + static_cast<int>((o + 2 * (span / 3)) % span));
+		}
+	}
 	waveSync();
 	const K p = s.k[f];
 	const int lo = f + 1;
@@ -112,13 +132,42 @@ __device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int*
 	return cut;
 }
 
+// Where a tile sort lists the sub-ranges std::sort would heap-sort (depth
+// budget spent, more than kLaneMax elements) instead of heap-sorting them one
+// lane each: their content goes to the shadow arrays, a verify entry (absolute
+// positions) is appended, and the wave sorts them on in free mode.  A later
+// pass heap-sorts, from the shadow copy, only the entries whose free result
+// holds a tie that matters (segsort.hpp).
+// A range of a segmented sort (segsort.hpp).
+struct SegRange
+{
+	int32_t f;    // absolute positions in the key / value arrays
+	int32_t l;
+	int32_t d;    // remaining depth budget (verify list: 1 = needs the exact heap sort)
+	int32_t tag;  // 0: exact mode; t > 0: free mode under verify entry t - 1
+};
+
+struct WaveVerify
+{
+	uint32_t* shadowKey;
+	uint32_t* shadowVal;
+	SegRange* list;
+	unsigned long long* count;
+	unsigned long long* error;  // bit 8 set if the list is full (cannot happen when sized as documented)
+	int64_t cap;
+};
+
+constexpr int kWaveFree = 1 << 30;  // WSeg.d flag: free mode
+constexpr int kWaveFreeDepth = 64;   // depth budget of a free-mode range
+
 // Sorts the range g[f, l) (l - f <= kTile) with remaining depth budget d,
 // through the wave's LDS tile; ends with the stable insertion pass.  The
 // sorted range goes back to g, or, with outV set, its values go to outV[f, l)
-// and its keys back to g only when keysBack is set.
+// and its keys back to g only when keysBack is set (or when a sub-range was
+// listed for verification, which reads the sorted keys).
 template <class K, int kTile, int kLaneMax>
 __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, WaveSortLds<K, kTile>& t,
-							 uint32_t* outV = nullptr, bool keysBack = false)
+							 uint32_t* outV = nullptr, bool keysBack = false, const WaveVerify* vfy = nullptr)
 {
 	const int lane = laneId();
 	const int n = l - f;
@@ -143,29 +192,55 @@ __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, Wave
 			// left (straight to heapsort): smaller part first keeps at most
 			// log2(kLaneMax / 16) + 1 entries on the stack
 			static_assert(kLaneMax <= 16 << 4, "lane stack bound");
-			exsort::introLoop<6>(s, w.f, w.l, w.d);
+			exsort::introLoop<6>(s, w.f, w.l, w.d & (kWaveFree - 1));
 			exsort::insertionSort(s, w.f, w.l);
 		}
 		nl = 0;
 		waveSync();
 	};
+	bool listed = false;
 	while (top > 0)
 	{
 		--top;
 		const WSeg w = t.stack[top];
 		waveSync();
-		if (w.l - w.f <= kLaneMax || w.d == 0)
+		const bool freeW = (w.d & kWaveFree) != 0;
+		const int dw = w.d & (kWaveFree - 1);
+		if (vfy && !freeW && dw == 0 && w.l - w.f > kLaneMax)
+		{
+			for (int i = w.f + lane; i < w.l; i += FH_WAVE)
+			{
+				vfy->shadowKey[f + i] = static_cast<uint32_t>(t.k[i]);
+				vfy->shadowVal[f + i] = t.v[i];
+			}
+			if (lane == 0)
+			{
+				const unsigned long long vi = atomicAdd(vfy->count, 1ULL);
+				if (static_cast<int64_t>(vi) < vfy->cap)
+					vfy->list[vi] = SegRange{f + w.f, f + w.l, 0, 0};
+				else
+					atomicOr(vfy->error, 256ULL);
+				t.stack[top] = WSeg{w.f, w.l, kWaveFreeDepth | kWaveFree};
+			}
+			top += 1;
+			listed = true;
+			waveSync();
+			continue;
+		}
+		if (w.l - w.f <= kLaneMax || dw == 0)
 		{
 			if (lane == 0) t.lanes[nl] = w;
 			++nl;
 			if (nl == FH_WAVE) flush();
 			continue;
 		}
-		const int cut = wavePartition(s, w.f, w.l, t.posL, t.posR);
+		const int cut = wavePartition(s, w.f, w.l, t.posL, t.posR,
+									  freeW ? static_cast<uint32_t>(f + w.f) * 2654435761u + 1u : 0u);
 		if (lane == 0)
 		{
 			// larger part below, smaller on top: the stack stays within log2(n) + 1
-			const WSeg a{w.f, cut, w.d - 1}, b{cut, w.l, w.d - 1};
+			const int dd = (dw - 1) | (freeW ? kWaveFree : 0);
+			const WSeg a{w.f, cut, dd}, b{cut, w.l, dd};
 			const bool leftSmaller = cut - w.f < w.l - cut;
 			t.stack[top] = leftSmaller ? b : a;
 			t.stack[top + 1] = leftSmaller ? a : b;
@@ -176,10 +251,11 @@ __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, Wave
 	if (nl) flush();
 	if (outV)
 	{
+		const bool kb = keysBack || listed;
 		for (int i = lane; i < n; i += FH_WAVE)
 		{
 			outV[f + i] = t.v[i];
-			if (keysBack) g.k[f + i] = t.k[i];
+			if (kb) g.k[f + i] = t.k[i];
 		}
 	}
 	else

@@ -501,7 +501,7 @@ SegSortBufs segBufs(Lane* L, int64_t n, int64_t nArrays)
 	b.big[1] = ensureT<SegRange>(c, L->bSegBig1, static_cast<size_t>(caps.bigCap));
 	b.leaf = ensureT<SegRange>(c, L->bSegLeaf, static_cast<size_t>(caps.leafCap));
 	b.heap = ensureT<SegRange>(c, L->bSegHeap, static_cast<size_t>(caps.heapCap));
-	b.verify = ensureT<SegRange>(c, L->bSegVerify, static_cast<size_t>(caps.heapCap));
+	b.verify = ensureT<SegRange>(c, L->bSegVerify, static_cast<size_t>(caps.verifyCap));
 	b.shadowKey = ensureT<uint32_t>(c, L->bSegShadowK, static_cast<size_t>(n) + 1);
 	b.shadowVal = ensureT<uint32_t>(c, L->bSegShadowV, static_cast<size_t>(n) + 1);
 	b.counters = ensureT<int64_t>(c, L->bSegCounters, kSegCounters);
@@ -516,6 +516,7 @@ SegSortBufs segBufs(Lane* L, int64_t n, int64_t nArrays)
 	b.bigCap = caps.bigCap;
 	b.leafCap = caps.leafCap;
 	b.heapCap = caps.heapCap;
+	b.verifyCap = caps.verifyCap;
 	b.chunkCap = caps.chunkCap;
 	b.hostCounters = L->hostSeg;
 	for (int t = 0; t < 3; ++t) b.aux[t] = L->aux[t];

@@ -249,9 +249,16 @@ __global__ void __launch_bounds__(FH_WAVE) segLeafKernel(SegSortBufs b, int64_t 
 	const int64_t i = blockIdx.x;
 	if (i >= nLeaf) return;
 	const SegRange w = b.leaf[i];
-	// free mode: the sorted keys stay in place for the verification
-	waveTileSort<uint32_t, kSegTile, 48>(exsort::Span<uint32_t>{b.key, b.val}, w.f, w.l, w.d, lds, b.outVal,
-										 w.tag != 0);
+	// exact-mode sub-ranges out of budget: shadow copy + verify entry + free
+	// mode, as for big ranges (verify entries need no shadow kernel)
+	const WaveVerify vfy{b.shadowKey, b.shadowVal, b.verify, reinterpret_cast<unsigned long long*>(&b.counters[kSegNVerify]),
+						 reinterpret_cast<unsigned long long*>(&b.counters[kSegError]), b.verifyCap};
+	// free mode (a leaf of a range listed for verification): free pivots, no
+	// nested verify entries, and the sorted keys stay in place for the
+	// verification of the enclosing entry
+	const bool freeLeaf = w.tag != 0;
+	waveTileSort<uint32_t, kSegTile, kSegLaneMax>(exsort::Span<uint32_t>{b.key, b.val}, w.f, w.l,
+												  w.d | (freeLeaf ? kWaveFree : 0), lds, b.outVal, freeLeaf, &vfy);
 }
 
 // exsort::siftStepPacked without branches: both children are loaded
@@ -510,8 +517,10 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 	return hipEventRecord(b.auxEv[4], b.aux[0]);
 }
 
-// Leaf bytes: key and value read, value written (12 per element).
-hipError_t runLeaves(SegSortBufs& b, hipStream_t s)
+// Leaf bytes: key and value read, value written (12 per element).  The leaves
+// append verify entries with their shadow copies already taken: *nVerified
+// moves past them (the round loop copies only the entries its splits add).
+hipError_t runLeaves(SegSortBufs& b, hipStream_t s, int64_t* nVerified)
 {
 	const int64_t nLeaf = b.hostCounters[kSegNLeaf];
 	if (nLeaf > 0)
@@ -522,9 +531,16 @@ hipError_t runLeaves(SegSortBufs& b, hipStream_t s)
 		hipError_t e = hipGetLastError();
 		if (e != hipSuccess) return e;
 	}
+	hipError_t e = hipMemsetAsync(b.counters + kSegNLeaf, 0, 2 * sizeof(int64_t), s);
+	if (e != hipSuccess) return e;
+	if (nLeaf > 0)
+	{
+		if ((e = download(b, s)) != hipSuccess) return e;
+		*nVerified = b.hostCounters[kSegNVerify];
+	}
 	b.hostCounters[kSegNLeaf] = 0;
 	b.hostCounters[kSegLeafElems] = 0;
-	return hipMemsetAsync(b.counters + kSegNLeaf, 0, 2 * sizeof(int64_t), s);
+	return hipSuccess;
 }
 
 } // namespace
@@ -534,6 +550,9 @@ hipError_t segSortCaps(int64_t n, int64_t nArrays, SegSortCaps* caps)
 	caps->bigCap = n / (kSegTile + 1) + 2;
 	caps->leafCap = nArrays + 2 * caps->bigCap + 64;
 	caps->heapCap = caps->bigCap;
+	// verify entries are disjoint ranges of more than kSegTile (rounds) or
+	// kSegLaneMax (leaves) elements
+	caps->verifyCap = n / (kSegLaneMax + 1) + caps->bigCap + 64;
 	caps->chunkCap = n / kSegChunk + caps->bigCap + 2;
 	size_t a = 0, c = 0;
 	hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, a, static_cast<const int64_t*>(nullptr),
@@ -582,7 +601,7 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* e
 		if (nR > b.bigCap || nChunks > b.chunkCap) return hipErrorInvalidValue;
 		if (b.hostCounters[kSegNLeaf] + 2 * nR > b.leafCap)
 		{
-			e = runLeaves(b, s);
+			e = runLeaves(b, s, &nVerified);
 			if (e != hipSuccess) return e;
 		}
 		const SegRange* rs = b.big[cur];
@@ -626,7 +645,7 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* e
 		cur ^= 1;
 	}
 	if (b.hostCounters[kSegError]) return hipErrorIllegalState;
-	e = runLeaves(b, s);
+	e = runLeaves(b, s, &nVerified);
 	if (e != hipSuccess) return e;
 	const int64_t nHeap = b.hostCounters[kSegNHeap];
 	if (std::getenv("FLYEHIP_SEGSORT_TRACE"))

@@ -18,7 +18,12 @@
 //     range is saved ("shadow" copy) and sorted on in free mode (partitions
 //     without a depth budget, children tagged with the range); afterwards its
 //     sorted keys are checked for an equal neighbour pair, and only a range
-//     that has one is heap-sorted from the shadow copy (one lane, in LDS).
+//     that has one is heap-sorted from the shadow copy (one lane, in LDS);
+//   * inside a leaf, a sub-range of more than kSegLaneMax elements whose
+//     budget is spent is handled the same way by the leaf's wave (shadow
+//     copy, verify entry, free-mode sort) instead of being heap-sorted by one
+//     lane, so the leaves of nearly sorted inputs (where std::sort exhausts its
+//     budget everywhere) stay parallel.
 // Optionally a range is dropped when the sorted order already known for it is
 // final: with `dup` set (exclusive prefix counts over the stable sorted order
 // of the positions whose key is shared with a neighbour, "tied" positions), a
@@ -42,18 +47,12 @@ namespace fh
 #define FH_SEG_TILE 1024
 #endif
 constexpr int kSegTile = FH_SEG_TILE;  // leaf size, elements (LDS tile of one wave)
+constexpr int kSegLaneMax = 48;        // leaf sub-ranges finished by one lane
 constexpr int kSegRows = 8;           // 64-element rows per chunk
 constexpr int kSegChunk = 64 * kSegRows;
 
 constexpr int kSegFreeDepth = 64;      // depth budget of a free-mode range
 
-struct SegRange
-{
-	int32_t f;    // absolute positions in the key / value arrays
-	int32_t l;
-	int32_t d;    // remaining depth budget (verify list: 1 = needs the exact heap sort)
-	int32_t tag;  // 0: exact mode; t > 0: free mode under verify entry t - 1
-};
 
 // Counter words (int64) shared by the init kernels and the rounds.
 enum SegCounter
@@ -101,7 +100,8 @@ struct SegSortBufs
 	size_t scanTempBytes;
 	int64_t bigCap;
 	int64_t leafCap;
-	int64_t heapCap;        // also the capacity of verify
+	int64_t heapCap;
+	int64_t verifyCap;      // verify entries: big ranges and leaf sub-ranges, all disjoint
 	int64_t chunkCap;
 	int64_t* hostCounters;  // pinned, kSegCounters words
 	hipStream_t aux[3];     // default-priority side streams for the heap-sort size classes
@@ -115,6 +115,7 @@ struct SegSortCaps
 	int64_t bigCap;
 	int64_t leafCap;
 	int64_t heapCap;
+	int64_t verifyCap;
 	int64_t chunkCap;
 	size_t scanTempBytes;
 };
