@@ -150,7 +150,7 @@ def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; the path shards by query batches "
-                          "across GPUs (DESIGN.md section 9) but no multi-GPU run is part of it",
+                          "across GPUs (replicated index, query batches dealt per GPU) but no multi-GPU run is part of it",
                           "n_gpus": args.gpus}))
         return 2
     import cuflye_amd.synth as S
