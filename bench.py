@@ -184,7 +184,7 @@ def main() -> int:
         # the whole workload: batch sizes are only comparable at the real
         # number of batches (a sample of a few batches hides the overlap)
         tuning = autotune.tune(g, ids, cfg, batches=(args.batch,) if args.batch else (1024, 2048, 3072, 4096),
-                               lanes=(args.dp_lanes,) if args.dp_lanes else (16, 32, 64),
+                               lanes=(args.dp_lanes,) if args.dp_lanes else (8, 4, 16, 32),
                                flights=(args.flights,) if args.flights else (2, 3), log=log)
         log(f"autotune best: {tuning['best']}")
 
