@@ -206,10 +206,11 @@ __global__ void passFlagKernel(const int32_t* order, const int32_t* groupPass, i
 }
 
 __global__ void passScatterKernel(const int32_t* order, const int32_t* flag, const int64_t* pos, int32_t nGroups,
-								  const GroupDesc* groups, int32_t bigMin, int32_t* passOrder, int64_t* nBig)
+								  const GroupDesc* groups, int32_t bigMin, int32_t* passOrder, int64_t* nBig,
+								  int64_t* tiers)
 {
 	const int qi = blockIdx.x * blockDim.x + threadIdx.x;
-	bool big = false;
+	bool big = false, mid = false, large = false;
 	long long hits = 0;
 	if (qi < nGroups && flag[qi])
 	{
@@ -217,13 +218,19 @@ __global__ void passScatterKernel(const int32_t* order, const int32_t* flag, con
 		passOrder[pos[qi]] = g;
 		hits = groups[g].n;
 		big = hits >= bigMin;
+		mid = hits >= kWalkMid;
+		large = hits >= kWalkBig;
 	}
 	const int c = __popcll(__ballot(big));
+	const int cm = __popcll(__ballot(mid));
+	const int cl = __popcll(__ballot(large));
 	hits = waveSum64(hits);
 	if (laneId() == 0)
 	{
 		if (c) atomicAdd(reinterpret_cast<unsigned long long*>(nBig), static_cast<unsigned long long>(c));
 		if (hits) atomicAdd(reinterpret_cast<unsigned long long*>(nBig + 1), static_cast<unsigned long long>(hits));
+		if (tiers && cl) atomicAdd(reinterpret_cast<unsigned long long*>(tiers), static_cast<unsigned long long>(cl));
+		if (tiers && cm) atomicAdd(reinterpret_cast<unsigned long long*>(tiers + 1), static_cast<unsigned long long>(cm));
 	}
 }
 
@@ -798,11 +805,11 @@ hipError_t launchPassFlags(const ChainArgs& a, int32_t* flag, hipStream_t stream
 }
 
 hipError_t launchPassScatter(const ChainArgs& a, const int32_t* flag, const int64_t* pos, int32_t* passOrder,
-							 int64_t* nBig, hipStream_t stream)
+							 int64_t* nBig, hipStream_t stream, int64_t* tiers)
 {
 	if (a.nGroups == 0) return hipSuccess;
 	hipLaunchKernelGGL(passScatterKernel, dim3((a.nGroups + 255) / 256), dim3(256), 0, stream, a.order, flag, pos,
-					   a.nGroups, a.groups, kWalkSmall, passOrder, nBig);
+					   a.nGroups, a.groups, kWalkSmall, passOrder, nBig, tiers);
 	return hipGetLastError();
 }
 
@@ -851,12 +858,19 @@ hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t 
 	return hipGetLastError();
 }
 
-hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, bool big, int blocks,
+hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, int tier, int blocks,
 						   hipStream_t stream)
 {
 	if (last <= first) return hipSuccess;
-	if (big) hipLaunchKernelGGL(chainWalkKernel<kWalkBig>, dim3(blocks), dim3(FH_WAVE), 0, stream, a, first, last);
-	else hipLaunchKernelGGL(chainWalkKernel<kWalkSmall>, dim3(blocks), dim3(FH_WAVE), 0, stream, a, first, last);
+	// no more waves than groups in the class (the large classes hold few)
+	const unsigned grid = static_cast<unsigned>(min(blocks, last - first));
+	switch (tier)
+	{
+	case 3: hipLaunchKernelGGL(chainWalkKernel<kWalkHuge>, dim3(grid), dim3(FH_WAVE), 0, stream, a, first, last); break;
+	case 2: hipLaunchKernelGGL(chainWalkKernel<kWalkBig>, dim3(grid), dim3(FH_WAVE), 0, stream, a, first, last); break;
+	case 1: hipLaunchKernelGGL(chainWalkKernel<kWalkMid>, dim3(grid), dim3(FH_WAVE), 0, stream, a, first, last); break;
+	default: hipLaunchKernelGGL(chainWalkKernel<kWalkSmall>, dim3(grid), dim3(FH_WAVE), 0, stream, a, first, last);
+	}
 	return hipGetLastError();
 }
 

@@ -77,10 +77,15 @@ struct ChainArgs
 	const uint32_t* logY;
 };
 
-// Walk kernel variants: backtrack pointers of groups up to kWalkSmall /
-// kWalkBig hits are staged in LDS (larger groups walk them in place).
+// Walk kernel variants by group size (powers of two, so each is a contiguous
+// part of the size-descending processing order): backtrack pointers of up to
+// kWalkSmall / kWalkMid / kWalkBig / kWalkHuge hits are staged in that much
+// LDS (8 / 32 / 64 / 128 KiB: the waves per CU fall as the stage grows, so each
+// class gets the smallest that holds it); larger groups walk them in place.
 constexpr int kWalkSmall = 2048;
+constexpr int kWalkMid = 8192;
 constexpr int kWalkBig = 16384;
+constexpr int kWalkHuge = 32768;
 
 // Flye's divergence logf (glibc-exact) for an array of arguments.
 hipError_t launchLogf(const float* x, int64_t n, float* out, const uint32_t* tabX, const uint32_t* tabY, int tabN,
@@ -93,8 +98,10 @@ hipError_t launchChainPrefilter(const ChainArgs& a, const SegSortBufs& sb, int b
 // nBig[0] counts passing groups of at least kWalkSmall hits, nBig[1] their hits
 // in total (both zeroed by the caller)
 hipError_t launchPassFlags(const ChainArgs& a, int32_t* flag, hipStream_t stream);
+// tiers (may be null): passing groups of at least kWalkBig hits (tiers[0])
+// and of at least kWalkMid hits (tiers[1])
 hipError_t launchPassScatter(const ChainArgs& a, const int32_t* flag, const int64_t* pos, int32_t* passOrder,
-							 int64_t* nBig, hipStream_t stream);
+							 int64_t* nBig, hipStream_t stream, int64_t* tiers = nullptr);
 // Groups holding an extPos-sort range whose exact heap sort is still running
 // (verify entries with d == 1) get deferred[g] = 1 (deferred zeroed by the
 // caller); the split flags then divide the passing groups of the processing
@@ -108,7 +115,8 @@ hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t 
 // score-order keys; pushes the score sorts to sb
 hipError_t launchChainScoreKeys(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream);
 // backtrack .. selection for passOrder[first, last)
-hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, bool big, int blocks,
+// tier: 0 below kWalkSmall hits, 1 below kWalkMid, 2 below kWalkBig, 3 above
+hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, int tier, int blocks,
 						   hipStream_t stream);
 hipError_t launchWriteRows(const ChainArgs& a, const int64_t* rowStart, flyehip_overlap_t* rows,
 						   hipStream_t stream);

@@ -602,7 +602,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	int64_t* nBig = ensureT<int64_t>(c, L->bCounters, 16);
 	zero32(L, bins, 96);
 	zero32(L, queues, 16);
-	HIPCHECK(hipMemsetAsync(nBig, 0, 6 * sizeof(int64_t), L->stream));
+	HIPCHECK(hipMemsetAsync(nBig, 0, 8 * sizeof(int64_t), L->stream));
 	SegSortBufs sb = segBufs(L, nHits, ng);
 	sb.key = ca.sortKey;
 	sb.val = ca.sortVal;
@@ -640,7 +640,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	runSegSort(L, sb, nullptr, &exactPending);
 	HIPCHECK(launchPassFlags(ca, flag, L->stream));
 	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
-	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream));
+	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream, nBig + 6));
 	int32_t* readyOrder = passOrder;
 	int32_t* deferOrder = nullptr;
 	int64_t* posReady = nullptr;
@@ -663,8 +663,10 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	}
 	int64_t counts[3] = {0, 0, 0};
 	int64_t split[5] = {0, 0, 0, 0, 0};  // ready (big groups, hits), deferred (big groups, hits), ready groups
+	int64_t walkTiers[2] = {0, 0};       // passing groups of at least kWalkBig / kWalkMid hits
 	download(L, &counts[0], pos + ng, sizeof(int64_t));
 	download(L, &counts[1], nBig, 2 * sizeof(int64_t));
+	download(L, walkTiers, nBig + 6, 2 * sizeof(int64_t));
 	if (exactPending)
 	{
 		download(L, split, nBig + 2, 4 * sizeof(int64_t));
@@ -711,9 +713,15 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	ca.queue = queues + 4;
 	// walk: score order and backtrack pointers read, 8 bytes per hit
 	const int tw = L->timer.begin(FLYEHIP_K_WALK, L->stream, 8.0 * passHits);
-	HIPCHECK(launchChainWalk(ca, 0, nBigPass, true, blocks, L->stream));
+	// size classes of the processing order (size-descending by power of two)
+	const int32_t nLarge = static_cast<int32_t>(walkTiers[0]), nMid = static_cast<int32_t>(walkTiers[1]);
+	HIPCHECK(launchChainWalk(ca, 0, nLarge, 3, blocks, L->stream));
+	ca.queue = queues + 8;
+	HIPCHECK(launchChainWalk(ca, nLarge, nMid, 2, blocks, L->stream));
+	ca.queue = queues + 9;
+	HIPCHECK(launchChainWalk(ca, nMid, nBigPass, 1, blocks, L->stream));
 	ca.queue = queues + 5;
-	HIPCHECK(launchChainWalk(ca, nBigPass, ca.nPass, false, blocks, L->stream));
+	HIPCHECK(launchChainWalk(ca, nBigPass, ca.nPass, 0, blocks, L->stream));
 	L->timer.end(tw, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[5], L->stream));
 	HIPCHECK(hipEventSynchronize(L->ev[5]));
