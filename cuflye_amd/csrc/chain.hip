@@ -537,8 +537,10 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 			for (int i = lane; i < n; i += FH_WAVE) btS[i] = a.bt[hb + i];
 		waveSync();
 
+		// onlyMaxExt keeps one proposal per group (outProps indexed by group;
+		// props unused); otherwise every chain of the group may be listed
 		Proposal* props = a.props + hb;
-		Proposal* outp = a.outProps + hb;
+		Proposal* outp = a.outProps + (p.only_max_ext ? static_cast<int64_t>(g) : hb);
 		const int32_t* filt = a.filtered + qd.filtStart;
 		const int filtN = qd.filtCount;
 		int np = 0;
@@ -695,7 +697,7 @@ __global__ void writeRowsKernel(ChainArgs a, const int64_t* rowStart, flyehip_ov
 	if (c == 0) return;
 	const GroupDesc gd = a.groups[g];
 	const QueryDesc qd = a.queries[gd.query];
-	const Proposal* props = a.outProps + gd.hitStart;
+	const Proposal* props = a.outProps + (a.params.only_max_ext ? static_cast<int64_t>(g) : gd.hitStart);
 	const int64_t o = rowStart[g];
 	for (int r = 0; r < c; ++r)
 	{

@@ -66,8 +66,8 @@ struct ChainArgs
 	int32_t* hext;
 	int32_t* score;
 	int32_t* bt;
-	Proposal* props;
-	Proposal* outProps;
+	Proposal* props;           // per hit (unused with only_max_ext)
+	Proposal* outProps;        // per hit; per group with only_max_ext
 	// per group
 	int32_t* groupPass;
 	int32_t* groupCount;

@@ -786,14 +786,16 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 	const size_t nm = static_cast<size_t>(nMatches) + 1;
 	ca.sortKey = ensureT<uint32_t>(c, L->bCKey, nm);
 	ca.sortVal = ensureT<uint32_t>(c, L->bCVal, nm);
-	ca.posL = ensureT<int32_t>(c, L->bCPosL, nm);
-	ca.posR = ensureT<int32_t>(c, L->bCPosR, nm);
+	// the match sort's partition scratch, free once the groups are built
+	ca.posL = ensureT<int32_t>(c, L->bPosL, nm);
+	ca.posR = ensureT<int32_t>(c, L->bPosR, nm);
 	ca.hcur = ensureT<int32_t>(c, L->bCHCur, nm);
 	ca.hext = ensureT<int32_t>(c, L->bCHExt, nm);
 	ca.score = ensureT<int32_t>(c, L->bCScore, nm);
 	ca.bt = ensureT<int32_t>(c, L->bCBt, nm);
-	ca.props = ensureT<Proposal>(c, L->bCProps, nm);
-	ca.outProps = ensureT<Proposal>(c, L->bCOut, nm);
+	// with onlyMaxExt a group keeps one proposal: per-group output, no list
+	ca.props = ensureT<Proposal>(c, L->bCProps, p.only_max_ext ? 1 : nm);
+	ca.outProps = ensureT<Proposal>(c, L->bCOut, p.only_max_ext ? static_cast<size_t>(nGroups) + 1 : nm);
 	ca.groupCount = ensureT<int32_t>(c, L->bGroupCount, static_cast<size_t>(nGroups) + 1);
 	ca.groupPass = ensureT<int32_t>(c, L->bGroupPass, static_cast<size_t>(nGroups) + 1);
 	ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
@@ -1465,8 +1467,8 @@ int flyehip_chain_lists(flyehip_ctx* c, const int32_t* cur_pos, const int32_t* e
 		ca.hext = ensureT<int32_t>(c, L->bCHExt, nm);
 		ca.score = ensureT<int32_t>(c, L->bCScore, nm);
 		ca.bt = ensureT<int32_t>(c, L->bCBt, nm);
-		ca.props = ensureT<Proposal>(c, L->bCProps, nm);
-		ca.outProps = ensureT<Proposal>(c, L->bCOut, nm);
+		ca.props = ensureT<Proposal>(c, L->bCProps, p->only_max_ext ? 1 : nm);
+		ca.outProps = ensureT<Proposal>(c, L->bCOut, p->only_max_ext ? gs.size() + 1 : nm);
 		ca.groupCount = ensureT<int32_t>(c, L->bGroupCount, gs.size() + 1);
 		ca.groupPass = ensureT<int32_t>(c, L->bGroupPass, gs.size() + 1);
 		ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
