@@ -310,6 +310,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 {
 	constexpr int kSubs = FH_WAVE / G;
 	constexpr int kRing = kRingTotal / kSubs;
+	static_assert(kRing >= G, "rows not yet stored (the last G) are read from the ring");
 	__shared__ int ring[kSubs][kRing];   // score[j] of the last kRing rows
 	__shared__ int ringM[kSubs][kRing];  // max(score[0..j]) of the same rows
 	const int lane = laneId();
@@ -340,6 +341,12 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 	int32_t* bt = nullptr;
 	int32_t* pmG = nullptr;
 	int pm = 0;
+	// rows are stored G at a time: lane (r mod G) of the subgroup holds row r's
+	// score, backtrack and prefix maximum until its G-row block is complete,
+	// then the block is written with one coalesced store per array (single
+	// 4-byte stores per row leave L2 lines partly dirty and multiply the
+	// written bytes); lookbacks within the LDS ring never read them back
+	int hs = 0, hbt = -1, hpm = 0;
 	bool done = false;
 	int iter = 0;
 	for (;;)
@@ -371,12 +378,18 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			pm = 0;
 			if (sl == 0)
 			{
-				sc[0] = 0;
-				bt[0] = -1;
-				pmG[0] = 0;
 				rg[0] = 0;
 				rm[0] = 0;
+				if (n == 1)
+				{
+					sc[0] = 0;
+					bt[0] = -1;
+					pmG[0] = 0;
+				}
 			}
+			hs = 0;  // row 0, held by lane 0
+			hbt = -1;
+			hpm = 0;
 			need = false;
 		}
 		if (__ballot(!done) == 0) break;
@@ -443,13 +456,26 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			}
 			const int32_t si = max(carry, k);
 			pm = max(pm, si);
+			if (sl == (i & (G - 1)))
+			{
+				hs = si;
+				hbt = carry > k ? carryId : -1;
+				hpm = pm;
+			}
 			if (sl == 0)
 			{
-				sc[i] = si;
-				bt[i] = carry > k ? carryId : -1;
-				pmG[i] = pm;
 				rg[i & (kRing - 1)] = si;
 				rm[i & (kRing - 1)] = pm;
+			}
+			if ((i & (G - 1)) == G - 1 || i + 1 >= n)
+			{
+				const int r = (i & ~(G - 1)) + sl;
+				if (r <= i)
+				{
+					sc[r] = hs;
+					bt[r] = hbt;
+					pmG[r] = hpm;
+				}
 			}
 			++i;
 		}
