@@ -262,13 +262,15 @@ __global__ void __launch_bounds__(FH_WAVE) segLeafKernel(SegSortBufs b, int64_t 
 }
 
 // exsort::siftStepPacked without branches: both children are loaded
-// together (indices clamped to the range; the clamped load is not used), keys
-// compared on their 32-bit halves.  Same result: the right child unless it is
-// below the left one, moved up unless it is below the value.
+// together from adjacent words (the first index clamped to the range; hx has
+// one spare word after n; a clamped or spare load is not used), keys compared
+// on their 32-bit halves.  Same result: the right child unless it is below
+// the left one, moved up unless it is below the value.
 __device__ inline bool heapStep(uint64_t* hx, int& h, int m, int n, uint64_t v)
 {
 	const int c1 = 2 * h + 1;
-	const uint64_t x1 = hx[min(c1, n - 1)], x2 = hx[min(c1 + 1, n - 1)];
+	const int cc = min(c1, n - 1);
+	const uint64_t x1 = hx[cc], x2 = hx[cc + 1];
 	const bool right = c1 + 1 < m && !(static_cast<uint32_t>(x2 >> 32) < static_cast<uint32_t>(x1 >> 32));
 	const uint64_t cv = right ? x2 : x1;
 	const bool up = c1 < m && !(static_cast<uint32_t>(cv >> 32) < static_cast<uint32_t>(v >> 32));
@@ -476,7 +478,7 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 	*firstSlot = -1;
 	static const bool bigLds =
 		hipFuncSetAttribute(reinterpret_cast<const void*>(segHeapLdsKernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-							16384 * 8) == hipSuccess;
+							(16384 + 1) * 8) == hipSuccess;
 	const int32_t caps[3] = {2048, 8192, bigLds ? 16384 : 8192};
 	const unsigned grid = static_cast<unsigned>(n < 8192 ? n : 8192);
 	hipError_t e = hipEventRecord(b.auxEv[3], s);
@@ -489,8 +491,9 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 		if ((e = hipStreamWaitEvent(b.aux[t], b.auxEv[3], 0)) != hipSuccess) return e;
 		const int slot = b.timer ? b.timer->begin(FLYEHIP_K_HEAP, b.aux[t]) : -1;
 		if (t == 0) *firstSlot = slot;
-		hipLaunchKernelGGL(segHeapLdsKernel, dim3(grid), dim3(FH_WAVE), static_cast<size_t>(caps[t]) * 8, b.aux[t], list,
-						   n, onlyFlagged, srcK, srcV, dstK, dstV, lo, caps[t]);
+		// one spare word after the largest range (heapStep's paired load)
+		hipLaunchKernelGGL(segHeapLdsKernel, dim3(grid), dim3(FH_WAVE), static_cast<size_t>(caps[t] + 1) * 8, b.aux[t],
+						   list, n, onlyFlagged, srcK, srcV, dstK, dstV, lo, caps[t]);
 		if (b.timer) b.timer->end(slot, b.aux[t]);
 		if (join)
 		{
