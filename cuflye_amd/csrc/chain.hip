@@ -30,7 +30,10 @@ namespace
 
 constexpr int kLaneMax = 48;
 constexpr uint32_t kWalkRelevant = 0x80000000u;  // score-order value bit: row has a backtrack pointer
-constexpr int kRingTotal = 2048;  // LDS ring entries per wave (two rings), split over groups
+// LDS ring entries per wave (two rings), split over groups: 4 KiB of LDS per
+// wave keeps the DP's occupancy at its VGPR limit (a 16 KiB ring held it to
+// about 2.5 waves per SIMD, and the lookback scan is latency-bound)
+constexpr int kRingTotal = 512;
 
 __device__ inline float flyeLogf(float x, const uint32_t* tabX, const uint32_t* tabY, int tabN)
 {
@@ -530,16 +533,21 @@ constexpr int kPropTile = 256;  // proposals sorted in LDS up to this count
 // the pointers it passes, so lane 0 re-checks each start) and records
 // (start, first, length); then every lane evaluates one recorded walk
 // (positions, overlapTest, the filtered-position count, logf) and the
-// surviving proposals are appended in walk order.
-template <int kBt>
+// surviving proposals are appended in walk order.  kOnlyMax (the onlyMaxExt
+// parameter) keeps the best proposal only; the variant without it also holds
+// the proposal sort, whose LDS and registers would otherwise cut the
+// occupancy of the onlyMaxExt walk.
+template <int kBt, bool kOnlyMax>
 __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t first, int32_t last)
 {
-	__shared__ int btS[kBt];
+	// backtrack pointers staged as 16-bit row indices (rows of a staged group
+	// are below kBt <= 32768; -1 stays -1)
+	static_assert(kBt <= 32768, "staged rows must fit int16_t");
+	__shared__ int16_t btS[kBt];
 	__shared__ int recStart[FH_WAVE];
 	__shared__ int recFirst[FH_WAVE];
 	__shared__ int recLen[FH_WAVE];
 	__shared__ int red[2];
-	__shared__ WaveSortLds<uint32_t, kPropTile> lds;
 	const int lane = laneId();
 	const flyehip_overlap_params_t& p = a.params;
 	const int32_t k = p.kmer_size;
@@ -558,15 +566,20 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		uint32_t* sk = a.sortKey + hb;
 		uint32_t* sv = a.sortVal + hb;
 		const bool inLds = n <= kBt;
-		int32_t* bt = inLds ? btS : a.bt + hb;
+		int32_t* btG = a.bt + hb;
+		auto btGet = [&](int r) -> int { return inLds ? static_cast<int>(btS[r]) : btG[r]; };
+		auto btClear = [&](int r) {
+			if (inLds) btS[r] = -1;
+			else btG[r] = -1;
+		};
 		if (inLds)
-			for (int i = lane; i < n; i += FH_WAVE) btS[i] = a.bt[hb + i];
+			for (int i = lane; i < n; i += FH_WAVE) btS[i] = static_cast<int16_t>(btG[i]);
 		waveSync();
 
 		// onlyMaxExt keeps one proposal per group (outProps indexed by group;
 		// props unused); otherwise every chain of the group may be listed
 		Proposal* props = a.props + hb;
-		Proposal* outp = a.outProps + (p.only_max_ext ? static_cast<int64_t>(g) : hb);
+		Proposal* outp = a.outProps + (kOnlyMax ? static_cast<int64_t>(g) : hb);
 		const int32_t* filt = a.filtered + qd.filtStart;
 		const int filtN = qd.filtCount;
 		int np = 0;
@@ -576,7 +589,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		{
 			const int q = q0 + lane;
 			const int myStart = q < n ? static_cast<int>(sv[q] & ~kWalkRelevant) : 0;
-			const bool live = q < n && bt[myStart] != -1;
+			const bool live = q < n && btGet(myStart) != -1;
 			const unsigned long long m = __ballot(live);
 			recStart[lane] = myStart;
 			waveSync();
@@ -589,14 +602,14 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 					const int l = __ffsll(static_cast<long long>(mm)) - 1;
 					mm &= mm - 1;
 					const int start = recStart[l];
-					if (bt[start] == -1) continue;
+					if (btGet(start) == -1) continue;
 					int f = 0, len = 0, pos = start;
 					while (pos != -1)
 					{
 						f = pos;
 						++len;
-						const int nxt = bt[pos];
-						bt[pos] = -1;
+						const int nxt = btGet(pos);
+						btClear(pos);
 						pos = nxt;
 					}
 					recFirst[nrec] = f;
@@ -643,7 +656,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 				}
 			}
 			const unsigned long long okm = __ballot(ok);
-			if (p.only_max_ext)
+			if constexpr (kOnlyMax)
 			{
 				// first strict maximum in walk order
 				const int mx = waveMax(ok ? o.score : (-0x7fffffff - 1));
@@ -663,7 +676,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 			waveSync();
 		}
 		int count = 0;
-		if (p.only_max_ext)
+		if constexpr (kOnlyMax)
 		{
 			waveSync();
 			count = (haveBest && outp[0].div < p.max_divergence) ? 1 : 0;
@@ -671,6 +684,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		else
 		{
 			// std::sort(extOverlaps, score desc), containment filter, divergence filter
+			__shared__ WaveSortLds<uint32_t, kPropTile> lds;
 			for (int i = lane; i < np; i += FH_WAVE)
 			{
 				sk[i] = static_cast<uint32_t>(0x7fffffff - props[i].score);
@@ -886,19 +900,37 @@ hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t 
 	return hipGetLastError();
 }
 
+template <bool kOnlyMax>
+void launchWalkTier(const ChainArgs& a, int32_t first, int32_t last, int tier, unsigned grid, hipStream_t stream)
+{
+	switch (tier)
+	{
+	case 3:
+		hipLaunchKernelGGL((chainWalkKernel<kWalkHuge, kOnlyMax>), dim3(grid), dim3(FH_WAVE), 0, stream, a, first,
+						   last);
+		break;
+	case 2:
+		hipLaunchKernelGGL((chainWalkKernel<kWalkBig, kOnlyMax>), dim3(grid), dim3(FH_WAVE), 0, stream, a, first,
+						   last);
+		break;
+	case 1:
+		hipLaunchKernelGGL((chainWalkKernel<kWalkMid, kOnlyMax>), dim3(grid), dim3(FH_WAVE), 0, stream, a, first,
+						   last);
+		break;
+	default:
+		hipLaunchKernelGGL((chainWalkKernel<kWalkSmall, kOnlyMax>), dim3(grid), dim3(FH_WAVE), 0, stream, a, first,
+						   last);
+	}
+}
+
 hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, int tier, int blocks,
 						   hipStream_t stream)
 {
 	if (last <= first) return hipSuccess;
 	// no more waves than groups in the class (the large classes hold few)
 	const unsigned grid = static_cast<unsigned>(min(blocks, last - first));
-	switch (tier)
-	{
-	case 3: hipLaunchKernelGGL(chainWalkKernel<kWalkHuge>, dim3(grid), dim3(FH_WAVE), 0, stream, a, first, last); break;
-	case 2: hipLaunchKernelGGL(chainWalkKernel<kWalkBig>, dim3(grid), dim3(FH_WAVE), 0, stream, a, first, last); break;
-	case 1: hipLaunchKernelGGL(chainWalkKernel<kWalkMid>, dim3(grid), dim3(FH_WAVE), 0, stream, a, first, last); break;
-	default: hipLaunchKernelGGL(chainWalkKernel<kWalkSmall>, dim3(grid), dim3(FH_WAVE), 0, stream, a, first, last);
-	}
+	if (a.params.only_max_ext) launchWalkTier<true>(a, first, last, tier, grid, stream);
+	else launchWalkTier<false>(a, first, last, tier, grid, stream);
 	return hipGetLastError();
 }
 

@@ -79,9 +79,10 @@ struct ChainArgs
 
 // Walk kernel variants by group size (powers of two, so each is a contiguous
 // part of the size-descending processing order): backtrack pointers of up to
-// kWalkSmall / kWalkMid / kWalkBig / kWalkHuge hits are staged in that much
-// LDS (8 / 32 / 64 / 128 KiB: the waves per CU fall as the stage grows, so each
-// class gets the smallest that holds it); larger groups walk them in place.
+// kWalkSmall / kWalkMid / kWalkBig / kWalkHuge hits are staged in LDS as
+// 16-bit row indices (4 / 16 / 32 / 64 KiB: the waves per CU fall as the
+// stage grows, so each class gets the smallest that holds it); larger groups
+// walk them in place.
 constexpr int kWalkSmall = 2048;
 constexpr int kWalkMid = 8192;
 constexpr int kWalkBig = 16384;
