@@ -26,6 +26,40 @@ __host__ __device__ inline int64_t toSigned(uint32_t iid)
 // ---- wave64 helpers ------------------------------------------------------
 __device__ inline int laneId() { return static_cast<int>(__lane_id()); }
 
+// ---- segment lookup ------------------------------------------------------
+// Largest r in [lo, n) with off[r] <= w (off ascending, empty segments
+// allowed; off[lo] <= w).
+__device__ inline int findSegFrom(const int64_t* off, int lo, int n, int64_t w)
+{
+	int hi = n - 1;
+	while (lo < hi)
+	{
+		const int mid = (lo + hi + 1) >> 1;
+		if (off[mid] <= w) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+__device__ inline int findSeg(const int64_t* off, int n, int64_t w) { return findSegFrom(off, 0, n, w); }
+
+// findSeg for kernels whose lanes hold ascending elements (one element per
+// thread in launch order): the binary search runs once per wave, for the
+// first active lane's element, on wave-uniform (scalar) loads, and every lane
+// steps forward over the few segment starts between that element and its
+// own; a lane still short after four steps (a run of empty segments) finishes
+// with a binary search over the rest.
+__device__ inline int findSegWave(const int64_t* off, int n, int64_t w)
+{
+	const uint64_t u = static_cast<uint64_t>(w);
+	const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u));
+	const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u >> 32));
+	const int64_t w0 = static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+	int r = findSeg(off, n, w0);
+	for (int s = 0; s < 4 && r + 1 < n && off[r + 1] <= w; ++s) ++r;
+	if (r + 1 < n && off[r + 1] <= w) r = findSegFrom(off, r + 1, n, w);
+	return r;
+}
+
 __device__ inline int waveSum(int v)
 {
 	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, FH_WAVE);

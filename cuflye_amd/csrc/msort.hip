@@ -27,23 +27,11 @@ namespace fh
 namespace
 {
 
-__device__ inline int findSeg(const int64_t* off, int n, int64_t w)
-{
-	int lo = 0, hi = n - 1;
-	while (lo < hi)
-	{
-		const int mid = (lo + hi + 1) >> 1;
-		if (off[mid] <= w) lo = mid;
-		else hi = mid - 1;
-	}
-	return lo;
-}
-
 __global__ void iotaKernel(const int64_t* qOff, int32_t nq, uint32_t* v, int64_t n)
 {
 	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	const int q = findSeg(qOff, nq, i);
+	const int q = findSegWave(qOff, nq, i);
 	v[i] = static_cast<uint32_t>(i - qOff[q]);
 }
 
@@ -52,7 +40,7 @@ __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* 
 {
 	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	const int q = findSeg(qOff, nq, i);
+	const int q = findSegWave(qOff, nq, i);
 	const int64_t b = qOff[q], e = qOff[q + 1];
 	const uint64_t k = skey[i];
 	int64_t first = i;

@@ -100,24 +100,11 @@ __global__ void __launch_bounds__(kScanNT) scanDownKernel(const int32_t* in, int
 	if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = blockSums[nb];
 }
 
-__device__ inline int findSeg(const int64_t* off, int n, int64_t w)
-{
-	// largest r in [0, n) with off[r] <= w
-	int lo = 0, hi = n - 1;
-	while (lo < hi)
-	{
-		const int mid = (lo + hi + 1) >> 1;
-		if (off[mid] <= w) lo = mid;
-		else hi = mid - 1;
-	}
-	return lo;
-}
-
 __global__ void countKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin, int k, HashTable t)
 {
 	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (w >= nWin) return;
-	const int r = findSeg(winOff, reads.n, w);
+	const int r = findSegWave(winOff, reads.n, w);
 	const int32_t pos = static_cast<int32_t>(w - winOff[r]);
 	bool flip;
 	const uint64_t c = canonical(kmerAt(reads.words, reads.wordOff[r], pos, k), k, flip);
@@ -219,7 +206,7 @@ __global__ void fillKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin
 {
 	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (w >= nWin) return;
-	const int r = findSeg(winOff, reads.n, w);
+	const int r = findSegWave(winOff, reads.n, w);
 	const int32_t pos = static_cast<int32_t>(w - winOff[r]);
 	bool flip;
 	const uint64_t c = canonical(kmerAt(reads.words, reads.wordOff[r], pos, k), k, flip);
@@ -279,7 +266,7 @@ struct WindowKmer
 __device__ inline WindowKmer windowAt(const JoinArgs& a, int64_t w)
 {
 	WindowKmer r;
-	r.q = findSeg(a.winOff, a.nQueries, w);
+	r.q = findSegWave(a.winOff, a.nQueries, w);
 	r.pos = static_cast<int32_t>(w - a.winOff[r.q]);
 	const JoinQuery Q = a.queries[r.q];
 	const int64_t wo = a.seqs.wordOff[Q.seq];
@@ -416,7 +403,7 @@ __global__ void __launch_bounds__(256) joinFillKernel(JoinArgs a)
 	int gflag = 0;
 	if (w < a.nWin)
 	{
-		const int q = findSeg(a.winOff, a.nQueries, w);
+		const int q = findSegWave(a.winOff, a.nQueries, w);
 		pos = static_cast<int32_t>(w - a.winOff[q]);
 		if (a.winFilt[w]) a.filtered[a.filtOff[w]] = pos;
 		if (a.winCount[w] > 0)
@@ -478,7 +465,7 @@ __global__ void writeCandidatesKernel(JoinArgs a, const int64_t* qMatchOff, cons
 {
 	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= nMatches) return;
-	const int q = findSeg(qMatchOff, a.nQueries, i);
+	const int q = findSegWave(qMatchOff, a.nQueries, i);
 	const int64_t qb = qMatchOff[q];
 	const uint64_t key = a.mkey[i];
 	const int32_t ext = perm ? a.mext[qb + perm[i]] : a.mext[i];
@@ -503,7 +490,7 @@ __global__ void gatherHitsKernel(const int64_t* qMatchOff, int32_t nQueries, con
 {
 	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= nMatches) return;
-	const int q = findSeg(qMatchOff, nQueries, i);
+	const int q = findSegWave(qMatchOff, nQueries, i);
 	const int64_t qb = qMatchOff[q];
 	const uint64_t k = key[i];
 	hcur[i] = static_cast<int32_t>(static_cast<uint32_t>(k));
@@ -518,7 +505,7 @@ __global__ void buildGroupsKernel(const int64_t* qMatchOff, int32_t nQueries, co
 	if (i >= nMatches) return;
 	if (!groupFlag[i]) return;
 	const int64_t g = groupOff[i];
-	const int q = findSeg(qMatchOff, nQueries, i);
+	const int q = findSegWave(qMatchOff, nQueries, i);
 	const uint32_t eid = static_cast<uint32_t>(key[i] >> 32);
 	GroupDesc d;
 	d.hitStart = i;
