@@ -101,6 +101,13 @@ __device__ inline int upperBound(const int32_t* a, int n, int32_t v)
 	return lo;
 }
 
+// A group's hits in processing order: the extPos-sorted copy, or the
+// (extId, curPos) order itself for groups that keep it.
+__device__ inline const int32_t* groupHits(const int32_t* sorted, const int32_t* inOrder, int64_t hb, bool extSorted)
+{
+	return (extSorted ? sorted : inOrder) + hb;
+}
+
 __device__ inline int pullGroup(int32_t* queue)
 {
 	int g = 0;
@@ -156,6 +163,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 			// unique curPos counted from prevPos = 0 on the (extId, curPos) order
 			int uniq = 0;
 			int mnE = 0x7fffffff, mxE = -0x7fffffff - 1;
+#pragma unroll 4
 			for (int i = lane; i < n; i += FH_WAVE)
 			{
 				const int32_t c = cur0[i];
@@ -182,6 +190,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 			{
 				uint32_t* sk = a.sortKey + hb;
 				uint32_t* sv = a.sortVal + hb;
+#pragma unroll 4
 				for (int i = lane; i < n; i += FH_WAVE)
 				{
 					sk[i] = static_cast<uint32_t>(ext0[i]);
@@ -285,22 +294,16 @@ __global__ void __launch_bounds__(FH_WAVE) chainGatherKernel(ChainArgs a)
 		const int32_t* ext0 = a.ext + hb;
 		int32_t* hc = a.hcur + hb;
 		int32_t* he = a.hext + hb;
+		// groups in curPos order are read in place (groupHits)
 		if (gd.extLen > qd.curLen)
 		{
 			const uint32_t* sv = a.sortVal + hb;
+#pragma unroll 4
 			for (int i = lane; i < n; i += FH_WAVE)
 			{
 				const uint32_t src = sv[i];
 				hc[i] = cur0[src];
 				he[i] = ext0[src];
-			}
-		}
-		else
-		{
-			for (int i = lane; i < n; i += FH_WAVE)
-			{
-				hc[i] = cur0[i];
-				he[i] = ext0[i];
 			}
 		}
 	}
@@ -373,8 +376,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			n = gd.n;
 			i = 1;
 			extSorted = gd.extLen > a.queries[gd.query].curLen;
-			hc = a.hcur + gd.hitStart;
-			he = a.hext + gd.hitStart;
+			hc = groupHits(a.hcur, a.cur, gd.hitStart, extSorted);
+			he = groupHits(a.hext, a.ext, gd.hitStart, extSorted);
 			sc = a.score + gd.hitStart;
 			bt = a.bt + gd.hitStart;
 			pmG = a.posR + gd.hitStart;  // free during the DP: holds max(score[0..j])
@@ -514,6 +517,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainScoreKeysKernel(ChainArgs a, Seg
 		const int32_t* bt = a.bt + hb;
 		uint32_t* sk = a.sortKey + hb;
 		uint32_t* sv = a.sortVal + hb;
+#pragma unroll 4
 		for (int i = lane; i < n; i += FH_WAVE)
 		{
 			sk[i] = static_cast<uint32_t>(0x7fffffff - sc[i]);
@@ -560,8 +564,9 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		const QueryDesc qd = a.queries[gd.query];
 		const int n = gd.n;
 		const int64_t hb = gd.hitStart;
-		const int32_t* hc = a.hcur + hb;
-		const int32_t* he = a.hext + hb;
+		const bool extSorted = gd.extLen > qd.curLen;
+		const int32_t* hc = groupHits(a.hcur, a.cur, hb, extSorted);
+		const int32_t* he = groupHits(a.hext, a.ext, hb, extSorted);
 		const int32_t* sc = a.score + hb;
 		uint32_t* sk = a.sortKey + hb;
 		uint32_t* sv = a.sortVal + hb;
@@ -573,7 +578,10 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 			else btG[r] = -1;
 		};
 		if (inLds)
+		{
+#pragma unroll 4
 			for (int i = lane; i < n; i += FH_WAVE) btS[i] = static_cast<int16_t>(btG[i]);
+		}
 		waveSync();
 
 		// onlyMaxExt keeps one proposal per group (outProps indexed by group;
