@@ -692,7 +692,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		else
 		{
 			// std::sort(extOverlaps, score desc), containment filter, divergence filter
-			__shared__ WaveSortLds<uint32_t, kPropTile> lds;
+			__shared__ WaveStdSortLds<uint32_t, kPropTile> lds;
 			for (int i = lane; i < np; i += FH_WAVE)
 			{
 				sk[i] = static_cast<uint32_t>(0x7fffffff - props[i].score);

@@ -31,16 +31,25 @@ struct WSeg
 	int d;
 };
 
+// LDS of one wave's tile sort.  Swap positions inside the tile are 16-bit
+// (kTile <= 65536): the smaller footprint raises the waves per CU.
 template <class K, int kTile>
 struct WaveSortLds
 {
+	static_assert(kTile <= 65536, "tile positions are 16-bit");
 	K k[kTile];
 	uint32_t v[kTile];
-	int posL[kTile / 2 + 1];
-	int posR[kTile / 2 + 1];
+	uint16_t posL[kTile / 2 + 1];
+	uint16_t posR[kTile / 2 + 1];
 	WSeg stack[40];
 	WSeg lanes[FH_WAVE];
-	WSeg gstack[40];  // ranges larger than the tile (waveStdSort)
+};
+
+// waveStdSort also keeps the stack of ranges larger than the tile.
+template <class K, int kTile>
+struct WaveStdSortLds : WaveSortLds<K, kTile>
+{
+	WSeg gstack[40];
 };
 
 __device__ inline void waveSync()
@@ -61,8 +70,8 @@ __device__ inline unsigned long long laneMaskLt()
 // do) the median of three distinct samples at a pseudo-random offset a third
 // of the range apart, which does not degenerate on the inputs that exhausted
 // std::sort's depth budget.
-template <class K>
-__device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int* posL, int* posR,
+template <class K, class P>
+__device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, P* posL, P* posR,
 									uint32_t freeSeed = 0)
 {
 	const int lane = laneId();
@@ -111,8 +120,8 @@ __device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int*
 		const int myLe = leLeft + __popcll(bl & lt);
 		const int leRight = LE - myLe - (le ? 1 : 0);
 		const bool swapGe = ge && myGe < leRight;
-		if (swapGe) posL[myGe] = i;
-		if (le && leRight < myGe) posR[leRight] = i;
+		if (swapGe) posL[myGe] = static_cast<P>(i);
+		if (le && leRight < myGe) posR[leRight] = static_cast<P>(i);
 		const unsigned long long bns = __ballot(ge && !swapGe);
 		if (lm == 0x7fffffff && bns) lm = base + __ffsll(static_cast<long long>(bns)) - 1;
 		m += __popcll(__ballot(swapGe));
@@ -125,7 +134,7 @@ __device__ inline int wavePartition(const exsort::Span<K>& s, int f, int l, int*
 	int cut = lm;
 	if (m > 0)
 	{
-		const int r = posR[m - 1];
+		const int r = static_cast<int>(posR[m - 1]);
 		if (r < cut) cut = r;
 	}
 	waveSync();
@@ -272,7 +281,7 @@ __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, Wave
 // Exact std::sort of g[0, n) by one wave.  posL/posR: global scratch of
 // n / 2 + 1 ints each (only touched when n > kTile).
 template <class K, int kTile, int kLaneMax>
-__device__ void waveStdSort(const exsort::Span<K>& g, int n, int* posL, int* posR, WaveSortLds<K, kTile>& t)
+__device__ void waveStdSort(const exsort::Span<K>& g, int n, int* posL, int* posR, WaveStdSortLds<K, kTile>& t)
 {
 	if (n < 2) return;
 	if (n <= kTile)
