@@ -311,6 +311,88 @@ __global__ void __launch_bounds__(FH_WAVE) chainGatherKernel(ChainArgs a)
 
 // ---- dp ----------------------------------------------------------------------
 
+// ---- subgroup steps of the DP ---------------------------------------------
+// The lookback tile of G lanes runs one prefix maximum and one maximum per
+// step.  For G <= 16 a subgroup lies inside one 16-lane DPP row, so both run
+// on DPP row moves; LDS permutes (__shfl) would put a round trip each on the
+// tile's dependent chain.  Larger subgroups span rows and keep the permutes.
+
+// Lane l of a row reads lane l - O of the same row (lanes with l < O keep
+// their own value; callers ignore the lanes whose source lies in another
+// subgroup).
+template <int O>
+__device__ inline int rowShr(int v)
+{
+	static_assert(O >= 1 && O <= 15, "row shift");
+	return __builtin_amdgcn_update_dpp(v, v, 0x110 + O, 0xF, 0xF, false);
+}
+
+// max(v of lanes 0..sl of the subgroup)
+template <int G>
+__device__ inline int subPrefixMax(int v, int sl)
+{
+	if constexpr (G <= 16)
+	{
+		if constexpr (G > 1)
+		{
+			const int t = rowShr<1>(v);
+			if (sl >= 1) v = max(v, t);
+		}
+		if constexpr (G > 2)
+		{
+			const int t = rowShr<2>(v);
+			if (sl >= 2) v = max(v, t);
+		}
+		if constexpr (G > 4)
+		{
+			const int t = rowShr<4>(v);
+			if (sl >= 4) v = max(v, t);
+		}
+		if constexpr (G > 8)
+		{
+			const int t = rowShr<8>(v);
+			if (sl >= 8) v = max(v, t);
+		}
+	}
+	else
+	{
+		for (int o = 1; o < G; o <<= 1)
+		{
+			const int t = __shfl_up(v, o, G);
+			if (sl >= o) v = max(v, t);
+		}
+	}
+	return v;
+}
+
+// v of lane sl - 1 of the subgroup (undefined for sl == 0)
+template <int G>
+__device__ inline int subPrev(int v)
+{
+	if constexpr (G <= 16) return rowShr<1>(v);
+	else return __shfl_up(v, 1, G);
+}
+
+// max of v over the subgroup, in every lane
+template <int G>
+__device__ inline int subMax(int v)
+{
+	if constexpr (G <= 16)
+	{
+		// quad_perm [1,0,3,2] and [2,3,0,1], then the two quads of an 8-lane
+		// half row (row_half_mirror), then the two halves of the row (row_mirror)
+		if constexpr (G > 1) v = max(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+		if constexpr (G > 2) v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+		if constexpr (G > 4) v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));
+		if constexpr (G > 8) v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));
+	}
+	else
+	{
+		for (int o = G / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, G));
+	}
+	return v;
+}
+
 template <int G>
 __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 {
@@ -422,20 +504,27 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 						const int32_t ms = min(min(dc, de), k);
 						const float pen = jd > p.gap_jump_threshold ? p.large_gap_penalty : p.small_gap_penalty;
 						const int32_t gc = static_cast<int32_t>(pen * static_cast<float>(jd));
-						const int32_t sj = (i - j < kRing) ? rg[j & (kRing - 1)] : sc[j];
+						int32_t sj;
+						if (i - j < kRing)
+						{
+							sj = rg[j & (kRing - 1)];
+						}
+						else
+						{
+							// far predecessor, from the stored rows; the empty asm on the
+							// loaded value keeps this load apart (merged with the ring
+							// read it would become a flat load on every step)
+							sj = sc[j];
+							asm volatile("" : "+v"(sj));
+						}
 						cand = sj + ms - gc;
 						special = (jd == 0 && dc < k);
 					}
 					stop = extSorted ? (de > maxJump) : (dc > maxJump);
 				}
 				// running maximum before this lane, in scan order
-				int incl = cand;
-				for (int o = 1; o < G; o <<= 1)
-				{
-					const int t = __shfl_up(incl, o, G);
-					if (sl >= o) incl = max(incl, t);
-				}
-				int before = __shfl_up(incl, 1, G);
+				const int incl = subPrefixMax<G>(cand, sl);
+				int before = subPrev<G>(incl);
 				if (sl == 0) before = -0x7fffffff - 1;
 				before = max(before, carry);
 				const bool improved = active && cand > before;
@@ -443,8 +532,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 				const unsigned long long bm = (__ballot(brk) & subMask) >> (sub * G);
 				const int lastLane = bm ? (__ffsll(static_cast<long long>(bm)) - 1) : (G - 1);
 				const int masked = (active && sl <= lastLane) ? cand : (-0x7fffffff - 1);
-				int mx = masked;
-				for (int o = G / 2; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, G));
+				const int mx = subMax<G>(masked);
 				if (mx > carry)
 				{
 					const unsigned long long hit = (__ballot(masked == mx) & subMask) >> (sub * G);
@@ -456,7 +544,16 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 				const int jn = tileBase - G;
 				if (bounded && jn >= 0)
 				{
-					const int mj = (i - jn < kRing) ? rm[jn & (kRing - 1)] : pmG[jn];
+					int mj;
+					if (i - jn < kRing)
+					{
+						mj = rm[jn & (kRing - 1)];
+					}
+					else
+					{
+						mj = pmG[jn];
+						asm volatile("" : "+v"(mj));
+					}
 					if (mj + k <= carry) break;
 				}
 			}

@@ -124,7 +124,7 @@ struct flyehip_ctx
 	std::vector<std::unique_ptr<fh::Lane>> lanes;  // lanes[0] serves every single-stream call
 
 	// tuning
-	int dpLanes = 64;
+	int dpLanes = 8;
 	int queriesPerBatch = 2048;
 	int batchesInFlight = 2;
 	int chainBlocks = 0;

@@ -209,9 +209,10 @@ int flyehip_abi_version(void);
 int flyehip_get_stats(const flyehip_ctx* ctx, flyehip_stats_t* out);
 
 /* Tuning knobs picked by the autotuner: lanes that chain one target group
- * (1, 4, 8, 16, 32 or 64), the query batch size, and the batches in flight
- * (1 .. 4): flyehip_overlaps* deal batches to that many streams, each driven
- * by its own host thread, so one batch's tail overlaps the next batch (the
+ * (1, 4, 8, 16, 32 or 64; 8 by default), the query batch size (2048 by
+ * default), and the batches in flight (1 .. 4, 2 by default):
+ * flyehip_overlaps* deal batches to that many streams, each driven by its
+ * own host thread, so one batch's tail overlaps the next batch (the
  * per-query path always uses one).  0 keeps the current value.  A new batch
  * size or number of batches in flight releases the per-batch device buffers
  * (each batch in flight holds its own; the next call sizes them again). */

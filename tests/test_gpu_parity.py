@@ -116,8 +116,15 @@ def test_overlaps_config2_small(gpu, oracle_mod):
             frows, froff = gpu.get_seq_overlaps(ids, cfg)
             assert np.array_equal(froff, eroff)
             assert_rows_equal(frows, erows, f"config2 overlaps, {fl} batches in flight")
+        # every DP lookback width (1 .. 64 lanes per group): scheduling only
+        for g in (1, 4, 16, 32, 64):
+            print(f"[c2] dp lanes {g}", flush=True)
+            gpu.set_tuning(dp_lanes_per_group=g, queries_per_batch=2048, batches_in_flight=2)
+            frows, froff = gpu.get_seq_overlaps(ids, cfg)
+            assert np.array_equal(froff, eroff)
+            assert_rows_equal(frows, erows, f"config2 overlaps, {g} DP lanes per group")
     finally:
-        gpu.set_tuning(queries_per_batch=2048, batches_in_flight=2)
+        gpu.set_tuning(dp_lanes_per_group=8, queries_per_batch=2048, batches_in_flight=2)
     # read-to-graph detector shape on the same data: containment filter path
     cfg2 = OverlapConfig.read_to_graph(15, 1000)
     rows, roff = gpu.get_seq_overlaps(ids[:60], cfg2)
