@@ -627,6 +627,39 @@ __global__ void __launch_bounds__(FH_WAVE) chainScoreKeysKernel(ChainArgs a, Seg
 
 constexpr int kPropTile = 256;  // proposals sorted in LDS up to this count
 
+// Lane 0's walks for the live starts m of one round (start rows in
+// recStart[lane]): each walk follows the backtrack pointers from its start,
+// clearing them, and ends on the first row without one (a chain start, or a
+// row an earlier walk passed); (start, first row, length) are recorded in
+// walk order.  One loop per pointer storage (16-bit LDS stage, or the rows
+// in place), so each step is one load and one store.
+template <class T>
+__device__ inline int walkStarts(T* bt, unsigned long long m, int* recStart, int* recFirst, int* recLen)
+{
+	int nrec = 0;
+	while (m)
+	{
+		const int l = __ffsll(static_cast<long long>(m)) - 1;
+		m &= m - 1;
+		const int start = recStart[l];
+		if (bt[start] == -1) continue;
+		int f = 0, len = 0, pos = start;
+		while (pos != -1)
+		{
+			f = pos;
+			++len;
+			const int nxt = bt[pos];
+			bt[pos] = -1;
+			pos = nxt;
+		}
+		recFirst[nrec] = f;
+		recLen[nrec] = len;
+		recStart[nrec] = start;  // slots below l are no longer read
+		++nrec;
+	}
+	return nrec;
+}
+
 // Backtrack, overlapTest, divergence and selection of one group per wave,
 // groups passOrder[first, last).  The rows are visited in score order 64 at
 // a time: the lanes test which starts still have a backtrack pointer, lane 0
@@ -659,7 +692,9 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		const int g = a.passOrder[qi];
 		const GroupDesc gd = a.groups[g];
 		const QueryDesc qd = a.queries[gd.query];
-		const int n = gd.n;
+		// wave-uniform (scalar) size: the staging choice below is a scalar
+		// branch, not an exec mask around every step of lane 0's walk
+		const int n = __builtin_amdgcn_readfirstlane(gd.n);
 		const int64_t hb = gd.hitStart;
 		const bool extSorted = gd.extLen > qd.curLen;
 		const int32_t* hc = groupHits(a.hcur, a.cur, hb, extSorted);
@@ -670,10 +705,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		const bool inLds = n <= kBt;
 		int32_t* btG = a.bt + hb;
 		auto btGet = [&](int r) -> int { return inLds ? static_cast<int>(btS[r]) : btG[r]; };
-		auto btClear = [&](int r) {
-			if (inLds) btS[r] = -1;
-			else btG[r] = -1;
-		};
 		if (inLds)
 		{
 #pragma unroll 4
@@ -699,31 +730,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 			recStart[lane] = myStart;
 			waveSync();
 			if (lane == 0)
-			{
-				int nrec = 0;
-				unsigned long long mm = m;
-				while (mm)
-				{
-					const int l = __ffsll(static_cast<long long>(mm)) - 1;
-					mm &= mm - 1;
-					const int start = recStart[l];
-					if (btGet(start) == -1) continue;
-					int f = 0, len = 0, pos = start;
-					while (pos != -1)
-					{
-						f = pos;
-						++len;
-						const int nxt = btGet(pos);
-						btClear(pos);
-						pos = nxt;
-					}
-					recFirst[nrec] = f;
-					recLen[nrec] = len;
-					recStart[nrec] = start;  // slots below l are no longer read
-					++nrec;
-				}
-				red[0] = nrec;
-			}
+				red[0] = inLds ? walkStarts(btS, m, recStart, recFirst, recLen)
+							   : walkStarts(btG, m, recStart, recFirst, recLen);
 			waveSync();
 			const int nrec = red[0];
 			bool ok = false;
