@@ -435,6 +435,10 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 	// 4-byte stores per row leave L2 lines partly dirty and multiply the
 	// written bytes); lookbacks within the LDS ring never read them back
 	int hs = 0, hbt = -1, hpm = 0;
+	// positions of the previous G rows in registers (lane sl holds row
+	// i - 1 - sl: the first lookback tile of a row reads no memory), and the
+	// current row's, loaded one row ahead
+	int32_t winC = 0, winE = 0, curN = 0, extN = 0;
 	bool done = false;
 	int iter = 0;
 	for (;;)
@@ -464,6 +468,10 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			bt = a.bt + gd.hitStart;
 			pmG = a.posR + gd.hitStart;  // free during the DP: holds max(score[0..j])
 			pm = 0;
+			winC = hc[0];  // row 0, for lane 0 of row 1's first tile
+			winE = he[0];
+			curN = n > 1 ? hc[1] : 0;
+			extN = n > 1 ? he[1] : 0;
 			if (sl == 0)
 			{
 				rg[0] = 0;
@@ -483,8 +491,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 		if (__ballot(!done) == 0) break;
 		if (!done)
 		{
-			const int32_t curN = hc[i];
-			const int32_t extN = he[i];
 			int carry = 0;
 			int carryId = 0;
 			for (int tileBase = i - 1; tileBase >= 0; tileBase -= G)
@@ -496,8 +502,19 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 				bool stop = false;
 				if (active)
 				{
-					const int32_t dc = curN - hc[j];
-					const int32_t de = extN - he[j];
+					int32_t cj, ej;
+					if (tileBase == i - 1)
+					{
+						cj = winC;
+						ej = winE;
+					}
+					else
+					{
+						cj = hc[j];
+						ej = he[j];
+					}
+					const int32_t dc = curN - cj;
+					const int32_t de = extN - ej;
 					const int32_t jd = iabs(dc - de);
 					if (0 < dc && dc < maxJump && 0 < de && de < maxJump && jd <= p.max_jump_gap)
 					{
@@ -580,6 +597,17 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 					pmG[r] = hpm;
 				}
 			}
+			const int32_t nc = i + 1 < n ? hc[i + 1] : 0;
+			const int32_t ne = i + 1 < n ? he[i + 1] : 0;
+			winC = subPrev<G>(winC);
+			winE = subPrev<G>(winE);
+			if (sl == 0)
+			{
+				winC = curN;
+				winE = extN;
+			}
+			curN = nc;
+			extN = ne;
 			++i;
 		}
 		// The ring write of this row is read by the next row's lanes: LDS
