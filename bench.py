@@ -43,7 +43,7 @@ KERNEL_NAMES = {
     "heap": "segHeapLdsKernel (+ segHeapGlobalKernel)",
     "prefilter": "chainPrefilterKernel",
     "dp": "chainDpKernel<G>",
-    "walk": "chainWalkKernel<16384> + chainWalkKernel<2048>",
+    "walk": "chainWalkKernel<32768/16384/8192/2048, onlyMaxExt>",
     "rounds": "segMedian/Map/Count/Rank/Swap/SplitKernel + scans",
 }
 
@@ -52,13 +52,14 @@ def pmc_traffic(family: str, tuning: dict):
     """HBM bytes per launch of a kernel family from the committed rocprofv3
     --pmc FETCH_SIZE / WRITE_SIZE runs (each scaled by what the counter reads
     for a 1 GiB device copy), or None when they were measured at another
-    tuning (bytes per launch grow with the batch size)."""
+    batch size (bytes per launch follow the batch; batches in flight and DP
+    lanes change scheduling only)."""
     path = ROOT / "profiles" / "pmc_traffic.json"
     if not path.exists():
         return None
     d = json.loads(path.read_text())
     t = d["measured_with"]["tuning"]
-    if any(tuning.get(k) != t[k] for k in ("queries_per_batch", "batches_in_flight")):
+    if tuning.get("queries_per_batch") != t["queries_per_batch"]:
         return None
     f = d["families"].get(family)
     return f["hbm_bytes_per_launch"] if f else None
