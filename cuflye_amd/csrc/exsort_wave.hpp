@@ -64,6 +64,63 @@ __device__ inline unsigned long long laneMaskLt()
 	return l == 0 ? 0ULL : (~0ULL >> (64 - l));
 }
 
+// exsort::introLoop for one lane's range of a tile: at most kLaneMax
+// elements, or no depth budget left (straight to the heap sort).  Same
+// partitions on the same ranges with the same budgets; the pending larger
+// parts are kept in four packed registers instead of a dynamically indexed
+// array, which the compiler places in scratch memory.  Continuing with the
+// smaller part bounds the pending parts: a part pushed while partitioning a
+// range of S elements leaves a current range of at most S / 2, and only
+// ranges above exsort::kThreshold (16) are partitioned, so from kLaneMax <= 63
+// elements at most two parts are pending.  Tile positions and budgets fit
+// 11 + 11 + 10 bits.
+template <class K, int kLaneMax>
+__device__ inline void laneIntroLoop(const exsort::Span<K>& s, int first, int last, int depth)
+{
+	static_assert(kLaneMax < 4 * exsort::kThreshold, "two pending parts at most");
+	auto pack = [](int f, int l, int d) -> uint32_t {
+		return static_cast<uint32_t>(f) | (static_cast<uint32_t>(l) << 11) | (static_cast<uint32_t>(d) << 22);
+	};
+	uint32_t st0 = 0, st1 = 0, st2 = 0, st3 = 0;
+	int top = 0;
+	int f = first, l = last, d = depth;
+	for (;;)
+	{
+		while (l - f > exsort::kThreshold)
+		{
+			if (d == 0)
+			{
+				exsort::heapSort(s, f, l);
+				break;
+			}
+			--d;
+			const int cut = exsort::partitionPivot(s, f, l);
+			st3 = st2;
+			st2 = st1;
+			st1 = st0;
+			++top;
+			if (cut - f < l - cut)
+			{
+				st0 = pack(cut, l, d);
+				l = cut;
+			}
+			else
+			{
+				st0 = pack(f, cut, d);
+				f = cut;
+			}
+		}
+		if (top == 0) break;
+		f = static_cast<int>(st0 & 0x7ffu);
+		l = static_cast<int>((st0 >> 11) & 0x7ffu);
+		d = static_cast<int>(st0 >> 22);
+		st0 = st1;
+		st1 = st2;
+		st2 = st3;
+		--top;
+	}
+}
+
 // Partition s[f, l) around the median moved to f; all 64 lanes call it.
 // posL / posR need (l - f) / 2 + 1 entries.  freeSeed 0: std::sort's median
 // of (f + 1, mid, l - 1); otherwise ("free mode", where any correct sort will
@@ -198,10 +255,8 @@ __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, Wave
 		{
 			const WSeg w = t.lanes[lane];
 			// a lane range has at most kLaneMax elements or no depth budget
-			// left (straight to heapsort): smaller part first keeps at most
-			// log2(kLaneMax / 16) + 1 entries on the stack
-			static_assert(kLaneMax <= 16 << 4, "lane stack bound");
-			exsort::introLoop<6>(s, w.f, w.l, w.d & (kWaveFree - 1));
+			// left (straight to heapsort)
+			laneIntroLoop<K, kLaneMax>(s, w.f, w.l, w.d & (kWaveFree - 1));
 			exsort::insertionSort(s, w.f, w.l);
 		}
 		nl = 0;
@@ -248,11 +303,12 @@ __device__ void waveTileSort(const exsort::Span<K>& g, int f, int l, int d, Wave
 		if (lane == 0)
 		{
 			// larger part below, smaller on top: the stack stays within log2(n) + 1
+			// (fields chosen one by one: a choice between two local structs
+			// compiles to scratch stores and reloads)
 			const int dd = (dw - 1) | (freeW ? kWaveFree : 0);
-			const WSeg a{w.f, cut, dd}, b{cut, w.l, dd};
 			const bool leftSmaller = cut - w.f < w.l - cut;
-			t.stack[top] = leftSmaller ? b : a;
-			t.stack[top + 1] = leftSmaller ? a : b;
+			t.stack[top] = WSeg{leftSmaller ? cut : w.f, leftSmaller ? w.l : cut, dd};
+			t.stack[top + 1] = WSeg{leftSmaller ? w.f : cut, leftSmaller ? cut : w.l, dd};
 		}
 		top += 2;
 		waveSync();
@@ -314,10 +370,9 @@ __device__ void waveStdSort(const exsort::Span<K>& g, int n, int* posL, int* pos
 		const int cut = wavePartition(g, w.f, w.l, posL, posR);
 		if (laneId() == 0)
 		{
-			const WSeg a{w.f, cut, w.d - 1}, b{cut, w.l, w.d - 1};
 			const bool leftSmaller = cut - w.f < w.l - cut;
-			t.gstack[top] = leftSmaller ? b : a;
-			t.gstack[top + 1] = leftSmaller ? a : b;
+			t.gstack[top] = WSeg{leftSmaller ? cut : w.f, leftSmaller ? w.l : cut, w.d - 1};
+			t.gstack[top + 1] = WSeg{leftSmaller ? w.f : cut, leftSmaller ? cut : w.l, w.d - 1};
 		}
 		top += 2;
 		waveSync();
