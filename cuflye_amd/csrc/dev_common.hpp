@@ -26,6 +26,23 @@ __host__ __device__ inline int64_t toSigned(uint32_t iid)
 // ---- wave64 helpers ------------------------------------------------------
 __device__ inline int laneId() { return static_cast<int>(__lane_id()); }
 
+// ---- XCD-aware block order ------------------------------------------------
+// Workgroups are dealt to the 8 XCDs round robin by block index, and every
+// XCD has its own L2.  For a kernel whose accesses are random inside a
+// contiguous region of its elements (a query's matches), this index gives the
+// blocks one XCD runs one contiguous eighth of the grid, so a region's lines
+// are fetched and written back through one L2 instead of up to eight (partial
+// lines from several L2s multiply the bytes written).  A bijection on
+// [0, gridDim.x).
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcdBlockIndex()
+{
+	const uint32_t n = gridDim.x, b = blockIdx.x;
+	const uint32_t q = n / kXcds, r = n % kXcds;
+	const uint32_t x = b % kXcds, k = b / kXcds;
+	return x * q + (x < r ? x : r) + k;
+}
+
 // ---- segment lookup ------------------------------------------------------
 // Largest r in [lo, n) with off[r] <= w (off ascending, empty segments
 // allowed; off[lo] <= w).

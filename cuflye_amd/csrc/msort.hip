@@ -38,7 +38,8 @@ __global__ void iotaKernel(const int64_t* qOff, int32_t nq, uint32_t* v, int64_t
 __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, const uint32_t* sval,
 							   uint32_t* rkey, int32_t* dupFlag, int64_t n)
 {
-	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	// rank keys are scattered inside the query's range: one XCD per range
+	const int64_t i = static_cast<int64_t>(xcdBlockIndex()) * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const int q = findSegWave(qOff, nq, i);
 	const int64_t b = qOff[q], e = qOff[q + 1];

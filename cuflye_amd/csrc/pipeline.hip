@@ -488,7 +488,8 @@ __global__ void writeCandidatesKernel(JoinArgs a, const int64_t* qMatchOff, cons
 __global__ void gatherHitsKernel(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const uint32_t* val,
 								 const int32_t* mext, int32_t* hcur, int32_t* hext, int32_t* groupFlag, int64_t nMatches)
 {
-	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	// extPos is gathered inside the query's range: one XCD per range
+	const int64_t i = static_cast<int64_t>(xcdBlockIndex()) * blockDim.x + threadIdx.x;
 	if (i >= nMatches) return;
 	const int q = findSegWave(qMatchOff, nQueries, i);
 	const int64_t qb = qMatchOff[q];
