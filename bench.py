@@ -83,6 +83,10 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson request rate (1/s)")
     ap.add_argument("--serve-iteration-reads", type=int, default=4096, help="reads per serving iteration")
     ap.add_argument("--serve-genome", type=int, default=200_000_000)
+    ap.add_argument("--dp-stress-lists", type=int, default=2_000_000, help="DP-stress hit lists (0 = skip the leg)")
+    ap.add_argument("--dp-stress-cpu-lists", type=int, default=100_000, help="DP-stress lists timed on the oracle")
+    ap.add_argument("--r2g-scale", type=float, default=0.25, help="read-to-graph graph scale (0 = skip the leg)")
+    ap.add_argument("--r2g-cpu-reads", type=int, default=300, help="reads whose quickSeqOverlaps the oracle times")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
     return ap.parse_args()
@@ -147,6 +151,135 @@ def serving_leg(args, cores: int) -> dict:
     return rep
 
 
+def dp_stress_leg(args, cores: int, dp_lanes: int) -> dict:
+    """Chain stage alone (flyehip_chain_lists: prefilters, extPos sort, DP,
+    score order, walk, selection) on the DP-stress hit lists: Pareto(1.5, 8)
+    list sizes truncated at 200,000, each a drifting diagonal plus 30%
+    spurious hits.  GPU rate over the whole set (best of two runs after a
+    warm-up); the oracle on the first lists as CPU baseline and parity check."""
+    import cuflye_amd.synth as S
+    from cuflye_amd.native import KERNEL_FAMILIES
+    from cuflye_amd.overlap import FlyeHip, OverlapConfig
+
+    t = time.time()
+    h = S.config3(n_lists=args.dp_stress_lists)
+    n, nh = len(h.offsets) - 1, len(h.cur)
+    log(f"dp stress: {n} lists, {nh} hits ({time.time() - t:.1f}s)")
+    cfg = OverlapConfig(kmer_size=15, minimum_overlap=200, check_overhang=False)
+    arrs = (h.cur, h.ext, h.offsets, h.cur_id, h.ext_id, h.cur_len, h.ext_len)
+    ctx = FlyeHip(device=0)
+    ctx.set_tuning(dp_lanes_per_group=dp_lanes)
+    rows, roff = ctx.chain_lists(*arrs, cfg)  # warm
+    # lookback tile width: these lists look back over many more predecessors
+    # than the read overlaps, so the sweep is run here (rows must not change)
+    sweep = {}
+    for lanes in (8, 16, 32, 64):
+        ctx.set_tuning(dp_lanes_per_group=lanes)
+        t = time.perf_counter()
+        r2, o2 = ctx.chain_lists(*arrs, cfg)
+        sweep[lanes] = time.perf_counter() - t
+        assert np.array_equal(o2, roff) and r2.tobytes() == rows.tobytes(), f"DP lanes {lanes} changed the rows"
+    dp_lanes = min(sweep, key=sweep.get)
+    ctx.set_tuning(dp_lanes_per_group=dp_lanes)
+    t = time.perf_counter()
+    rows, roff = ctx.chain_lists(*arrs, cfg)
+    best = min(time.perf_counter() - t, sweep[dp_lanes])
+    st = ctx.stats()
+    out = {"workload": "2,000,000 DP-stress hit lists (Pareto(1.5, 8) sizes up to 200,000; diagonal with drift "
+                       "plus 30% spurious hits), onlyMaxExt, k=15, minimum overlap 200",
+           "lists": n, "hits": nh, "rows": int(len(rows)), "ms": best * 1e3, "lists_per_s": n / best,
+           "hits_per_s": nh / best, "dp_lanes": dp_lanes,
+           "dp_lanes_sweep_ms": {str(k): v * 1e3 for k, v in sweep.items()},
+           "stage_ms": {k: st[k] for k in ("ms_chain", "ms_total")},
+           "kernels_ms": dict(zip(KERNEL_FAMILIES, [float(x) for x in st["kernel_ms"]]))}
+    if not args.skip_cpu:
+        from oracle import oracle as O
+        m = min(n, args.dp_stress_cpu_lists)
+        e = int(h.offsets[m])
+        sub = (h.cur[:e], h.ext[:e], h.offsets[: m + 1], h.cur_id[:m], h.ext_id[:m], h.cur_len[:m], h.ext_len[:m])
+        t = time.perf_counter()
+        erows, eroff = O.chain_lists(*sub, cfg, threads=cores)
+        cpu_s = time.perf_counter() - t
+        grows, groff = ctx.chain_lists(*sub, cfg)
+        out["parity_sample"] = bool(np.array_equal(groff, eroff) and grows.tobytes() == erows.tobytes())
+        out["cpu_baseline"] = {"value": e / cpu_s, "unit": "hits/s", "cores": cores, "kind": "port",
+                               "sample": f"first {m} lists ({e} hits, {cpu_s:.1f} s)"}
+        out["speedup_vs_cpu"] = (nh / best) / (e / cpu_s)
+    ctx.close()
+    return out
+
+
+def read_to_graph_leg(args, cores: int) -> dict:
+    """Read-to-graph alignment on the synthetic repeat graph (config 4 at
+    --r2g-scale): edge-sequence index, quickSeqOverlaps of every read (the
+    read-to-graph detector shape of getSeqOverlaps, reads handed over from the
+    host), the chain-input filter + sort and chainReadAlignments
+    (flyehip_read_align_chains).  The oracle runs the chaining on all reads
+    and quickSeqOverlaps on the first reads, as CPU baseline and parity check."""
+    import cuflye_amd.synth as S
+    from cuflye_amd import readtograph as R
+    from cuflye_amd.native import KERNEL_FAMILIES
+    from cuflye_amd.overlap import FlyeHip
+
+    t = time.time()
+    g4 = S.config4(scale=args.r2g_scale, seed=4)
+    reads = g4.reads
+    log(f"read-to-graph: {len(g4.left)} edges ({len(g4.edges_bases)} bp), {reads.n_reads} reads "
+        f"({time.time() - t:.1f}s)")
+    prm = R.ReadToGraphParams()
+    cfg = prm.detector()
+    ids = np.arange(1, reads.n_reads + 1, dtype=np.int64)
+    ctx = FlyeHip(device=0)
+    ctx.store_reads(g4.edges_bases, g4.edges_offsets)
+    ctx.build_index(prm.kmer_size, solid_min=1, rep_max=1 << 20)
+    ctx.get_seq_overlaps_external(reads.bases, reads.offsets, ids, cfg)  # warm
+    t = time.perf_counter()
+    rows, roff = ctx.get_seq_overlaps_external(reads.bases, reads.offsets, ids, cfg)
+    ov_s = time.perf_counter() - t
+    st = ctx.stats()
+    aln, aoff = R.edge_alignments(rows, roff, g4.left, g4.right)
+    ctx.read_align_chains(aln, aoff, R.LONG_EDGE, R.BIG_ALN, prm.chain())  # warm
+    t = time.perf_counter()
+    gi, gk, gs, go = ctx.read_align_chains(aln, aoff, R.LONG_EDGE, R.BIG_ALN, prm.chain())
+    ch_s = time.perf_counter() - t
+    out = {"workload": f"synthetic repeat graph, config 4 at scale {args.r2g_scale} ({len(g4.left)} edges, "
+                       f"{reads.n_reads} reads, {len(reads.bases)} bases)",
+           "reads": int(reads.n_reads), "alignments": int(len(aln)), "segments": int(len(gs)),
+           "quick_seq_overlaps": {"ms": ov_s * 1e3, "reads_per_s": reads.n_reads / ov_s,
+                                  "note": "reads handed over from host memory per call",
+                                  "stage_ms": {k: st[k] for k in ("ms_join", "ms_sort", "ms_group", "ms_chain",
+                                                                  "ms_extract", "ms_total")},
+                                  "counts": {k: st[k] for k in ("matches", "groups", "overlaps", "batches")},
+                                  "kernels_ms": dict(zip(KERNEL_FAMILIES, [float(x) for x in st["kernel_ms"]]))},
+           "chain": {"ms": ch_s * 1e3, "reads_per_s": reads.n_reads / ch_s, "alignments_per_s": len(aln) / ch_s}}
+    if not args.skip_cpu:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        ei, ek, es, eo = O.read_align_chains(aln, aoff, R.LONG_EDGE, R.BIG_ALN, prm.chain())
+        cpu_ch = time.perf_counter() - t
+        chain_parity = bool(np.array_equal(gk, ek) and np.array_equal(gi, ei) and np.array_equal(go, eo)
+                            and gs.tobytes() == es.tobytes())
+        orc = O.Oracle(g4.edges_bases, g4.edges_offsets)
+        orc.build_index(prm.kmer_size, solid_min=1, rep_max=1 << 20)
+        m = min(reads.n_reads, args.r2g_cpu_reads)
+        sample = ids[:m]
+        sb = reads.bases[: reads.offsets[m]]
+        so = reads.offsets[: m + 1]
+        t = time.perf_counter()
+        erows, eroff = orc.overlaps(sample, cfg, threads=cores, seqs=(sb, so))
+        cpu_ov = time.perf_counter() - t
+        grows, groff = ctx.get_seq_overlaps_external(sb, so, sample, cfg)
+        ov_parity = bool(np.array_equal(groff, eroff) and grows.tobytes() == erows.tobytes())
+        out["parity"] = {"chain_all_reads": chain_parity, "quick_seq_overlaps_sample": ov_parity}
+        out["cpu_baseline"] = {"chain": {"value": reads.n_reads / cpu_ch, "unit": "reads/s", "cores": 1,
+                                         "kind": "port", "sample": f"all {reads.n_reads} reads ({cpu_ch:.2f} s)"},
+                               "quick_seq_overlaps": {"value": m / cpu_ov, "unit": "reads/s", "cores": cores,
+                                                      "kind": "port",
+                                                      "sample": f"first {m} reads ({cpu_ov:.1f} s)"}}
+    ctx.close()
+    return out
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -185,7 +318,7 @@ def main() -> int:
         # the whole workload: batch sizes are only comparable at the real
         # number of batches (a sample of a few batches hides the overlap)
         tuning = autotune.tune(g, ids, cfg, batches=(args.batch,) if args.batch else (2048, 3072, 4096, 5120, 6144),
-                               lanes=(args.dp_lanes,) if args.dp_lanes else (8, 4, 16, 32),
+                               lanes=(args.dp_lanes,) if args.dp_lanes else (8, 4, 16, 32, 64),
                                flights=(args.flights,) if args.flights else (2, 3), log=log)
         log(f"autotune best: {tuning['best']}")
 
@@ -256,6 +389,13 @@ def main() -> int:
     serving = serving_leg(args, cores) if args.serve_seconds > 0 else None
     if serving:
         log(f"serving: {serving}")
+    dp_lanes_run = (tuning["best"]["dp_lanes"] if tuning else args.dp_lanes) or 8
+    dp_stress = dp_stress_leg(args, cores, dp_lanes_run) if args.dp_stress_lists > 0 else None
+    if dp_stress:
+        log(f"dp stress: {dp_stress}")
+    r2g = read_to_graph_leg(args, cores) if args.r2g_scale > 0 else None
+    if r2g:
+        log(f"read-to-graph: {r2g}")
 
     run_tuning = tuning["best"] if tuning else {"dp_lanes": args.dp_lanes, "queries_per_batch": args.batch,
                                                  "batches_in_flight": args.flights}
@@ -287,6 +427,8 @@ def main() -> int:
         "per_query": per_query,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "serving": serving,
+        "dp_stress": dp_stress,
+        "read_to_graph": r2g,
         "stage_ms": {k: stats[k] for k in ("ms_join", "ms_sort", "ms_group", "ms_chain", "ms_extract", "ms_total")},
         "counts": {k: stats[k] for k in ("windows", "matches", "groups", "overlaps", "batches", "tie_queries")},
         "index_build_s": index_s,

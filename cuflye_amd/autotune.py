@@ -27,7 +27,7 @@ ERR_BUDGET = 3  # FLYEHIP_ERR_BUDGET
 
 
 def tune(ctx: FlyeHip, query_ids: np.ndarray, config: OverlapConfig, batches=(2048, 3072, 4096, 5120, 6144),
-         lanes=(8, 4, 16, 32), flights=(2, 3), repeats: int = 1, log=None) -> dict:
+         lanes=(8, 4, 16, 32, 64), flights=(2, 3), repeats: int = 1, log=None) -> dict:
     results = []
     refused = []
     reference = [None]

@@ -313,9 +313,30 @@ __global__ void __launch_bounds__(FH_WAVE) chainGatherKernel(ChainArgs a)
 
 // ---- subgroup steps of the DP ---------------------------------------------
 // The lookback tile of G lanes runs one prefix maximum and one maximum per
-// step.  For G <= 16 a subgroup lies inside one 16-lane DPP row, so both run
-// on DPP row moves; LDS permutes (__shfl) would put a round trip each on the
-// tile's dependent chain.  Larger subgroups span rows and keep the permutes.
+// step, on DPP moves: LDS permutes (__shfl) would put a round trip each on the
+// tile's dependent chain.  For G <= 16 a subgroup lies inside one 16-lane DPP
+// row; 32- and 64-lane subgroups add the row broadcasts (row_bcast:15 and
+// row_bcast:31, as in a wave64 scan), wave_shr:1 and a readlane of the
+// subgroup's last lane.
+
+constexpr int kNegInf = -0x7fffffff - 1;
+
+// max over lanes [row start, l] after the four row shifts, then across rows
+// (wide subgroups only)
+template <int G>
+__device__ inline int widePrefixMax(int v)
+{
+	static_assert(G == 32 || G == 64, "wide subgroups");
+	v = max(v, __builtin_amdgcn_update_dpp(kNegInf, v, 0x111, 0xF, 0xF, false));
+	v = max(v, __builtin_amdgcn_update_dpp(kNegInf, v, 0x112, 0xF, 0xF, false));
+	v = max(v, __builtin_amdgcn_update_dpp(kNegInf, v, 0x114, 0xF, 0xF, false));
+	v = max(v, __builtin_amdgcn_update_dpp(kNegInf, v, 0x118, 0xF, 0xF, false));
+	// rows 1 and 3 take the last lane of rows 0 and 2
+	v = max(v, __builtin_amdgcn_update_dpp(kNegInf, v, 0x142, 0xA, 0xF, false));
+	// rows 2 and 3 take lane 31
+	if constexpr (G == 64) v = max(v, __builtin_amdgcn_update_dpp(kNegInf, v, 0x143, 0xC, 0xF, false));
+	return v;
+}
 
 // Lane l of a row reads lane l - O of the same row (lanes with l < O keep
 // their own value; callers ignore the lanes whose source lies in another
@@ -356,11 +377,7 @@ __device__ inline int subPrefixMax(int v, int sl)
 	}
 	else
 	{
-		for (int o = 1; o < G; o <<= 1)
-		{
-			const int t = __shfl_up(v, o, G);
-			if (sl >= o) v = max(v, t);
-		}
+		v = widePrefixMax<G>(v);
 	}
 	return v;
 }
@@ -370,7 +387,7 @@ template <int G>
 __device__ inline int subPrev(int v)
 {
 	if constexpr (G <= 16) return rowShr<1>(v);
-	else return __shfl_up(v, 1, G);
+	else return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
 }
 
 // max of v over the subgroup, in every lane
@@ -388,7 +405,16 @@ __device__ inline int subMax(int v)
 	}
 	else
 	{
-		for (int o = G / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, G));
+		const int pre = widePrefixMax<G>(v);
+		if constexpr (G == 64)
+		{
+			v = __builtin_amdgcn_readlane(pre, 63);
+		}
+		else
+		{
+			const int lo = __builtin_amdgcn_readlane(pre, 31), hi = __builtin_amdgcn_readlane(pre, 63);
+			v = laneId() < 32 ? lo : hi;
+		}
 	}
 	return v;
 }
