@@ -83,6 +83,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson request rate (1/s)")
     ap.add_argument("--serve-iteration-reads", type=int, default=4096, help="reads per serving iteration")
     ap.add_argument("--serve-genome", type=int, default=200_000_000)
+    # one GPU batch per serving iteration: per-batch fixed costs are paid once
+    ap.add_argument("--serve-batch", type=int, default=8192, help="serving: query records per GPU batch")
+    ap.add_argument("--serve-flights", type=int, default=1, help="serving: batches in flight")
     ap.add_argument("--dp-stress-lists", type=int, default=2_000_000, help="DP-stress hit lists (0 = skip the leg)")
     ap.add_argument("--dp-stress-cpu-lists", type=int, default=100_000, help="DP-stress lists timed on the oracle")
     ap.add_argument("--r2g-scale", type=float, default=0.25, help="read-to-graph graph scale (0 = skip the leg)")
@@ -110,6 +113,7 @@ def serving_leg(args, cores: int) -> dict:
     log(f"serve: genome {len(genome)} bp in {n_contigs} contigs, read pool {len(pool_o) - 1} "
         f"({time.time() - t:.1f}s)")
     ctx = FlyeHip(device=0)
+    ctx.set_tuning(queries_per_batch=args.serve_batch, batches_in_flight=args.serve_flights)
     t = time.time()
     ctx.store_reads(genome, offs)
     info = ctx.build_index(15, solid_min=1, rep_max=64)
@@ -147,6 +151,7 @@ def serving_leg(args, cores: int) -> dict:
     rep["seconds"] = args.serve_seconds
     rep["request_reads"] = "log-uniform 500-20000"
     rep["iteration_reads"] = args.serve_iteration_reads
+    rep["batch"] = {"queries_per_batch": args.serve_batch, "batches_in_flight": args.serve_flights}
     ctx.close()
     return rep
 
