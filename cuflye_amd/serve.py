@@ -65,6 +65,8 @@ class OverlapServer:
         self.active: list[Request] = []
         self.completed: list[Request] = []
         self.iterations = 0
+        self.iteration_ms: list[float] = []
+        self.iteration_reads: list[int] = []
         self._stop = threading.Event()
         self._thread: threading.Thread | None = None
         self.error: BaseException | None = None
@@ -132,6 +134,8 @@ class OverlapServer:
                 ids = np.array([req.query_ids[i] for req, i in picks], np.int64)
                 rows, roff = self.ctx.get_seq_overlaps_external(bases, offs, ids, self.config)
                 self.iterations += 1
+                self.iteration_ms.append(float(self.ctx.stats()["ms_total"]))
+                self.iteration_reads.append(len(picks))
                 now = time.perf_counter()
                 for j, (req, i) in enumerate(picks):
                     req.rows[i] = rows[roff[j]:roff[j + 1]].copy()
@@ -160,8 +164,14 @@ class OverlapServer:
             "p50_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "max_ms": float(lat.max() * 1e3),
+            # offered load while the server keeps up (first arrival to last finish)
             "reads_per_s": float(reads / (t1 - t0)) if t1 > t0 else 0.0,
             "iterations": self.iterations,
+            # the library's time per iteration (one flyehip_overlaps_external call)
+            "iteration_ms_mean": float(np.mean(self.iteration_ms)) if self.iteration_ms else 0.0,
+            "iteration_reads_mean": float(np.mean(self.iteration_reads)) if self.iteration_reads else 0.0,
+            "capacity_reads_per_s": (float(sum(self.iteration_reads) / (sum(self.iteration_ms) / 1e3))
+                                     if self.iteration_ms and sum(self.iteration_ms) > 0 else 0.0),
         }
 
 
