@@ -556,6 +556,33 @@ __global__ void divStatsKernel(const flyehip_overlap_t* rows, const int64_t* qRo
 	divCount[q] = cnt;
 }
 
+// 2-bit packing of ASCII sequences: one thread per output word (16 bases,
+// most significant first; a sequence starts on a word boundary and ends with
+// two spare zero words).  A byte other than A/C/G/T sets *err.
+__global__ void packBasesKernel(const uint8_t* bases, const int64_t* baseOff, const int64_t* wordOff,
+								const int32_t* lens, int32_t n, int64_t nWords, uint32_t* words, uint32_t* err)
+{
+	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (w >= nWords) return;
+	const int q = findSegWave(wordOff, n, w);
+	const int64_t j0 = (w - wordOff[q]) * 16;
+	const int64_t len = lens[q];
+	const uint8_t* src = bases + (baseOff[q] - baseOff[0]);
+	uint32_t v = 0;
+	bool bad = false;
+	for (int t = 0; t < 16; ++t)
+	{
+		const int64_t j = j0 + t;
+		if (j >= len) break;
+		const uint8_t b = src[j];
+		const int code = b == 'A' ? 0 : b == 'C' ? 1 : b == 'G' ? 2 : b == 'T' ? 3 : -1;
+		bad |= code < 0;
+		v |= static_cast<uint32_t>(code & 3) << (30 - 2 * t);
+	}
+	words[w] = v;
+	if (bad) atomicOr(err, 1u);
+}
+
 inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n + tpb - 1) / tpb)); }
 
 } // namespace
@@ -617,6 +644,16 @@ hipError_t launchBucketFinish(HashTable t, uint64_t* ordKeys, uint64_t* entries,
 {
 	hipLaunchKernelGGL(bucketFinishKernel, gridFor(static_cast<int64_t>(t.mask + 1), 256), dim3(256), 0, s, t, ordKeys,
 					   entries, readLen, k);
+	return hipGetLastError();
+}
+
+hipError_t launchPackBases(const uint8_t* bases, const int64_t* baseOff, const int64_t* wordOff,
+						   const int32_t* lens, int32_t n, int64_t nWords, uint32_t* words, uint32_t* err,
+						   hipStream_t s)
+{
+	if (nWords == 0) return hipSuccess;
+	hipLaunchKernelGGL(packBasesKernel, gridFor(nWords, 256), dim3(256), 0, s, bases, baseOff, wordOff, lens, n,
+					   nWords, words, err);
 	return hipGetLastError();
 }
 

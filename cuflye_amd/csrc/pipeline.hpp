@@ -82,6 +82,12 @@ struct JoinArgs
 	int32_t* filtered;         // filtered positions
 };
 
+// 2-bit packs n ASCII sequences (bases[baseOff[i] - baseOff[0] ..], lens[i]
+// bases) into words at wordOff (16 bases per word, two spare words per
+// sequence); *err gets bit 0 for a byte other than A/C/G/T.
+hipError_t launchPackBases(const uint8_t* bases, const int64_t* baseOff, const int64_t* wordOff,
+						   const int32_t* lens, int32_t n, int64_t nWords, uint32_t* words, uint32_t* err,
+						   hipStream_t s);
 hipError_t launchJoinCount(const JoinArgs& a, hipStream_t s);
 hipError_t launchJoinFill(const JoinArgs& a, hipStream_t s);
 

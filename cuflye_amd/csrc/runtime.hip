@@ -107,6 +107,7 @@ struct Lane
 	DevBuf bQueue, bCounters, bQRowCount, bCurLen, bDivOff, bDivCount, bDivOut, bQueryIds, bCand;
 	DevBuf bExtWords, bExtWordOff, bExtLen;
 	DevBuf bAln, bAlnOff, bRA;
+	DevBuf bRawBases, bRawOff, bPackErr;  // ASCII bases being packed (packSeqsDevice)
 };
 
 } // namespace
@@ -224,10 +225,14 @@ int baseCode(uint8_t b)
 }
 
 // Packs sequences 16 bases per word (MSB first) with two spare words each.
-void packSeqs(const uint8_t* bases, const int64_t* offsets, int32_t n, std::vector<uint32_t>& words,
-			  std::vector<int64_t>& wordOff, std::vector<int32_t>& lens)
+// 2-bit packing of sequences on the device: the host computes the word offsets and lengths
+// (one pass over the sequences, not the bases), the ASCII bytes cross PCIe as
+// they are and packBasesKernel packs them; only a failed check goes back to
+// the host copy to name the first byte other than A/C/G/T.
+void packSeqsDevice(flyehip_ctx* c, Lane* L, const uint8_t* bases, const int64_t* offsets, int32_t n,
+					DevBuf& dWords, DevBuf& dWordOff, DevBuf& dLens, std::vector<int64_t>& wordOff,
+					std::vector<int32_t>& lens)
 {
-	words.clear();
 	wordOff.assign(static_cast<size_t>(n) + 1, 0);
 	lens.assign(static_cast<size_t>(n), 0);
 	int64_t total = 0;
@@ -236,23 +241,32 @@ void packSeqs(const uint8_t* bases, const int64_t* offsets, int32_t n, std::vect
 		const int64_t len = offsets[i + 1] - offsets[i];
 		if (len < 0 || len > 0x7fffffff) fail(FLYEHIP_ERR_ARG, "sequence %d has invalid length %lld", i, static_cast<long long>(len));
 		wordOff[static_cast<size_t>(i)] = total;
+		lens[static_cast<size_t>(i)] = static_cast<int32_t>(len);
 		total += (len + 15) / 16 + 2;
 	}
 	wordOff[static_cast<size_t>(n)] = total;
-	words.assign(static_cast<size_t>(total), 0);
-	for (int32_t i = 0; i < n; ++i)
-	{
-		const int64_t len = offsets[i + 1] - offsets[i];
-		lens[static_cast<size_t>(i)] = static_cast<int32_t>(len);
-		const uint8_t* s = bases + offsets[i];
-		uint32_t* w = words.data() + wordOff[static_cast<size_t>(i)];
-		for (int64_t j = 0; j < len; ++j)
-		{
-			const int b = baseCode(s[j]);
-			if (b < 0) fail(FLYEHIP_ERR_ARG, "sequence %d has a non-ACGT byte 0x%02x at %lld", i, s[j], static_cast<long long>(j));
-			w[j >> 4] |= static_cast<uint32_t>(b) << (30 - 2 * (j & 15));
-		}
-	}
+	const int64_t nb = offsets[n] - offsets[0];
+	uint8_t* dB = ensureT<uint8_t>(c, L->bRawBases, static_cast<size_t>(nb) + 1);
+	int64_t* dBO = ensureT<int64_t>(c, L->bRawOff, static_cast<size_t>(n) + 1);
+	uint32_t* dErr = ensureT<uint32_t>(c, L->bPackErr, 1);
+	uint32_t* dW = ensureT<uint32_t>(c, dWords, static_cast<size_t>(total));
+	int64_t* dWO = ensureT<int64_t>(c, dWordOff, static_cast<size_t>(n) + 1);
+	int32_t* dL = ensureT<int32_t>(c, dLens, static_cast<size_t>(n) + 1);
+	upload(L, dB, bases + offsets[0], static_cast<size_t>(nb));
+	upload(L, dBO, offsets, (static_cast<size_t>(n) + 1) * 8);
+	upload(L, dWO, wordOff.data(), wordOff.size() * 8);
+	upload(L, dL, lens.data(), lens.size() * 4);
+	HIPCHECK(hipMemsetAsync(dErr, 0, 4, L->stream));
+	HIPCHECK(launchPackBases(dB, dBO, dWO, dL, n, total, dW, dErr, L->stream));
+	uint32_t err = 0;
+	download(L, &err, dErr, 4);
+	sync(L);
+	if (err)
+		for (int32_t i = 0; i < n; ++i)
+			for (int64_t j = offsets[i]; j < offsets[i + 1]; ++j)
+				if (baseCode(bases[j]) < 0)
+					fail(FLYEHIP_ERR_ARG, "sequence %d has a non-ACGT byte 0x%02x at %lld", i, bases[j],
+						 static_cast<long long>(j - offsets[i]));
 }
 
 void checkParams(const flyehip_overlap_params_t* p, int k)
@@ -889,7 +903,7 @@ void releaseBatchBuffers(flyehip_ctx* c, Lane& l)
 					  &l.bSegShadowV, &l.bQueue, &l.bCounters, &l.bQRowCount, &l.bCurLen, &l.bDivOff, &l.bDivCount,
 					  &l.bDivOut, &l.bQueryIds, &l.bCand, &l.bExtWords, &l.bExtWordOff, &l.bExtLen, &l.bAln,
 					  &l.bAlnOff, &l.bRA, &l.bGroupDefer, &l.bFlagReady, &l.bFlagDefer, &l.bPosReady, &l.bPosDefer,
-					  &l.bReadyOrder, &l.bDeferOrder})
+					  &l.bReadyOrder, &l.bDeferOrder, &l.bRawBases, &l.bRawOff, &l.bPackErr})
 		release(c, *b);
 }
 
@@ -1196,18 +1210,13 @@ int flyehip_store_reads(flyehip_ctx* c, const uint8_t* bases, const int64_t* off
 	return guarded(c, [&]() {
 		Lane* L = c->lanes[0].get();
 		if (!bases || !offsets || n_reads <= 0) fail(FLYEHIP_ERR_ARG, "empty read store");
-		std::vector<uint32_t> words;
 		std::vector<int64_t> wordOff;
 		std::vector<int32_t> lens;
-		packSeqs(bases, offsets, n_reads, words, wordOff, lens);
 		c->haveIndex = false;
+		packSeqsDevice(c, L, bases, offsets, n_reads, c->dWords, c->dWordOff, c->dReadLen, wordOff, lens);
 		c->nReads = n_reads;
 		c->readLen = lens;
 		c->readWordOff = wordOff;
-		upload(L, ensure(c, c->dWords, words.size() * 4), words.data(), words.size() * 4);
-		upload(L, ensure(c, c->dWordOff, wordOff.size() * 8), wordOff.data(), wordOff.size() * 8);
-		upload(L, ensure(c, c->dReadLen, lens.size() * 4), lens.data(), lens.size() * 4);
-		sync(L);
 	});
 }
 
@@ -1377,13 +1386,9 @@ int flyehip_overlaps_external(flyehip_ctx* c, const uint8_t* bases, const int64_
 	return guarded(c, [&]() {
 		Lane* L = c->lanes[0].get();
 		if (!c->haveIndex) fail(FLYEHIP_ERR_STATE, "no index: call flyehip_store_reads and flyehip_build_index first");
-		std::vector<uint32_t> words;
 		std::vector<int64_t> wordOff;
 		std::vector<int32_t> lens;
-		packSeqs(bases, offsets, n, words, wordOff, lens);
-		upload(L, ensure(c, L->bExtWords, words.size() * 4), words.data(), words.size() * 4);
-		upload(L, ensure(c, L->bExtWordOff, wordOff.size() * 8), wordOff.data(), wordOff.size() * 8);
-		upload(L, ensure(c, L->bExtLen, lens.size() * 4 + 4), lens.data(), lens.size() * 4);
+		packSeqsDevice(c, L, bases, offsets, n, L->bExtWords, L->bExtWordOff, L->bExtLen, wordOff, lens);
 		PackedSeqs s;
 		s.words = static_cast<const uint32_t*>(L->bExtWords.p);
 		s.wordOff = static_cast<const int64_t*>(L->bExtWordOff.p);
