@@ -262,9 +262,11 @@ EXS_HD void heapSortPacked(uint64_t* a, int n)
 // level per step in lock step, each on its own level.  A pop may start only
 // when the previous one has taken two steps, and when no running pop's hole
 // is an ancestor of the position the new pop reads its value from and writes
-// the popped root to (the last heap position).  heapSortPipelinedModel runs
-// exactly this schedule on the host; the GPU kernel runs the same steps with
-// one lane per running pop.
+// the popped root to (the last heap position).  The steps run in pairs and a
+// pop may start only in the first step of a pair (so two steps apart at
+// least; a pop that cannot start waits for the next pair, which the checks
+// above make harmless).  heapSortPipelinedModel runs exactly this schedule on
+// the host; the GPU kernel runs the same steps with one lane per running pop.
 
 EXS_HD bool isAncestorOrSelf(int h, int x)
 {
@@ -339,13 +341,13 @@ inline void heapSortPipelinedModel(uint64_t* a, int n)
 	bool act[kSlots] = {};
 	int hole[kSlots] = {}, size[kSlots] = {};
 	uint64_t val[kSlots] = {};
-	int L = n, lastStart = -1000, pops = 0;
+	int L = n, pops = 0;
 	for (int cyc = 0;; ++cyc)
 	{
-		if (L > 1)
+		if (L > 1 && (cyc & 1) == 0)
 		{
 			const int slot = pops % kSlots;
-			bool ok = cyc - lastStart >= 2 && !act[slot];
+			bool ok = !act[slot];
 			for (int j = 0; j < kSlots; ++j)
 				if (act[j] && isAncestorOrSelf(hole[j], L - 1)) ok = false;
 			if (ok)
@@ -355,14 +357,13 @@ inline void heapSortPipelinedModel(uint64_t* a, int n)
 				hole[slot] = 0;
 				size[slot] = L - 1;
 				act[slot] = true;
-				lastStart = cyc;
 				--L;
 				++pops;
 			}
 		}
 		bool any = false;
 		for (int j = 0; j < kSlots; ++j) any |= act[j];
-		if (!any && L <= 1) break;
+		if (!any && L <= 1 && (cyc & 1) == 0) break;
 		for (int j = 0; j < kSlots; ++j)
 			if (act[j]) act[j] = siftStepPacked(a, hole[j], size[j], val[j]);
 	}

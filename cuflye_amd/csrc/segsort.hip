@@ -332,14 +332,18 @@ __global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list
 			bool act = false;
 			int h = 0, m = 0;
 			uint64_t v = 0;
-			int L = n, lastStart = -1000, pops = 0;
+			int L = n, pops = 0;
 			// LDS operations of one wave complete in program order, so a cycle's
 			// writes reach the next cycle's reads without waiting; the wave
-			// barrier only keeps the compiler from moving them across cycles
-			for (int cyc = 0;; ++cyc)
+			// barrier only keeps the compiler from moving them across cycles.
+			// Pops start at most every other cycle, so cycles run in pairs and
+			// only the first of a pair may start one: the start checks (slot
+			// free, no sift above the last element) make the result independent
+			// of when a pop starts, and the second cycle is a plain step.
+			for (;;)
 			{
 				unsigned long long running = __ballot(act);
-				if (L > 1 && cyc - lastStart >= 2)
+				if (L > 1)
 				{
 					const int slot = pops % kSlots;
 					const bool blocked = __ballot(act && heapAncestorOrSelf(h, L - 1)) != 0;
@@ -354,12 +358,13 @@ __global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list
 							act = true;
 						}
 						running |= 1ULL << slot;
-						lastStart = cyc;
 						--L;
 						++pops;
 					}
 				}
 				if (running == 0 && L <= 1) break;
+				if (act) act = heapStep(hx, h, m, n, v);
+				__builtin_amdgcn_wave_barrier();
 				if (act) act = heapStep(hx, h, m, n, v);
 				__builtin_amdgcn_wave_barrier();
 			}
