@@ -95,6 +95,16 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -312,7 +322,9 @@ def main() -> int:
     from cuflye_amd import autotune
     from cuflye_amd.overlap import FlyeHip, OverlapConfig
 
-    cores = os.cpu_count() or 1
+    # host threads for the oracle: the CPUs this process may run on, at most
+    # the 16 a one-GPU share of the host provides
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
     t0 = time.time()
     reads = S.config2(seed=args.seed, scale=args.scale)
     log(f"reads={reads.n_reads} bases={len(reads.bases)} gen={time.time() - t0:.1f}s")
@@ -402,8 +414,15 @@ def main() -> int:
         cpu_s = time.perf_counter() - t
         grows, groff = g.get_seq_overlaps(sample, cfg)
         parity = bool(np.array_equal(groff, eroff) and grows.tobytes() == erows.tobytes())
+        # one thread on the first tenth of the sample (deterministic timing)
+        one = sample[: max(1, len(sample) // 10)]
+        t = time.perf_counter()
+        orc.overlaps(one, cfg, threads=1)
+        one_s = time.perf_counter() - t
         cpu = {"value": len(sample) / cpu_s, "unit": "query records/s", "cores": cores, "kind": "port",
-               "sample": f"first {len(sample)} query records of the same workload ({cpu_s:.1f} s)"}
+               "sample": f"first {len(sample)} query records of the same workload ({cpu_s:.1f} s)",
+               "single_thread": {"value": len(one) / one_s, "sample": f"first {len(one)} query records"},
+               "host": {"nproc": os.cpu_count(), "cpu_model": cpu_model()}}
         log(f"cpu sample {len(sample)} in {cpu_s:.2f}s on {cores} threads; parity={parity}")
     g.close()
 
