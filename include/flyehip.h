@@ -222,8 +222,9 @@ int flyehip_set_tuning(flyehip_ctx* ctx, int32_t dp_lanes_per_group,
 /* ---- resident read store and k-mer index (VertexIndex) ---------------- */
 
 /* Uploads reads 1..n_reads: bases are ASCII A/C/G/T (other bytes are an
- * error), read i spans bases[offsets[i] .. offsets[i+1]).  The store is kept
- * 2-bit packed in HBM.  Replaces what VertexIndex reads from
+ * error), read i spans bases[offsets[i] .. offsets[i+1]).  The bytes are
+ * copied to the device as they are and packed there; the store is kept 2-bit
+ * packed in HBM.  Replaces what VertexIndex reads from
  * SequenceContainer (patch 0002 file lines 52-95). */
 int flyehip_store_reads(flyehip_ctx* ctx, const uint8_t* bases,
                         const int64_t* offsets, int32_t n_reads);
