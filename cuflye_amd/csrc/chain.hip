@@ -649,34 +649,43 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 
 // ---- finish ----------------------------------------------------------------------
 
-// Score-order sort keys of every passing group (std::sort(indices,
-// score[a] > score[b]) as an ascending key), pushed to the segmented sort.
-// The walk skips every start without a backtrack pointer, so among equal
-// scores only the order of rows that have one can matter: those carry
-// kWalkRelevant in their value.
-__global__ void __launch_bounds__(FH_WAVE) chainScoreKeysKernel(ChainArgs a, SegSortBufs sb)
+// start[g] = first hit of group g, start[nGroups] = nHits.
+__global__ void groupStartKernel(const GroupDesc* groups, int32_t nGroups, int64_t nHits, int64_t* start)
 {
-	const int lane = laneId();
-	for (;;)
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g < nGroups) start[g] = groups[g].hitStart;
+	if (g == nGroups) start[g] = nHits;
+}
+
+// Score-order sort keys of every passing group (std::sort(indices,
+// score[a] > score[b]) as an ascending key).  The walk skips every start
+// without a backtrack pointer, so among equal scores only the order of rows
+// that have one can matter: those carry kWalkRelevant in their value.  One
+// thread per hit (the groups' hits tile [0, nHits) in group order, so a hit
+// finds its group in the group starts); streaming, unlike a wave per group.
+__global__ void chainScoreKeysKernel(ChainArgs a, const int64_t* start, int64_t nHits)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= nHits) return;
+	const int g = findSegWave(start, a.nGroups, i);
+	if (!a.groupPass[g]) return;
+	a.sortKey[i] = static_cast<uint32_t>(0x7fffffff - a.score[i]);
+	a.sortVal[i] = static_cast<uint32_t>(i - start[g]) | (a.bt[i] != -1 ? kWalkRelevant : 0u);
+}
+
+// Pushes every passing group's range to the score sort (one lane per group).
+__global__ void chainScorePushKernel(ChainArgs a, SegSortBufs sb)
+{
+	const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+	const bool push = qi < a.nPass;
+	int32_t f = 0, n = 0;
+	if (push)
 	{
-		const int qi = pullGroup(a.queue);
-		if (qi >= a.nPass) break;
 		const GroupDesc gd = a.groups[a.passOrder[qi]];
-		const int n = gd.n;
-		const int64_t hb = gd.hitStart;
-		const int32_t* sc = a.score + hb;
-		const int32_t* bt = a.bt + hb;
-		uint32_t* sk = a.sortKey + hb;
-		uint32_t* sv = a.sortVal + hb;
-#pragma unroll 4
-		for (int i = lane; i < n; i += FH_WAVE)
-		{
-			sk[i] = static_cast<uint32_t>(0x7fffffff - sc[i]);
-			sv[i] = static_cast<uint32_t>(i) | (bt[i] != -1 ? kWalkRelevant : 0u);
-		}
-		segPushWave(sb, sb.big[0], lane == 0, static_cast<int32_t>(hb), static_cast<int32_t>(hb + n),
-					exsort::depthLimit(static_cast<uint32_t>(n)), 0);
+		f = static_cast<int32_t>(gd.hitStart);
+		n = gd.n;
 	}
+	segPushWave(sb, sb.big[0], push, f, f + n, exsort::depthLimit(static_cast<uint32_t>(n)), 0);
 }
 
 constexpr int kPropTile = 256;  // proposals sorted in LDS up to this count
@@ -1046,9 +1055,17 @@ hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream)
 	return hipGetLastError();
 }
 
-hipError_t launchChainScoreKeys(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream)
+hipError_t launchChainScoreKeys(const ChainArgs& a, const SegSortBufs& sb, int64_t* start, int64_t nHits,
+								hipStream_t stream)
 {
-	hipLaunchKernelGGL(chainScoreKeysKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a, sb);
+	const int tpb = 256;
+	hipLaunchKernelGGL(groupStartKernel, dim3((a.nGroups + 1 + tpb - 1) / tpb), dim3(tpb), 0, stream, a.groups,
+					   a.nGroups, nHits, start);
+	if (nHits > 0)
+		hipLaunchKernelGGL(chainScoreKeysKernel, dim3(static_cast<unsigned>((nHits + tpb - 1) / tpb)), dim3(tpb), 0,
+						   stream, a, static_cast<const int64_t*>(start), nHits);
+	if (a.nPass > 0)
+		hipLaunchKernelGGL(chainScorePushKernel, dim3((a.nPass + tpb - 1) / tpb), dim3(tpb), 0, stream, a, sb);
 	return hipGetLastError();
 }
 

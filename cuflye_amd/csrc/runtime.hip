@@ -108,6 +108,7 @@ struct Lane
 	DevBuf bExtWords, bExtWordOff, bExtLen;
 	DevBuf bAln, bAlnOff, bRA;
 	DevBuf bRawBases, bRawOff, bPackErr;  // ASCII bases being packed (packSeqsDevice)
+	DevBuf bGroupStart;                   // group starts of a batch's hits (score keys)
 };
 
 } // namespace
@@ -722,7 +723,8 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	HIPCHECK(segSortReset(sb, L->stream));
 	sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
 	ca.queue = queues + 3;
-	HIPCHECK(launchChainScoreKeys(ca, sb, blocks, L->stream));
+	HIPCHECK(launchChainScoreKeys(ca, sb, ensureT<int64_t>(c, L->bGroupStart, static_cast<size_t>(ng) + 1), nHits,
+								  L->stream));
 	runSegSort(L, sb, nullptr);
 	ca.queue = queues + 4;
 	// walk: score order and backtrack pointers read, 8 bytes per hit
@@ -903,7 +905,7 @@ void releaseBatchBuffers(flyehip_ctx* c, Lane& l)
 					  &l.bSegShadowV, &l.bQueue, &l.bCounters, &l.bQRowCount, &l.bCurLen, &l.bDivOff, &l.bDivCount,
 					  &l.bDivOut, &l.bQueryIds, &l.bCand, &l.bExtWords, &l.bExtWordOff, &l.bExtLen, &l.bAln,
 					  &l.bAlnOff, &l.bRA, &l.bGroupDefer, &l.bFlagReady, &l.bFlagDefer, &l.bPosReady, &l.bPosDefer,
-					  &l.bReadyOrder, &l.bDeferOrder, &l.bRawBases, &l.bRawOff, &l.bPackErr})
+					  &l.bReadyOrder, &l.bDeferOrder, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart})
 		release(c, *b);
 }
 
