@@ -465,6 +465,10 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 	// i - 1 - sl: the first lookback tile of a row reads no memory), and the
 	// current row's, loaded one row ahead
 	int32_t winC = 0, winE = 0, curN = 0, extN = 0;
+	// the same rows' scores and prefix maxima (what the LDS rings hold for
+	// them): the first tile reads its scores and the bounded-break test its
+	// prefix maximum without an LDS round trip
+	int32_t winS = 0, winP = 0;
 	bool done = false;
 	int iter = 0;
 	for (;;)
@@ -496,6 +500,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			pm = 0;
 			winC = hc[0];  // row 0, for lane 0 of row 1's first tile
 			winE = he[0];
+			winS = 0;  // row 0's ring entries
+			winP = 0;
 			curN = n > 1 ? hc[1] : 0;
 			extN = n > 1 ? he[1] : 0;
 			if (sl == 0)
@@ -528,8 +534,9 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 				bool stop = false;
 				if (active)
 				{
+					const bool firstTile = tileBase == i - 1;
 					int32_t cj, ej;
-					if (tileBase == i - 1)
+					if (firstTile)
 					{
 						cj = winC;
 						ej = winE;
@@ -548,7 +555,11 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 						const float pen = jd > p.gap_jump_threshold ? p.large_gap_penalty : p.small_gap_penalty;
 						const int32_t gc = static_cast<int32_t>(pen * static_cast<float>(jd));
 						int32_t sj;
-						if (i - j < kRing)
+						if (firstTile)
+						{
+							sj = winS;
+						}
+						else if (i - j < kRing)
 						{
 							sj = rg[j & (kRing - 1)];
 						}
@@ -585,7 +596,15 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 				}
 				if (bm) break;
 				const int jn = tileBase - G;
-				if (bounded && jn >= 0)
+				if (bounded && jn >= 0 && tileBase == i - 1)
+				{
+					// after the first tile: lane G - 1's window holds the prefix
+					// maximum of row jn + 1, which is at least that of row jn, so
+					// the test is the same bound, never looser
+					const unsigned long long bb = __ballot(sl == G - 1 && winP + k <= carry) & subMask;
+					if (bb) break;
+				}
+				else if (bounded && jn >= 0)
 				{
 					int mj;
 					if (i - jn < kRing)
@@ -627,10 +646,14 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			const int32_t ne = i + 1 < n ? he[i + 1] : 0;
 			winC = subPrev<G>(winC);
 			winE = subPrev<G>(winE);
+			winS = subPrev<G>(winS);
+			winP = subPrev<G>(winP);
 			if (sl == 0)
 			{
 				winC = curN;
 				winE = extN;
+				winS = si;
+				winP = pm;
 			}
 			curN = nc;
 			extN = ne;
