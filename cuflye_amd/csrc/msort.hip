@@ -27,14 +27,6 @@ namespace fh
 namespace
 {
 
-__global__ void iotaKernel(const int64_t* qOff, int32_t nq, uint32_t* v, int64_t n)
-{
-	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const int q = findSegWave(qOff, nq, i);
-	v[i] = static_cast<uint32_t>(i - qOff[q]);
-}
-
 __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, const uint32_t* sval,
 							   uint32_t* rkey, int32_t* dupFlag, int64_t n)
 {
@@ -70,7 +62,6 @@ hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBit
 {
 	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
 	const int tpb = 256;
-	hipLaunchKernelGGL(iotaKernel, gridFor(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.aval, nMatches);
 	hipError_t e = hipcub::DeviceSegmentedRadixSort::SortPairs(temp, tempBytes, a.akey, a.skey, a.aval, a.sval,
 															   static_cast<int>(nMatches), a.nQueries, a.qOff,
 															   a.qOff + 1, 32, 32 + idBits, s);

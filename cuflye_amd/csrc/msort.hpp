@@ -12,7 +12,7 @@ struct MsortArgs
 	const int64_t* qOff;   // per query match offsets (nQueries + 1)
 	int32_t nQueries;
 	const uint64_t* akey;  // emission-order keys: (internal extId << 32) | curPos
-	uint32_t* aval;        // emission index within the query (filled here)
+	const uint32_t* aval;  // emission index within the query (written by the join)
 	uint64_t* skey;        // stable sorted keys
 	uint32_t* sval;        // stable sorted permutation (emission index within the query)
 	uint32_t* rkey;        // emission-order rank keys (filled here)

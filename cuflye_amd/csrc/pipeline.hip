@@ -398,6 +398,7 @@ __global__ void __launch_bounds__(256) joinFillKernel(JoinArgs a)
 	uint64_t start = 0;
 	int self = -1;
 	int64_t outBase = 0;
+	int64_t qBase = 0;
 	uint32_t qiid = 0;
 	int32_t pos = 0;
 	int gflag = 0;
@@ -418,6 +419,7 @@ __global__ void __launch_bounds__(256) joinFillKernel(JoinArgs a)
 			gflag = g ? 1 : 0;
 			self = selfIndex(a, Q, pos, g, start, raw);
 			outBase = a.matchOff[w];
+			qBase = a.qMatchOff[q];
 		}
 	}
 	const int incl = waveInclusiveSum(raw);
@@ -440,6 +442,7 @@ __global__ void __launch_bounds__(256) joinFillKernel(JoinArgs a)
 		const uint64_t stL = __shfl(start, l, FH_WAVE);
 		const int selfL = __shfl(self, l, FH_WAVE);
 		const int64_t obL = __shfl(outBase, l, FH_WAVE);
+		const int64_t qbL = __shfl(qBase, l, FH_WAVE);
 		const int gL = __shfl(gflag, l, FH_WAVE);
 		const int32_t posL = __shfl(pos, l, FH_WAVE);
 		if (t < total)
@@ -454,6 +457,7 @@ __global__ void __launch_bounds__(256) joinFillKernel(JoinArgs a)
 				const int64_t o = obL + j - ((selfL >= 0 && j > selfL) ? 1 : 0);
 				a.mkey[o] = (static_cast<uint64_t>(id) << 32) | static_cast<uint32_t>(posL);
 				a.mext[o] = p;
+				a.mval[o] = static_cast<uint32_t>(o - qbL);
 			}
 		}
 	}

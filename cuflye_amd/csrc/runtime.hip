@@ -485,10 +485,12 @@ BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& 
 	bo.filtered = tot[1];
 	ja.mkey = ensureT<uint64_t>(c, L->bMKey, static_cast<size_t>(bo.matches) + 1);
 	ja.mext = ensureT<int32_t>(c, L->bMExt, static_cast<size_t>(bo.matches) + 1);
+	ja.mval = ensureT<uint32_t>(c, L->bAVal, static_cast<size_t>(bo.matches) + 1);
+	ja.qMatchOff = qMatchOff;
 	ja.filtered = ensureT<int32_t>(c, L->bFiltered, static_cast<size_t>(bo.filtered) + 1);
 	HIPCHECK(hipEventRecord(L->ev[6], L->stream));
 	const int tf = L->timer.begin(FLYEHIP_K_JOIN, L->stream,
-								  seqBytes + 8.0 * static_cast<double>(nWin) + 20.0 * static_cast<double>(bo.matches));
+								  seqBytes + 8.0 * static_cast<double>(nWin) + 24.0 * static_cast<double>(bo.matches));
 	HIPCHECK(launchJoinFill(ja, L->stream));
 	L->timer.end(tf, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[7], L->stream));
@@ -560,7 +562,7 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 	ms.qOff = static_cast<const int64_t*>(L->bQMatchOff.p);
 	ms.nQueries = nq;
 	ms.akey = ja.mkey;
-	ms.aval = ensureT<uint32_t>(c, L->bAVal, nm);
+	ms.aval = ja.mval;
 	ms.skey = ensureT<uint64_t>(c, L->bSKey, nm);
 	ms.sval = ensureT<uint32_t>(c, L->bVal, nm);
 	ms.rkey = ensureT<uint32_t>(c, L->bRKey, nm);
@@ -574,15 +576,15 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 	void* temp = ensure(c, L->bMsortTemp, tempBytes + 256);
 	SegSortBufs sb = segBufs(L, nMatches, nq);
 	sb.key = ms.rkey;
-	sb.val = ms.aval;
+	sb.val = ja.mval;
 	sb.outVal = ms.sval;
 	sb.dup = dup;
 	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
 	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
-	// stable order: 4 (index payload) + 2 radix passes x 24 (key, value read
-	// and written) + 20 (rank keys and tie flags) bytes per match
-	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, 72.0 * static_cast<double>(nMatches));
+	// stable order: 2 radix passes x 24 (key, value read and written) + 20
+	// (rank keys and tie flags) bytes per match
+	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, 68.0 * static_cast<double>(nMatches));
 	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, temp, tempBytes, L->stream));
 	L->timer.end(tr, L->stream);
 	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, L->stream));
