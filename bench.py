@@ -350,7 +350,7 @@ def main() -> int:
     else:
         # the whole workload: batch sizes are only comparable at the real
         # number of batches (a sample of a few batches hides the overlap)
-        tuning = autotune.tune(g, ids, cfg, batches=(args.batch,) if args.batch else (2048, 3072, 4096, 5120, 6144),
+        tuning = autotune.tune(g, ids, cfg, batches=(args.batch,) if args.batch else None,
                                lanes=(args.dp_lanes,) if args.dp_lanes else (8, 4, 16, 32, 64),
                                flights=(args.flights,) if args.flights else (2, 3), log=log)
         log(f"autotune best: {tuning['best']}")

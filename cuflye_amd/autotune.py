@@ -26,8 +26,16 @@ DP_LANES = (1, 4, 8, 16, 32, 64)
 ERR_BUDGET = 3  # FLYEHIP_ERR_BUDGET
 
 
-def tune(ctx: FlyeHip, query_ids: np.ndarray, config: OverlapConfig, batches=(2048, 3072, 4096, 5120, 6144),
+BATCHES = (2048, 3072, 4096, 5120, 6144)
+
+
+def tune(ctx: FlyeHip, query_ids: np.ndarray, config: OverlapConfig, batches=None,
          lanes=(8, 4, 16, 32, 64), flights=(2, 3), repeats: int = 1, log=None) -> dict:
+    """batches=None tries BATCHES plus, for every flight count, the run cut
+    into that many equal batches (one batch per stream, no lone last batch)."""
+    if batches is None:
+        n = len(query_ids)
+        batches = tuple(sorted(set(BATCHES) | {-(-n // fl) for fl in flights if n > 0}))
     results = []
     refused = []
     reference = [None]

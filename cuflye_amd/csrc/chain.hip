@@ -508,12 +508,12 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			{
 				rg[0] = 0;
 				rm[0] = 0;
-				if (n == 1)
-				{
-					sc[0] = 0;
-					bt[0] = -1;
-					pmG[0] = 0;
-				}
+				// row 0 is stored now: with one lane per group (G == 1)
+				// lane 0 takes row 1 before any block flush, so no flush
+				// stores it (wider tiles store it again, unchanged)
+				sc[0] = 0;
+				bt[0] = -1;
+				pmG[0] = 0;
 			}
 			hs = 0;  // row 0, held by lane 0
 			hbt = -1;

@@ -173,7 +173,9 @@ void* ensure(flyehip_ctx* c, DevBuf& b, size_t bytes)
 	if (bytes == 0) bytes = 16;
 	if (b.bytes >= bytes) return b.p;
 	release(c, b);
-	size_t want = bytes + bytes / 8;
+	// headroom for a somewhat larger next batch: 1/8, 1/32 from 1 GiB up
+	// (the per-match buffers, where 1/8 of three lanes is tens of GB)
+	size_t want = bytes + (bytes >= (size_t{1} << 30) ? bytes / 32 : bytes / 8);
 	want = (want + 255) & ~static_cast<size_t>(255);
 	{
 		// reserve under the lock, allocate outside it
@@ -802,15 +804,20 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 	ca.ext = hext;
 	ca.filtered = ja.filtered;
 	const size_t nm = static_cast<size_t>(nMatches) + 1;
-	ca.sortKey = ensureT<uint32_t>(c, L->bCKey, nm);
-	ca.sortVal = ensureT<uint32_t>(c, L->bCVal, nm);
-	// the match sort's partition scratch, free once the groups are built
+	// The match sort's per-match buffers are free once the groups are built
+	// and hold the chain's: its partition scratch, the rank keys and tie
+	// flags (sort keys and values), the tied-position prefix and the
+	// emission-order keys (8 bytes a match each: two int32 arrays, the second
+	// at a 256-byte boundary).  A lane then holds 24 bytes a match less.
+	const size_t half = (nm + 63) & ~static_cast<size_t>(63);
+	ca.sortKey = ensureT<uint32_t>(c, L->bRKey, nm);
+	ca.sortVal = ensureT<uint32_t>(c, L->bDupFlag, nm);
 	ca.posL = ensureT<int32_t>(c, L->bPosL, nm);
 	ca.posR = ensureT<int32_t>(c, L->bPosR, nm);
-	ca.hcur = ensureT<int32_t>(c, L->bCHCur, nm);
-	ca.hext = ensureT<int32_t>(c, L->bCHExt, nm);
-	ca.score = ensureT<int32_t>(c, L->bCScore, nm);
-	ca.bt = ensureT<int32_t>(c, L->bCBt, nm);
+	ca.hcur = ensureT<int32_t>(c, L->bDup, 2 * half);
+	ca.hext = ca.hcur + half;
+	ca.score = ensureT<int32_t>(c, L->bMKey, 2 * half);
+	ca.bt = ca.score + half;
 	// with onlyMaxExt a group keeps one proposal: per-group output, no list
 	ca.props = ensureT<Proposal>(c, L->bCProps, p.only_max_ext ? 1 : nm);
 	ca.outProps = ensureT<Proposal>(c, L->bCOut, p.only_max_ext ? static_cast<size_t>(nGroups) + 1 : nm);
