@@ -1036,17 +1036,18 @@ void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<Join
 	const int nl = std::max(1, std::min(std::min(c->batchesInFlight, maxLanes), nBatches));
 	while (static_cast<int>(c->lanes.size()) < nl) createLane(c);
 	std::vector<BatchResult> res(static_cast<size_t>(nBatches));
-	std::atomic<int32_t> next{0};
+	// lane i takes batch i first (the same lane gets the same batch on every
+	// call, so its grow-only buffers fit it after the first call), then the
+	// rest first-come
+	std::atomic<int32_t> next{nl};
 	std::vector<std::exception_ptr> errs(static_cast<size_t>(nl));
 	auto worker = [&](int li) {
 		try
 		{
 			HIPCHECK(hipSetDevice(c->device));
 			Lane* L = c->lanes[static_cast<size_t>(li)].get();
-			for (;;)
+			for (int32_t b = li; b < nBatches; b = next.fetch_add(1))
 			{
-				const int32_t b = next.fetch_add(1);
-				if (b >= nBatches) break;
 				const int32_t b0 = b * batchQueries;
 				runBatch(L, seqs, allQ, queryIds, *p, b0, std::min(n, b0 + batchQueries), divOffsets != nullptr,
 						 res[static_cast<size_t>(b)]);
