@@ -1,5 +1,6 @@
 """Host logic of the launch tuning and of the benchmark's reporting, without a
-GPU: the autotuner's search order, its memory-budget skipping and its check
+GPU: the autotuner's search order and default sizes (with the equal splits),
+its memory-budget skipping and its check
 that every candidate returns the same rows; the bench's refusal of several
 GPUs and its lookup of the committed HBM-traffic table."""
 import json
@@ -64,6 +65,20 @@ def test_autotune_rejects_a_candidate_that_changes_results():
     ctx = FakeCtx(budget=1 << 30, bad=(8, 4096, 2))
     with pytest.raises(RuntimeError, match="changed the results"):
         autotune.tune(ctx, np.arange(10), OverlapConfig(), batches=(2048, 4096), lanes=(8,), flights=(2,))
+
+
+def test_autotune_default_sizes_add_equal_splits():
+    # 20,418 query records: the fixed sizes plus one batch per stream
+    # (ceil(20418 / 2) = 10209, ceil(20418 / 3) = 6806), in growing order
+    ctx = FakeCtx(budget=1 << 30)
+    res = autotune.tune(ctx, np.arange(20418), OverlapConfig(), lanes=(8,), flights=(2, 3))
+    sizes = [c["queries_per_batch"] for c in res["candidates"]]
+    assert sorted(set(sizes)) == sorted(set(autotune.BATCHES) | {10209, 6806})
+    assert sizes == sorted(sizes)
+    # an explicit list is taken as it is
+    ctx = FakeCtx(budget=1 << 30)
+    res = autotune.tune(ctx, np.arange(20418), OverlapConfig(), batches=(4096,), lanes=(8,), flights=(2, 3))
+    assert {c["queries_per_batch"] for c in res["candidates"]} == {4096}
 
 
 def test_autotune_raises_when_nothing_fits():
