@@ -473,8 +473,19 @@ def main() -> int:
         "counts": {k: stats[k] for k in ("windows", "matches", "groups", "overlaps", "batches", "tie_queries")},
         "index_build_s": index_s,
     }
+    # the metric is bit-exact output: a failed parity check withholds the value
+    failed = [name for name, ok in (("parity_sample", parity),
+                                    ("dp_stress", (dp_stress or {}).get("parity_sample")),
+                                    ("read_to_graph.chain", ((r2g or {}).get("parity") or {}).get("chain_all_reads")),
+                                    ("read_to_graph.quick_seq_overlaps",
+                                     ((r2g or {}).get("parity") or {}).get("quick_seq_overlaps_sample")))
+              if ok is False]
+    if failed:
+        out["value_withheld"] = out["value"]
+        out["value"] = None
+        out["parity_failed"] = failed
     print(json.dumps(out))
-    return 0
+    return 1 if failed else 0
 
 
 if __name__ == "__main__":

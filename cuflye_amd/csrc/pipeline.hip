@@ -219,7 +219,23 @@ __global__ void fillKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin
 	ordKeys[start + slot] = (static_cast<uint64_t>(r) << 33) | (static_cast<uint64_t>(pos) << 1) | (flip ? 1ULL : 0ULL);
 }
 
-__global__ void bucketFinishKernel(HashTable t, uint64_t* ordKeys, uint64_t* entries, const int32_t* readLen, int k)
+// Bucket entry of one fill key (read << 33 | pos << 1 | flip): the stored
+// strand's internal id and the position where it spells the canonical k-mer.
+__device__ inline uint64_t bucketEntry(uint64_t x, const int32_t* readLen, int k)
+{
+	const uint32_t r = static_cast<uint32_t>(x >> 33);
+	const int32_t pos = static_cast<int32_t>((x >> 1) & 0xffffffffULL);
+	if (x & 1ULL) return (static_cast<uint64_t>(2 * r + 1) << 32) | static_cast<uint32_t>(readLen[r] - pos - k);
+	return (static_cast<uint64_t>(2 * r) << 32) | static_cast<uint32_t>(pos);
+}
+
+// Buckets of at most kBucketSerial entries are put in (read, pos) order by
+// their own thread (insertion sort); larger ones are listed for
+// bucketBlockSortKernel.
+constexpr int kBucketSerial = 32;
+
+__global__ void bucketFinishKernel(HashTable t, uint64_t* ordKeys, uint64_t* entries, const int32_t* readLen, int k,
+								   uint32_t* bigList, unsigned long long* bigCount, uint64_t bigCap)
 {
 	const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (h > t.mask) return;
@@ -227,6 +243,12 @@ __global__ void bucketFinishKernel(HashTable t, uint64_t* ordKeys, uint64_t* ent
 	if (!indexed(v)) return;
 	const uint64_t start = v >> kCountBits;
 	const int n = static_cast<int>(v & kCountMask);
+	if (n > kBucketSerial)
+	{
+		const unsigned long long i = atomicAdd(bigCount, 1ULL);
+		if (i < bigCap) bigList[i] = static_cast<uint32_t>(h);
+		return;
+	}
 	uint64_t* o = ordKeys + start;
 	for (int i = 1; i < n; ++i)
 	{
@@ -239,15 +261,66 @@ __global__ void bucketFinishKernel(HashTable t, uint64_t* ordKeys, uint64_t* ent
 		}
 		o[j] = x;
 	}
-	for (int i = 0; i < n; ++i)
+	for (int i = 0; i < n; ++i) entries[start + i] = bucketEntry(o[i], readLen, k);
+}
+
+// One block per listed bucket (grid-stride over the list): an ascending
+// bitonic network over the bucket in place, the positions past its end taken
+// as +infinity (every compare-exchange puts the minimum at the lower index,
+// so a pair with its upper index past the end never swaps and is skipped).
+// The keys of a bucket are distinct, so the result is the (read, pos) order.
+__global__ void __launch_bounds__(1024) bucketBlockSortKernel(HashTable t, uint64_t* ordKeys, uint64_t* entries,
+															   const int32_t* readLen, int k, const uint32_t* bigList,
+															   const unsigned long long* bigCount)
+{
+	const uint64_t nb = *bigCount;
+	for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x)
 	{
-		const uint64_t x = o[i];
-		const uint32_t r = static_cast<uint32_t>(x >> 33);
-		const int32_t pos = static_cast<int32_t>((x >> 1) & 0xffffffffULL);
-		uint64_t e;
-		if (x & 1ULL) e = (static_cast<uint64_t>(2 * r + 1) << 32) | static_cast<uint32_t>(readLen[r] - pos - k);
-		else e = (static_cast<uint64_t>(2 * r) << 32) | static_cast<uint32_t>(pos);
-		entries[start + i] = e;
+		const uint64_t v = t.vals[bigList[b]];
+		const uint64_t start = v >> kCountBits;
+		const int64_t n = static_cast<int64_t>(v & kCountMask);
+		uint64_t* o = ordKeys + start;
+		int64_t np2 = 1;
+		while (np2 < n) np2 <<= 1;
+		for (int64_t kk = 2; kk <= np2; kk <<= 1)
+		{
+			const int64_t half = kk >> 1;
+			for (int64_t p = threadIdx.x; p < (np2 >> 1); p += blockDim.x)
+			{
+				const int64_t i = (p / half) * kk + (p % half);
+				const int64_t j = (p / half) * kk + kk - 1 - (p % half);
+				if (j < n)
+				{
+					const uint64_t a = o[i], c = o[j];
+					if (c < a)
+					{
+						o[i] = c;
+						o[j] = a;
+					}
+				}
+			}
+			__syncthreads();
+			for (int64_t jj = half >> 1; jj >= 1; jj >>= 1)
+			{
+				for (int64_t p = threadIdx.x; p < (np2 >> 1); p += blockDim.x)
+				{
+					const int64_t i = (p / jj) * (2 * jj) + (p % jj);
+					const int64_t j = i + jj;
+					if (j < n)
+					{
+						const uint64_t a = o[i], c = o[j];
+						if (c < a)
+						{
+							o[i] = c;
+							o[j] = a;
+						}
+					}
+				}
+				__syncthreads();
+			}
+		}
+		for (int64_t i = threadIdx.x; i < n; i += blockDim.x) entries[start + i] = bucketEntry(o[i], readLen, k);
+		__syncthreads();
 	}
 }
 
@@ -644,10 +717,15 @@ hipError_t launchFill(const PackedSeqs& reads, const int64_t* winOff, int64_t nW
 }
 
 hipError_t launchBucketFinish(HashTable t, uint64_t* ordKeys, uint64_t* entries, const int32_t* readLen, int k,
-							  hipStream_t s)
+							  uint32_t* bigList, uint64_t* bigCount, uint64_t bigCap, hipStream_t s)
 {
+	auto* cnt = reinterpret_cast<unsigned long long*>(bigCount);
 	hipLaunchKernelGGL(bucketFinishKernel, gridFor(static_cast<int64_t>(t.mask + 1), 256), dim3(256), 0, s, t, ordKeys,
-					   entries, readLen, k);
+					   entries, readLen, k, bigList, cnt, bigCap);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return e;
+	hipLaunchKernelGGL(bucketBlockSortKernel, dim3(512), dim3(1024), 0, s, t, ordKeys, entries, readLen, k,
+					   static_cast<const uint32_t*>(bigList), static_cast<const unsigned long long*>(cnt));
 	return hipGetLastError();
 }
 

@@ -46,8 +46,11 @@ hipError_t launchClassify(HashTable t, int solidMin, int repMax, int32_t* sizes,
 hipError_t launchAssign(HashTable t, const int32_t* sizes, const int64_t* starts, hipStream_t s);
 hipError_t launchFill(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
 					  uint64_t* ordKeys, hipStream_t s);
+// Puts every bucket in (read, pos) order and writes its entries: buckets of at
+// most 32 entries by one thread each, larger ones (listed in bigList, at most
+// bigCap; *bigCount zeroed by the caller) by one 1,024-thread block each.
 hipError_t launchBucketFinish(HashTable t, uint64_t* ordKeys, uint64_t* entries, const int32_t* readLen,
-							  int k, hipStream_t s);
+							  int k, uint32_t* bigList, uint64_t* bigCount, uint64_t bigCap, hipStream_t s);
 
 // ---- join ---------------------------------------------------------------------
 // One query record of a join batch.

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <condition_variable>
 #include <mutex>
 #include <exception>
 #include <new>
@@ -83,6 +84,53 @@ struct DevBuf
 	size_t bytes = 0;
 };
 
+// Keeps device allocation out of the region where several lanes have work in
+// flight.  A lane holds the gate (enter .. leave) while it runs a batch; a
+// lane whose buffer must grow drains its own streams, gives up its hold and
+// waits until no lane holds the gate (every other lane is between batches,
+// its streams idle), allocates, and takes its hold back.  Waiting writers keep
+// new batches from starting, so a growth waits for at most one batch per
+// other lane.
+struct AllocGate
+{
+	std::mutex m;
+	std::condition_variable cv;
+	int active = 0;   // lanes holding the gate
+	int writers = 0;  // lanes waiting to allocate or allocating
+	bool writing = false;
+
+	void enter()
+	{
+		std::unique_lock<std::mutex> l(m);
+		cv.wait(l, [&] { return writers == 0; });
+		++active;
+	}
+	void leave()
+	{
+		std::lock_guard<std::mutex> l(m);
+		--active;
+		cv.notify_all();
+	}
+	void growBegin()
+	{
+		std::unique_lock<std::mutex> l(m);
+		--active;
+		++writers;
+		cv.notify_all();
+		cv.wait(l, [&] { return active == 0 && !writing; });
+		writing = true;
+	}
+	void growEnd()
+	{
+		std::unique_lock<std::mutex> l(m);
+		writing = false;
+		--writers;
+		cv.notify_all();
+		cv.wait(l, [&] { return writers == 0; });
+		++active;
+	}
+};
+
 // One batch pipeline: a stream (plus side streams) and the grow-only batch
 // buffers.  Batches on different lanes run concurrently; they share the read
 // store, the index and the memory budget of the context.
@@ -121,6 +169,8 @@ struct flyehip_ctx
 	uint64_t inUse = 0;
 	uint64_t peak = 0;
 	std::mutex memMu;  // budget accounting (lanes allocate from their own threads)
+	fh::AllocGate gate;           // allocation while several lanes run
+	int lanesRunning = 0;         // lanes of the current multi-lane call
 	std::string err;
 	std::mutex mu;
 	std::vector<std::unique_ptr<fh::Lane>> lanes;  // lanes[0] serves every single-stream call
@@ -168,10 +218,38 @@ void release(flyehip_ctx* c, DevBuf& b)
 	}
 }
 
+// The lane the calling host thread drives inside a multi-lane call.
+thread_local Lane* tlsLane = nullptr;
+
+// Waits for everything a lane has enqueued (its stream and side streams).
+void quiesce(Lane* L)
+{
+	HIPCHECK(hipStreamSynchronize(L->stream));
+	for (hipStream_t a : L->aux) HIPCHECK(hipStreamSynchronize(a));
+}
+
 void* ensure(flyehip_ctx* c, DevBuf& b, size_t bytes)
 {
 	if (bytes == 0) bytes = 16;
 	if (b.bytes >= bytes) return b.p;
+	// inside a multi-lane call: allocate only while no other lane has work
+	// in flight (AllocGate)
+	Lane* const L = tlsLane;
+	const bool gated = L != nullptr && c->lanesRunning > 1;
+	if (gated)
+	{
+		quiesce(L);
+		c->gate.growBegin();
+	}
+	struct Regate
+	{
+		flyehip_ctx* c;
+		bool on;
+		~Regate()
+		{
+			if (on) c->gate.growEnd();
+		}
+	} regate{c, gated};
 	release(c, b);
 	// headroom for a somewhat larger next batch: 1/8, 1/32 from 1 GiB up
 	// (the per-match buffers, where 1/8 of three lanes is tens of GB)
@@ -1041,14 +1119,33 @@ void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<Join
 	// rest first-come
 	std::atomic<int32_t> next{nl};
 	std::vector<std::exception_ptr> errs(static_cast<size_t>(nl));
+	c->lanesRunning = nl;
 	auto worker = [&](int li) {
+		Lane* L = c->lanes[static_cast<size_t>(li)].get();
+		tlsLane = L;
 		try
 		{
 			HIPCHECK(hipSetDevice(c->device));
-			Lane* L = c->lanes[static_cast<size_t>(li)].get();
 			for (int32_t b = li; b < nBatches; b = next.fetch_add(1))
 			{
 				const int32_t b0 = b * batchQueries;
+				// hold the allocation gate for the batch; leave it with the
+				// lane's streams idle
+				struct Hold
+				{
+					flyehip_ctx* c;
+					Lane* L;
+					Hold(flyehip_ctx* c_, Lane* L_) : c(c_), L(L_) { if (c->lanesRunning > 1) c->gate.enter(); }
+					~Hold()
+					{
+						if (c->lanesRunning > 1)
+						{
+							(void)hipStreamSynchronize(L->stream);
+							for (hipStream_t a : L->aux) (void)hipStreamSynchronize(a);
+							c->gate.leave();
+						}
+					}
+				} hold(c, L);
 				runBatch(L, seqs, allQ, queryIds, *p, b0, std::min(n, b0 + batchQueries), divOffsets != nullptr,
 						 res[static_cast<size_t>(b)]);
 			}
@@ -1058,11 +1155,13 @@ void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<Join
 			errs[static_cast<size_t>(li)] = std::current_exception();
 			next = nBatches;
 		}
+		tlsLane = nullptr;
 	};
 	std::vector<std::thread> th;
 	for (int li = 1; li < nl; ++li) th.emplace_back(worker, li);
 	worker(0);
 	for (auto& t : th) t.join();
+	c->lanesRunning = 0;
 	for (auto& e : errs)
 		if (e) std::rethrow_exception(e);
 
@@ -1103,8 +1202,13 @@ void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<Join
 	c->stats.queries = n;
 	c->stats.ms_total = total.ms();
 	if (rowsNeeded) *rowsNeeded = written;
-	if (capacityShort) fail(FLYEHIP_ERR_CAPACITY, "row capacity %lld too small: %lld rows needed",
-							static_cast<long long>(rowCap), static_cast<long long>(written));
+	// div_offsets[n] already holds the divStats values needed
+	const bool divShort = divStats && divWritten > divCap;
+	if (capacityShort || divShort)
+		fail(FLYEHIP_ERR_CAPACITY, "capacity too small: %lld rows needed (capacity %lld), %lld divStats values needed "
+				 "(capacity %lld)",
+			 static_cast<long long>(written), static_cast<long long>(rowCap), static_cast<long long>(divWritten),
+			 static_cast<long long>(divCap));
 }
 
 std::vector<JoinQuery> residentQueries(flyehip_ctx* c, const int64_t* ids, int32_t n)
@@ -1292,7 +1396,13 @@ int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehi
 		uint64_t* ord = ensureT<uint64_t>(c, L->bMKey, static_cast<size_t>(nEntries) + 1);
 		ensure(c, c->dEntries, static_cast<size_t>(nEntries) * 8 + 8);
 		HIPCHECK(launchFill(rs, dWinOff, nWin, k, t, ord, L->stream));
-		HIPCHECK(launchBucketFinish(t, ord, static_cast<uint64_t*>(c->dEntries.p), static_cast<const int32_t*>(c->dReadLen.p), k, L->stream));
+		// buckets above 32 entries (at most nEntries / 33 of them) are sorted
+		// by a block each; counters[2] counts them (zeroed above)
+		const uint64_t bigCap = static_cast<uint64_t>(nEntries) / 33 + 1;
+		uint32_t* bigList = ensureT<uint32_t>(c, L->bCVal, static_cast<size_t>(bigCap));
+		HIPCHECK(launchBucketFinish(t, ord, static_cast<uint64_t*>(c->dEntries.p),
+									static_cast<const int32_t*>(c->dReadLen.p), k, bigList, counters + 2, bigCap,
+									L->stream));
 		sync(L);
 		c->info = flyehip_index_info_t{};
 		c->info.kmer_size = k;

@@ -197,11 +197,12 @@ class FlyeHip:
 
     # -- overlaps ---------------------------------------------------------------
     def get_seq_overlaps(self, query_ids, config: OverlapConfig, with_div_stats: bool = False,
-                         per_query: bool = False, row_capacity: int = 0):
+                         per_query: bool = False, row_capacity: int = 0, div_capacity: int = 0):
         """getSeqOverlaps for each query record: (rows, row_offsets[, divs, div_offsets])."""
         q = np.ascontiguousarray(query_ids, dtype=np.int64)
         p = config.to_c()
         cap = row_capacity or max(1024, 64 * len(q))
+        dcap = div_capacity or 64 * len(q)
         need = C.c_int64(0)
         roff = np.zeros(len(q) + 1, np.int64)
         while True:
@@ -211,20 +212,20 @@ class FlyeHip:
                                                          N.ptr(rows), cap, N.ptr(roff), C.byref(need))
                 divs = doff = None
             else:
-                dcap = int(sum(1 for _ in q)) * 64
                 divs = np.zeros(dcap, np.float32) if with_div_stats else None
                 doff = np.zeros(len(q) + 1, np.int64) if with_div_stats else None
                 rc = self.lib.flyehip_overlaps(self.ctx, N.ptr(q), len(q), C.byref(p), N.ptr(rows), cap,
                                                N.ptr(roff), C.byref(need), N.ptr(divs), dcap if with_div_stats else 0,
                                                N.ptr(doff))
             if rc == N.FLYEHIP_ERR_CAPACITY:
-                cap = need.value
+                # rows_needed and div_offsets[n] hold what the call needs
+                cap = max(cap, need.value)
+                if doff is not None:
+                    dcap = max(dcap, int(doff[-1]))
                 continue
             self._check(rc)
             rows = rows[:need.value]
             if with_div_stats and not per_query:
-                if doff[-1] > len(divs):
-                    raise FlyeHipError(N.FLYEHIP_ERR_CAPACITY, "divStats capacity too small")
                 return rows, roff, divs[:doff[-1]], doff
             return rows, roff
 

@@ -264,7 +264,11 @@ int flyehip_collect_sorted_candidates(flyehip_ctx* ctx, const int64_t* query_ids
  * and writes each query's detectedOverlaps in Flye's order.  row_offsets has
  * n_queries + 1 entries.  div_stats (optional, may be NULL) receives, per
  * query, the divergences OvlpDivStats::add would get from its 10 kb windows
- * (patch 0023:245-269), div_offsets delimiting them. */
+ * (patch 0023:245-269), div_offsets delimiting them.  When row_capacity or
+ * div_capacity is too small the call returns FLYEHIP_ERR_CAPACITY with
+ * *rows_needed and div_offsets[n_queries] set to the counts it needs (a query
+ * of length L has at most L / 10000 + 1 values); a second call with those
+ * capacities succeeds. */
 int flyehip_overlaps(flyehip_ctx* ctx, const int64_t* query_ids, int32_t n_queries,
                      const flyehip_overlap_params_t* params,
                      flyehip_overlap_t* rows, int64_t row_capacity,

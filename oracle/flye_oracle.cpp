@@ -323,6 +323,53 @@ struct CE
 	int32_t ext;
 };
 
+// The chaining DP of one group (tools/replay_overlap_chains.py:280-313) over
+// `hits` in their final order.  earlyBreak = false drops the break after an
+// improvement on an exact diagonal step shorter than k: only the tests use
+// that form, to show which inputs the rule changes.
+void chainDp(const std::vector<CE>& hits, const flyehip_overlap_params_t& p, bool extSorted,
+			 bool earlyBreak, std::vector<int32_t>& score, std::vector<int32_t>& bt)
+{
+	const int32_t n = static_cast<int32_t>(hits.size());
+	const int32_t k = p.kmer_size;
+	score.assign(static_cast<size_t>(n), 0);
+	bt.assign(static_cast<size_t>(n), -1);
+	for (int32_t i = 1; i < n; ++i)
+	{
+		int32_t maxScore = 0;
+		int32_t maxId = 0;
+		const int32_t curNext = hits[i].cur;
+		const int32_t extNext = hits[i].ext;
+		for (int32_t j = i - 1; j >= 0; --j)
+		{
+			const int32_t curPrev = hits[j].cur;
+			const int32_t extPrev = hits[j].ext;
+			const int32_t dc = curNext - curPrev;
+			const int32_t de = extNext - extPrev;
+			const int32_t jumpDiv = std::abs(dc - de);
+			if (0 < dc && dc < p.maximum_jump && 0 < de && de < p.maximum_jump &&
+				jumpDiv <= p.max_jump_gap)
+			{
+				const int32_t matchScore = std::min(std::min(dc, de), k);
+				const float pen = jumpDiv > p.gap_jump_threshold ? p.large_gap_penalty
+																 : p.small_gap_penalty;
+				const int32_t gapCost = static_cast<int32_t>(pen * static_cast<float>(jumpDiv));
+				const int32_t next = score[j] + matchScore - gapCost;
+				if (next > maxScore)
+				{
+					maxScore = next;
+					maxId = j;
+					if (earlyBreak && jumpDiv == 0 && dc < k) break;
+				}
+			}
+			if (extSorted && de > p.maximum_jump) break;
+			if (!extSorted && dc > p.maximum_jump) break;
+		}
+		score[i] = std::max(maxScore, k);
+		if (maxScore > k) bt[i] = maxId;
+	}
+}
+
 // One (query, target) group: tools/replay_overlap_chains.py:228-380.
 // `hits` is the group in (extId, curPos) order; `filtered` ascending.
 void chainGroup(std::vector<CE>& hits, int64_t curId, int32_t curLen, int64_t extId,
@@ -365,42 +412,8 @@ void chainGroup(std::vector<CE>& hits, int64_t curId, int32_t curLen, int64_t ex
 				  [](const CE& a, const CE& b) { return a.ext < b.ext; });
 	}
 
-	std::vector<int32_t> score(static_cast<size_t>(n), 0);
-	std::vector<int32_t> bt(static_cast<size_t>(n), -1);
-	for (int32_t i = 1; i < n; ++i)
-	{
-		int32_t maxScore = 0;
-		int32_t maxId = 0;
-		const int32_t curNext = hits[i].cur;
-		const int32_t extNext = hits[i].ext;
-		for (int32_t j = i - 1; j >= 0; --j)
-		{
-			const int32_t curPrev = hits[j].cur;
-			const int32_t extPrev = hits[j].ext;
-			const int32_t dc = curNext - curPrev;
-			const int32_t de = extNext - extPrev;
-			const int32_t jumpDiv = std::abs(dc - de);
-			if (0 < dc && dc < p.maximum_jump && 0 < de && de < p.maximum_jump &&
-				jumpDiv <= p.max_jump_gap)
-			{
-				const int32_t matchScore = std::min(std::min(dc, de), k);
-				const float pen = jumpDiv > p.gap_jump_threshold ? p.large_gap_penalty
-																 : p.small_gap_penalty;
-				const int32_t gapCost = static_cast<int32_t>(pen * static_cast<float>(jumpDiv));
-				const int32_t next = score[j] + matchScore - gapCost;
-				if (next > maxScore)
-				{
-					maxScore = next;
-					maxId = j;
-					if (jumpDiv == 0 && dc < k) break;
-				}
-			}
-			if (extSorted && de > p.maximum_jump) break;
-			if (!extSorted && dc > p.maximum_jump) break;
-		}
-		score[i] = std::max(maxScore, k);
-		if (maxScore > k) bt[i] = maxId;
-	}
+	std::vector<int32_t> score, bt;
+	chainDp(hits, p, extSorted, true, score, bt);
 
 	std::vector<int32_t> order(static_cast<size_t>(n));
 	for (int32_t i = 0; i < n; ++i) order[i] = i;
@@ -901,5 +914,101 @@ void orc_std_sort_perm(const uint64_t* keys, int64_t n, int32_t* perm)
 
 // glibc logf, the function Flye's divergence calls.
 float orc_logf(float x) { return logf(x); }
+
+// ---- pieces of one group, exported for the tests that pin them ------------
+
+// The DP table of one group over hits in their final order (no sort).
+void orc_chain_dp(const int32_t* cur, const int32_t* ext, int32_t n,
+				  const flyehip_overlap_params_t* params, int32_t ext_sorted, int32_t early_break,
+				  int32_t* score, int32_t* bt)
+{
+	std::vector<CE> hits(static_cast<size_t>(n));
+	for (int32_t i = 0; i < n; ++i) hits[static_cast<size_t>(i)] = {cur[i], ext[i]};
+	std::vector<int32_t> s, b;
+	chainDp(hits, *params, ext_sorted != 0, early_break != 0, s, b);
+	for (int32_t i = 0; i < n; ++i)
+	{
+		score[i] = s[static_cast<size_t>(i)];
+		bt[i] = b[static_cast<size_t>(i)];
+	}
+}
+
+// One (query, target) group with the query's filtered positions (ascending):
+// chainGroup as getSeqOverlaps runs it.  Returns the row count; writes at
+// most cap rows.
+int64_t orc_chain_group(const int32_t* cur, const int32_t* ext, int32_t n, int64_t cur_id,
+						int32_t cur_len, int64_t ext_id, int32_t ext_len, const int32_t* filtered,
+						int32_t n_filtered, const flyehip_overlap_params_t* params,
+						flyehip_overlap_t* rows, int64_t cap)
+{
+	if (n <= 0) return 0;
+	std::vector<CE> hits(static_cast<size_t>(n));
+	for (int32_t i = 0; i < n; ++i) hits[static_cast<size_t>(i)] = {cur[i], ext[i]};
+	std::vector<int32_t> filt(filtered, filtered + n_filtered);
+	std::vector<Ovlp> det;
+	chainGroup(hits, cur_id, cur_len, ext_id, ext_len, filt, *params, det);
+	for (size_t i = 0; i < det.size() && static_cast<int64_t>(i) < cap; ++i) toRow(det[i], rows[i]);
+	return static_cast<int64_t>(det.size());
+}
+
+// overlapTest of one proposal (cur/ext ids, ranges and lengths from the row).
+int32_t orc_overlap_test(const flyehip_overlap_t* row, const flyehip_overlap_params_t* params)
+{
+	Ovlp o{};
+	o.curId = row->cur_id;
+	o.extId = row->ext_id;
+	o.curBegin = row->cur_begin;
+	o.curEnd = row->cur_end;
+	o.curLen = row->cur_len;
+	o.extBegin = row->ext_begin;
+	o.extEnd = row->ext_end;
+	o.extLen = row->ext_len;
+	return overlapTest(o, *params) ? 1 : 0;
+}
+
+// divStats windows of one query's detected overlaps, in row order.
+int64_t orc_div_stats(const flyehip_overlap_t* rows, int64_t n, int32_t cur_len, float* out,
+					  int64_t cap)
+{
+	std::vector<Ovlp> ovs(static_cast<size_t>(n));
+	for (int64_t i = 0; i < n; ++i)
+	{
+		const flyehip_overlap_t& r = rows[i];
+		Ovlp& o = ovs[static_cast<size_t>(i)];
+		o.curId = r.cur_id;
+		o.extId = r.ext_id;
+		o.curBegin = r.cur_begin;
+		o.curEnd = r.cur_end;
+		o.curLen = r.cur_len;
+		o.extBegin = r.ext_begin;
+		o.extEnd = r.ext_end;
+		o.extLen = r.ext_len;
+		o.score = r.score;
+		o.div = r.seq_divergence;
+		o.chainLength = r.chain_length;
+	}
+	std::vector<float> d;
+	divStatsOf(ovs, cur_len, d);
+	for (size_t i = 0; i < d.size() && static_cast<int64_t>(i) < cap; ++i) out[i] = d[i];
+	return static_cast<int64_t>(d.size());
+}
+
+// The index tables after orc_build_index: counts[0..2] = indexed k-mers,
+// bucket entries, repetitive k-mers; each array is written when its pointer is
+// not NULL (kmers ascending, start = bucket offsets, entry ids / positions in
+// bucket order, repetitive k-mers ascending).
+void orc_index_tables(void* h, int64_t* counts, uint64_t* kmers, int64_t* start, uint32_t* ids,
+					  int32_t* pos, uint64_t* repetitive)
+{
+	const Index& ix = *static_cast<Oracle*>(h)->index;
+	counts[0] = static_cast<int64_t>(ix.kmers.size());
+	counts[1] = ix.start.empty() ? 0 : ix.start.back();
+	counts[2] = static_cast<int64_t>(ix.repetitive.size());
+	if (kmers) std::copy(ix.kmers.begin(), ix.kmers.end(), kmers);
+	if (start) std::copy(ix.start.begin(), ix.start.end(), start);
+	if (ids) std::copy(ix.entryId.begin(), ix.entryId.end(), ids);
+	if (pos) std::copy(ix.entryPos.begin(), ix.entryPos.end(), pos);
+	if (repetitive) std::copy(ix.repetitive.begin(), ix.repetitive.end(), repetitive);
+}
 
 } // extern "C"

@@ -78,6 +78,16 @@ def lib() -> C.CDLL:
     L.orc_std_sort_perm.argtypes = [P, i64, P]
     L.orc_logf.argtypes = [f32]
     L.orc_logf.restype = f32
+    L.orc_chain_dp.argtypes = [P, P, i32, P, i32, i32, P, P]
+    L.orc_chain_dp.restype = None
+    L.orc_chain_group.argtypes = [P, P, i32, i64, i32, i64, i32, P, i32, P, P, i64]
+    L.orc_chain_group.restype = i64
+    L.orc_overlap_test.argtypes = [P, P]
+    L.orc_overlap_test.restype = i32
+    L.orc_div_stats.argtypes = [P, i64, i32, P, i64]
+    L.orc_div_stats.restype = i64
+    L.orc_index_tables.argtypes = [P, P, P, P, P, P, P]
+    L.orc_index_tables.restype = None
     _lib = L
     return L
 
@@ -124,6 +134,58 @@ def logf(x: float) -> float:
     return lib().orc_logf(x)
 
 
+def _params(params):
+    return params.to_c() if hasattr(params, "to_c") else params
+
+
+def chain_dp(cur, ext, params, ext_sorted: bool = False, early_break: bool = True):
+    """Score and backtrack tables of one group's DP, hits in their final order."""
+    c = np.ascontiguousarray(cur, np.int32)
+    e = np.ascontiguousarray(ext, np.int32)
+    score = np.zeros(len(c), np.int32)
+    bt = np.zeros(len(c), np.int32)
+    p = _params(params)
+    lib().orc_chain_dp(_p(c), _p(e), len(c), C.byref(p), int(ext_sorted), int(early_break), _p(score), _p(bt))
+    return score, bt
+
+
+def chain_group(cur, ext, cur_id: int, cur_len: int, ext_id: int, ext_len: int, filtered, params):
+    """detectedOverlaps of one (query, target) group, hits in (extId, curPos) order."""
+    N = _dt()
+    c = np.ascontiguousarray(cur, np.int32)
+    e = np.ascontiguousarray(ext, np.int32)
+    f = np.ascontiguousarray(filtered if filtered is not None else [], np.int32)
+    p = _params(params)
+    cap = 16
+    while True:
+        rows = np.zeros(cap, N.OVERLAP_DTYPE)
+        n = lib().orc_chain_group(_p(c), _p(e), len(c), cur_id, cur_len, ext_id, ext_len, _p(f), len(f),
+                                  C.byref(p), _p(rows), cap)
+        if n <= cap:
+            return rows[:n]
+        cap = n
+
+
+def overlap_test(fields: dict, params) -> bool:
+    """Flye's overlapTest of one proposal; `fields` names OVERLAP_DTYPE fields
+    (cur_id, cur_begin, cur_end, cur_len, ext_id, ext_begin, ext_end, ext_len)."""
+    N = _dt()
+    r = np.zeros(1, N.OVERLAP_DTYPE)
+    for name, value in fields.items():
+        r[0][name] = value
+    p = _params(params)
+    return bool(lib().orc_overlap_test(_p(r), C.byref(p)))
+
+
+def div_stats(rows, cur_len: int) -> np.ndarray:
+    """OvlpDivStats values of one query's detected overlaps (rows in order)."""
+    N = _dt()
+    r = np.ascontiguousarray(rows, N.OVERLAP_DTYPE)
+    out = np.zeros(cur_len // 10000 + 2, np.float32)
+    n = lib().orc_div_stats(_p(r), len(r), cur_len, _p(out), len(out))
+    return out[:n]
+
+
 class Oracle:
     """Read store + VertexIndex + getSeqOverlaps restated on the CPU."""
 
@@ -147,6 +209,19 @@ class Oracle:
         self.info = dict(solid_min=int(out[0]), rep_max=int(out[1]), distinct=int(out[2]),
                          solid=int(out[3]), repetitive=int(out[4]), positions=int(out[5]))
         return self.info
+
+    def index_tables(self) -> dict:
+        """kmers (ascending), bucket starts, entry ids / positions in bucket
+        order and the repetitive k-mers of the built index."""
+        counts = np.zeros(3, np.int64)
+        lib().orc_index_tables(self.h, _p(counts), None, None, None, None, None)
+        kmers = np.zeros(counts[0], np.uint64)
+        start = np.zeros(counts[0] + 1, np.int64)
+        ids = np.zeros(counts[1], np.uint32)
+        pos = np.zeros(counts[1], np.int32)
+        rep = np.zeros(counts[2], np.uint64)
+        lib().orc_index_tables(self.h, _p(counts), _p(kmers), _p(start), _p(ids), _p(pos), _p(rep))
+        return dict(kmers=kmers, start=start, ids=ids, pos=pos, repetitive=rep)
 
     def histogram(self) -> np.ndarray:
         n = lib().orc_histogram(self.h, None, 0)

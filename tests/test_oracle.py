@@ -135,24 +135,6 @@ def test_candidates_brute_force(oracle_mod):
     assert info["positions"] == sum(len(r) - k + 1 for r in reads)
 
 
-def test_index_thresholds_arithmetic(oracle_mod):
-    """solid_min = first c >= 2 with hist[c] <= hist[c+1]; rep_max =
-    floor(rate * mean count over solid k-mers) (definition in include/flyehip.h)."""
-    O = oracle_mod
-    import cuflye_amd.synth as S
-    r = S.config1(scale=0.1, seed=11)
-    orc = O.Oracle(r.bases, r.offsets)
-    info = orc.build_index(15)
-    h = orc.histogram()
-    c = next(c for c in range(2, len(h) - 1) if h[c] <= h[c + 1])
-    assert info["solid_min"] == c
-    cs = np.arange(len(h))
-    sel = cs >= c
-    mean = float((cs[sel] * h[sel]).sum()) / (float(h[sel].sum()) + 1.0)
-    assert info["rep_max"] == max(int(np.floor(np.float64(np.float32(10.0)) * mean)), c)
-    assert info["solid"] + info["repetitive"] == int(h[c:].sum())
-
-
 def _params(**kw):
     return OverlapConfig(**kw)
 
@@ -179,10 +161,12 @@ def test_chain_closed_form_single_diagonal(oracle_mod):
     assert r["seq_divergence"] == div
 
 
-def test_chain_early_break_and_gap_penalty(oracle_mod):
-    """Hits at 1 bp spacing on one diagonal break right after the first
-    predecessor (jumpDiv == 0 and dc < k); an off-diagonal hit with jumpDiv
-    120 costs int(2.0 * 120) = 240 and is never chosen."""
+def test_chain_unit_spacing_diagonal(oracle_mod):
+    """Hits at 1 bp spacing on one exact diagonal: every step gains
+    min(1, 1, 15) = 1 and the scan stops right after its first predecessor
+    (jumpDiv == 0 and dc < k).  The gap tiers, the max_jump_gap reject and the
+    early break's effect are pinned in test_oracle_closed_form.py and by the
+    reference vectors."""
     O = oracle_mod
     cur = np.array(list(range(0, 2000)) , np.int32)
     ext = cur.copy()
