@@ -197,7 +197,25 @@ __global__ void readAlignWriteKernel(ReadAlignArgs a, const int64_t* segOff, fly
 	}
 }
 
+// Compacts each read's kept alignment indices (sorted order) to
+// out[keptOff[r] ..), keptOff being the exclusive scan of keptCount.
+__global__ void readAlignKeptKernel(ReadAlignArgs a, const int64_t* keptOff, int32_t* out)
+{
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= a.nReads) return;
+	const int32_t* kept = a.kept + a.alnOff[r];
+	const int64_t o = keptOff[r];
+	for (int i = 0; i < a.keptCount[r]; ++i) out[o + i] = kept[i];
+}
+
 } // namespace
+
+hipError_t launchReadAlignKept(const ReadAlignArgs& a, const int64_t* keptOff, int32_t* out, hipStream_t s)
+{
+	if (a.nReads == 0) return hipSuccess;
+	hipLaunchKernelGGL(readAlignKeptKernel, dim3((a.nReads + 63) / 64), dim3(64), 0, s, a, keptOff, out);
+	return hipGetLastError();
+}
 
 hipError_t launchReadAlignChain(const ReadAlignArgs& a, hipStream_t s)
 {

@@ -36,5 +36,7 @@ struct ReadAlignArgs
 hipError_t launchReadAlignChain(const ReadAlignArgs& a, hipStream_t s);
 hipError_t launchReadAlignWrite(const ReadAlignArgs& a, const int64_t* segOff, flyehip_chain_segment_t* out,
 								hipStream_t s);
+// out[keptOff[r] + i] = kept index i of read r (keptOff: exclusive scan of keptCount).
+hipError_t launchReadAlignKept(const ReadAlignArgs& a, const int64_t* keptOff, int32_t* out, hipStream_t s);
 
 } // namespace fh

@@ -37,6 +37,31 @@
 extern "C" const unsigned char fh_logf_table[];
 extern "C" const unsigned char fh_logf_table_end[];
 
+// ---- no-allocation scratch ------------------------------------------------
+// flyehip_read_align_chains_noalloc keeps every intermediate buffer in one
+// file-scope device array, so a call allocates nothing.  Limits: 2^20
+// alignments (aln_offsets[n_reads]) and 2^17 reads per call; the array
+// (about 43 MiB) is held on every device the library runs on until the
+// process ends.  All calls on one device share it: calls that overlap on one
+// device race.  Every word a call reads is written by the same call first
+// (per-read counts, scans, per-alignment chain state), so nothing needs
+// clearing between calls.  Host code never passes `scratch` itself to a
+// kernel: it uses the device address hipGetSymbolAddress returns for the
+// current device (scratchOn).
+namespace fh
+{
+constexpr int64_t kNoAllocMaxAlignments = int64_t{1} << 20;
+constexpr int32_t kNoAllocMaxReads = 1 << 17;
+constexpr size_t na256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+constexpr size_t kNaI32 = na256(static_cast<size_t>(kNoAllocMaxAlignments) * 4);    // one int32 per alignment
+constexpr size_t kNaU64 = na256(static_cast<size_t>(kNoAllocMaxAlignments) * 8);    // sort keys
+constexpr size_t kNaRead = na256((static_cast<size_t>(kNoAllocMaxReads) + 1) * 4);  // one int32 per read
+constexpr size_t kNaScan = na256((static_cast<size_t>(kNoAllocMaxReads) / 4096 + 2 + 64) * 8);
+constexpr size_t kNoAllocScratchBytes = kNaU64 + 8 * kNaI32 + 3 * kNaRead + kNaScan;
+} // namespace fh
+
+static __device__ __align__(256) unsigned char scratch[fh::kNoAllocScratchBytes];
+
 namespace fh
 {
 namespace
@@ -197,6 +222,9 @@ struct flyehip_ctx
 	// logf table
 	fh::DevBuf dLogX, dLogY;
 	int logN = 0;
+
+	// device address of this device's no-allocation scratch array
+	unsigned char* scratch = nullptr;
 
 	flyehip_stats_t stats{};
 };
@@ -1211,6 +1239,26 @@ void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<Join
 			 static_cast<long long>(divCap));
 }
 
+// ---- no-allocation read-to-graph chaining (the scratch array is at the top) ----
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMu;
+unsigned char* gScratch[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current on the calling thread),
+// looked up once per device.
+unsigned char* scratchOn(int device)
+{
+	if (device < 0 || device >= kMaxDevices) fail(FLYEHIP_ERR_ARG, "device %d beyond %d", device, kMaxDevices);
+	std::lock_guard<std::mutex> lock(gScratchMu);
+	if (!gScratch[device])
+	{
+		void* p = nullptr;
+		HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+		gScratch[device] = static_cast<unsigned char*>(p);
+	}
+	return gScratch[device];
+}
+
 std::vector<JoinQuery> residentQueries(flyehip_ctx* c, const int64_t* ids, int32_t n)
 {
 	if (!c->haveIndex) fail(FLYEHIP_ERR_STATE, "no index: call flyehip_store_reads and flyehip_build_index first");
@@ -1258,6 +1306,7 @@ int flyehip_open(int device, uint64_t memory_budget_bytes, flyehip_ctx** out_ctx
 		size_t freeB = 0, totalB = 0;
 		HIPCHECK(hipMemGetInfo(&freeB, &totalB));
 		c->budget = memory_budget_bytes ? memory_budget_bytes : static_cast<uint64_t>(static_cast<double>(freeB) * 0.9);
+		c->scratch = scratchOn(device);
 		createLane(c);
 		loadLogTable(c);
 	}
@@ -1716,6 +1765,81 @@ int flyehip_read_align_chains(flyehip_ctx* c, const flyehip_edge_alignment_t* al
 		c->stats.matches = m;
 		c->stats.overlaps = nSeg;
 		if (nSeg > segment_cap) fail(FLYEHIP_ERR_CAPACITY, "segment capacity %lld too small: %lld needed", static_cast<long long>(segment_cap), static_cast<long long>(nSeg));
+	});
+}
+
+int flyehip_read_align_chains_noalloc(flyehip_ctx* c, const flyehip_edge_alignment_t* d_alignments,
+									  const int64_t* d_aln_offsets, int32_t n_reads, int32_t long_edge,
+									  int32_t big_aln, const flyehip_chain_params_t* params, int32_t* d_sorted_index,
+									  int64_t* d_kept_offsets, flyehip_chain_segment_t* d_segments,
+									  int64_t segment_cap, int64_t* d_segment_offsets, int64_t* segments_needed)
+{
+	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
+		if (!params) fail(FLYEHIP_ERR_ARG, "null chain parameters");
+		if (n_reads < 0 || n_reads > kNoAllocMaxReads)
+			fail(FLYEHIP_ERR_ARG, "%d reads exceed the no-allocation limit of %d reads per call", n_reads,
+				 kNoAllocMaxReads);
+		if (n_reads > 0 && (!d_alignments || !d_aln_offsets || !d_sorted_index || !d_kept_offsets ||
+							!d_segment_offsets || (segment_cap > 0 && !d_segments)))
+			fail(FLYEHIP_ERR_ARG, "null device buffer");
+		if (!c->scratch) fail(FLYEHIP_ERR_STATE, "no-allocation scratch not set up");
+		resetStats(c);
+		Timer tc;
+		int64_t total = 0;
+		if (n_reads > 0)
+		{
+			download(L, &total, d_aln_offsets + n_reads, sizeof(int64_t));
+			sync(L);
+		}
+		if (total < 0 || total > kNoAllocMaxAlignments)
+			fail(FLYEHIP_ERR_ARG, "aln_offsets[n_reads] = %lld exceeds the no-allocation limit of %lld alignments",
+				 static_cast<long long>(total), static_cast<long long>(kNoAllocMaxAlignments));
+		// carve the scratch array (offsets fixed at compile time)
+		unsigned char* s = c->scratch;
+		auto take = [&s](size_t bytes) {
+			unsigned char* p = s;
+			s += bytes;
+			return p;
+		};
+		ReadAlignArgs ra{};
+		ra.aln = d_alignments;
+		ra.alnOff = d_aln_offsets;
+		ra.nReads = n_reads;
+		ra.longEdge = long_edge;
+		ra.bigAln = big_aln;
+		ra.p = *params;
+		ra.key = reinterpret_cast<uint64_t*>(take(kNaU64));
+		ra.val = reinterpret_cast<uint32_t*>(take(kNaI32));
+		ra.kept = reinterpret_cast<int32_t*>(take(kNaI32));
+		ra.chainScore = reinterpret_cast<int32_t*>(take(kNaI32));
+		ra.chainPrev = reinterpret_cast<int32_t*>(take(kNaI32));
+		ra.chainLen = reinterpret_cast<int32_t*>(take(kNaI32));
+		ra.active = reinterpret_cast<int32_t*>(take(kNaI32));
+		ra.frozen = reinterpret_cast<int32_t*>(take(kNaI32));
+		ra.accepted = reinterpret_cast<int32_t*>(take(kNaI32));
+		ra.keptCount = reinterpret_cast<int32_t*>(take(kNaRead));
+		ra.acceptedCount = reinterpret_cast<int32_t*>(take(kNaRead));
+		ra.segCount = reinterpret_cast<int32_t*>(take(kNaRead));
+		int64_t* tmp = reinterpret_cast<int64_t*>(take(kNaScan));
+		if (s != c->scratch + kNoAllocScratchBytes) fail(FLYEHIP_ERR_STATE, "scratch layout mismatch");
+		HIPCHECK(launchReadAlignChain(ra, L->stream));
+		HIPCHECK(exclusiveScan(ra.keptCount, d_kept_offsets, n_reads, tmp, L->stream));
+		HIPCHECK(launchReadAlignKept(ra, d_kept_offsets, d_sorted_index, L->stream));
+		HIPCHECK(exclusiveScan(ra.segCount, d_segment_offsets, n_reads, tmp, L->stream));
+		int64_t nSeg = 0;
+		download(L, &nSeg, d_segment_offsets + n_reads, sizeof(int64_t));
+		sync(L);
+		if (segments_needed) *segments_needed = nSeg;
+		if (nSeg > segment_cap)
+			fail(FLYEHIP_ERR_CAPACITY, "segment capacity %lld too small: %lld needed",
+				 static_cast<long long>(segment_cap), static_cast<long long>(nSeg));
+		HIPCHECK(launchReadAlignWrite(ra, d_segment_offsets, d_segments, L->stream));
+		sync(L);
+		c->stats.ms_chain = tc.ms();
+		c->stats.queries = n_reads;
+		c->stats.matches = total;
+		c->stats.overlaps = nSeg;
 	});
 }
 

@@ -30,6 +30,7 @@ EXPORTED = (
     "flyehip_collect_sorted_candidates", "flyehip_overlaps",
     "flyehip_overlaps_external", "flyehip_overlaps_per_query",
     "flyehip_chain_lists", "flyehip_read_align_chains", "flyehip_divergence_logf",
+    "flyehip_read_align_chains_noalloc",
 )
 
 
@@ -141,6 +142,8 @@ def load_flyehip(path: Path | None = None) -> C.CDLL:
         "flyehip_read_align_chains": ([_P, _P, _P, i32, i32, i32, C.POINTER(ChainParams), _P, _P, _P,
                                        i64, _P, _P], C.c_int),
         "flyehip_divergence_logf": ([_P, _P, i64, _P], C.c_int),
+        "flyehip_read_align_chains_noalloc": ([_P, _P, _P, i32, i32, i32, C.POINTER(ChainParams), _P, _P,
+                                               _P, i64, _P, _P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -331,6 +331,27 @@ int flyehip_read_align_chains(flyehip_ctx* ctx,
                               int64_t segment_capacity, int64_t* segment_offsets,
                               int64_t* segments_needed);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): flyehip_read_align_chains with every array in device
+ * memory the caller owns - d_alignments / d_aln_offsets in, d_sorted_index
+ * (capacity aln_offsets[n_reads]), d_kept_offsets and d_segment_offsets
+ * (n_reads + 1 each) and d_segments out - and every intermediate buffer in
+ * one fixed device array the library holds per device (about 43 MiB), so the
+ * call allocates nothing.  Limits: n_reads <= 131072 and
+ * aln_offsets[n_reads] <= 1048576 (FLYEHIP_ERR_ARG above them).  Calls that
+ * overlap on one device share that array and race; serialise them.  Results
+ * equal flyehip_read_align_chains'. */
+int flyehip_read_align_chains_noalloc(flyehip_ctx* ctx,
+                                      const flyehip_edge_alignment_t* d_alignments,
+                                      const int64_t* d_aln_offsets, int32_t n_reads,
+                                      int32_t long_edge, int32_t big_aln,
+                                      const flyehip_chain_params_t* params,
+                                      int32_t* d_sorted_index, int64_t* d_kept_offsets,
+                                      flyehip_chain_segment_t* d_segments,
+                                      int64_t segment_capacity, int64_t* d_segment_offsets,
+                                      int64_t* segments_needed);
+
 /* ---- the divergence logarithm --------------------------------------------- */
 
 /* out[i] = logf(x[i]) exactly as glibc's logf returns it, computed on the GPU
