@@ -35,17 +35,8 @@ sys.path.insert(0, str(ROOT))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HIP kernels behind each kernel family of the library's statistics
-KERNEL_NAMES = {
-    "join": "joinCountKernel + joinFillKernel",
-    "radix": "iotaKernel + rocprim segmented radix sort + rankKeysKernel",
-    "leaf": "segLeafKernel",
-    "heap": "segHeapLdsKernel (+ segHeapGlobalKernel)",
-    "prefilter": "chainPrefilterKernel",
-    "dp": "chainDpKernel<G>",
-    "walk": "chainWalkKernel<32768/16384/8192/2048, onlyMaxExt>",
-    "rounds": "segMedian/Map/Count/Rank/Swap/SplitKernel + scans",
-}
+# 32-bit integer vector operations per second: 256 CUs x 64 lanes x 2.4 GHz
+INT_VALU_PEAK = 256 * 64 * 2.4e9
 
 
 def pmc_traffic(family: str, tuning: dict):
@@ -63,6 +54,24 @@ def pmc_traffic(family: str, tuning: dict):
         return None
     f = d["families"].get(family)
     return f["hbm_bytes_per_launch"] if f else None
+
+
+def pmc_valu(kernel: str, cells: float, ms: float, launches: int):
+    """VALU instructions of a kernel from the committed rocprofv3 --pmc
+    SQ_INSTS_VALU run (profiles/pmc_valu.json): lane operations per DP cell
+    and the achieved fraction of the integer vector peak, scaled to this
+    run's launches; None without the profile."""
+    path = ROOT / "profiles" / "pmc_valu.json"
+    if not path.exists() or cells <= 0 or ms <= 0:
+        return None
+    d = json.loads(path.read_text()).get("kernels", {}).get(kernel)
+    if not d:
+        return None
+    insts = float(d["sq_insts_valu_per_launch"]) * launches
+    lane_ops = 64.0 * insts
+    return {"valu_insts": insts, "lane_ops_per_cell": lane_ops / cells,
+            "frac_of_int_valu_peak": lane_ops / (ms / 1000.0) / INT_VALU_PEAK,
+            "source": "profiles/pmc_valu.json"}
 
 
 def parse():
@@ -372,19 +381,37 @@ def main() -> int:
     ms = 1000.0 * float(np.mean(times))
     value = len(ids) / (ms / 1000.0)
 
-    # dominant kernel family of the timed step: GPU time from HIP events on
+    # dominant single kernel of the timed step: GPU time from HIP events on
     # the stream each launch ran on, algorithmic bytes from the per-unit
     # figures the library applies (documented with the roofline)
-    from cuflye_amd.native import KERNEL_FAMILIES
+    from cuflye_amd.native import KERNEL_FAMILIES, KERNEL_NAMES, SINGLE_KERNELS
     fam_ms = dict(zip(KERNEL_FAMILIES, stats["kernel_ms"]))
     fam_n = dict(zip(KERNEL_FAMILIES, stats["kernel_launches"]))
     fam_b = dict(zip(KERNEL_FAMILIES, stats["kernel_bytes"]))
-    dom = max(fam_ms, key=lambda f: fam_ms[f])
+    dom = max(SINGLE_KERNELS, key=lambda f: fam_ms[f])
     achieved = fam_b[dom] / (fam_ms[dom] / 1000.0) / 1e9 if fam_ms[dom] > 0 else 0.0
     kernels = {f: {"ms": fam_ms[f], "launches": fam_n[f], "avg_us": 1000.0 * fam_ms[f] / max(1, fam_n[f]),
                    "GBps": (fam_b[f] / (fam_ms[f] / 1000.0) / 1e9) if fam_ms[f] > 0 else 0.0}
-               for f in KERNEL_FAMILIES}
-    log(f"kernel families: {json.dumps(kernels)}")
+               for f in KERNEL_FAMILIES if fam_n[f] > 0}
+    log(f"kernels: {json.dumps(kernels)}")
+    # whole step on the per-unit bytes of the path: join L/4 + 8 W + 8 H + 16 H
+    # (2-bit query, one slot probe per window, bucket entry read, match
+    # written) and chain 8 n + 8 n + 40 r (hits read, score and backtrack
+    # written - they leave LDS here - and overlap rows written), n = H
+    lens = np.diff(reads.offsets)
+    q_bases = float(lens[np.abs(ids) - 1].sum())
+    W, H, R = float(stats["windows"]), float(stats["matches"]), float(stats["overlaps"])
+    step_bytes = q_bases / 4.0 + 8.0 * W + 24.0 * H + 16.0 * H + 40.0 * R
+    end_to_end = {"algorithmic_bytes_per_step": step_bytes, "achieved": step_bytes / (ms / 1000.0) / 1e9,
+                  "frac": step_bytes / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
+                  "formula": "L/4 + 8 W + 24 H + 16 n + 40 r with n = H (join + chain, score/backtrack spilled)",
+                  "units": {"query_bases": q_bases, "windows": W, "hits": H, "rows": R}}
+    # chaining DP: (row, predecessor) pairs evaluated against the vector
+    # integer peak (256 CUs x 64 lanes x 2.4 GHz)
+    dp_cells = float(stats["dp_cells"])
+    dp = {"cells": dp_cells, "ms": fam_ms["dp"],
+          "cells_per_s": dp_cells / (fam_ms["dp"] / 1000.0) if fam_ms["dp"] > 0 else None,
+          "int_valu_peak_ops": INT_VALU_PEAK, "valu": pmc_valu("dp", dp_cells, fam_ms["dp"], fam_n["dp"])}
 
     # plain per-query launch path on a sample
     pq = ids[: min(args.per_query_sample, len(ids))]
@@ -460,7 +487,8 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(dom, run_tuning),
                      "kernel": KERNEL_NAMES[dom], "family": dom, "launches": fam_n[dom],
                      "avg_launch_us": kernels[dom]["avg_us"],
-                     "algorithmic_bytes_per_launch": fam_b[dom] / max(1, fam_n[dom])},
+                     "algorithmic_bytes_per_launch": fam_b[dom] / max(1, fam_n[dom]),
+                     "end_to_end": end_to_end, "dp": dp},
         "kernels": kernels,
         "cpu_baseline": cpu,
         "parity_sample": parity,
@@ -470,7 +498,8 @@ def main() -> int:
         "dp_stress": dp_stress,
         "read_to_graph": r2g,
         "stage_ms": {k: stats[k] for k in ("ms_join", "ms_sort", "ms_group", "ms_chain", "ms_extract", "ms_total")},
-        "counts": {k: stats[k] for k in ("windows", "matches", "groups", "overlaps", "batches", "tie_queries")},
+        "counts": {k: stats[k] for k in ("windows", "matches", "groups", "overlaps", "batches", "tie_queries",
+                                         "dp_cells")},
         "index_build_s": index_s,
     }
     # the metric is bit-exact output: a failed parity check withholds the value

@@ -471,6 +471,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 	int32_t winS = 0, winP = 0;
 	bool done = false;
 	int iter = 0;
+	uint32_t cells = 0;  // (row, predecessor) pairs this lane evaluated
 	for (;;)
 	{
 		// subgroups without a group pull the next passing one
@@ -529,6 +530,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			{
 				const int j = tileBase - sl;
 				const bool active = j >= 0;
+				cells += active ? 1u : 0u;
 				int cand = -0x7fffffff - 1;
 				bool special = false;
 				bool stop = false;
@@ -667,6 +669,11 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 		++iter;
 		if ((iter & 31) == 0) __threadfence_block();
 		__builtin_amdgcn_wave_barrier();
+	}
+	if (a.dpCells)
+	{
+		const long long total = waveSum64(static_cast<long long>(cells));
+		if (lane == 0) atomicAdd(a.dpCells, static_cast<unsigned long long>(total));
 	}
 }
 

@@ -75,6 +75,7 @@ struct ChainArgs
 	// glibc logf agreement table (sorted by argument bits)
 	const uint32_t* logX;
 	const uint32_t* logY;
+	unsigned long long* dpCells;  // adds the (row, predecessor) pairs the DP evaluates (may be null)
 };
 
 // Walk kernel variants by group size (powers of two, so each is a contiguous

@@ -13,7 +13,7 @@ namespace fh
 
 struct KTimer
 {
-	static constexpr int kMax = 256;
+	static constexpr int kMax = 2048;
 	hipEvent_t ev[2 * kMax] = {};
 	int family[kMax] = {};
 	double bytes[kMax] = {};

@@ -65,7 +65,14 @@ hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBit
 	hipError_t e = hipcub::DeviceSegmentedRadixSort::SortPairs(temp, tempBytes, a.akey, a.skey, a.aval, a.sval,
 															   static_cast<int>(nMatches), a.nQueries, a.qOff,
 															   a.qOff + 1, 32, 32 + idBits, s);
-	if (e != hipSuccess) return e;
+	(void)tpb;
+	return e;
+}
+
+hipError_t launchRankKeys(const MsortArgs& a, int64_t nMatches, hipStream_t s)
+{
+	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
+	const int tpb = 256;
 	hipLaunchKernelGGL(rankKeysKernel, gridFor(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.sval,
 					   a.rkey, a.dupFlag, nMatches);
 	return hipGetLastError();

@@ -567,12 +567,12 @@ BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& 
 	ja.matchOff = matchOff;
 	ja.filtOff = filtOff;
 	// join bytes: 2-bit read (L / 4 per query) + one 8-byte slot probe per
-	// window; the fill adds 8 bytes per bucket entry read and 12 per match
-	// written (u64 key, i32 extPos)
+	// window; the fill adds 8 bytes per bucket entry read and 16 per match
+	// written (u64 key, i32 extPos, u32 emission index)
 	double seqBytes = 0.0;
 	for (const auto& q : qs) seqBytes += static_cast<double>((q.len + 3) / 4);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
-	const int tj = L->timer.begin(FLYEHIP_K_JOIN, L->stream, seqBytes + 8.0 * static_cast<double>(nWin));
+	const int tj = L->timer.begin(FLYEHIP_K_JOIN_COUNT, L->stream, seqBytes + 8.0 * static_cast<double>(nWin));
 	HIPCHECK(launchJoinCount(ja, L->stream));
 	L->timer.end(tj, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[1], L->stream));
@@ -597,7 +597,7 @@ BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& 
 	ja.qMatchOff = qMatchOff;
 	ja.filtered = ensureT<int32_t>(c, L->bFiltered, static_cast<size_t>(bo.filtered) + 1);
 	HIPCHECK(hipEventRecord(L->ev[6], L->stream));
-	const int tf = L->timer.begin(FLYEHIP_K_JOIN, L->stream,
+	const int tf = L->timer.begin(FLYEHIP_K_JOIN_FILL, L->stream,
 								  seqBytes + 8.0 * static_cast<double>(nWin) + 24.0 * static_cast<double>(bo.matches));
 	HIPCHECK(launchJoinFill(ja, L->stream));
 	L->timer.end(tf, L->stream);
@@ -690,11 +690,15 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
 	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
-	// stable order: 2 radix passes x 24 (key, value read and written) + 20
-	// (rank keys and tie flags) bytes per match
-	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, 68.0 * static_cast<double>(nMatches));
+	// stable order: 2 radix passes x 24 bytes per match (key and value read
+	// and written); rank keys: sorted key and value read, rank key and tie
+	// flag written (20 bytes per match)
+	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, 48.0 * static_cast<double>(nMatches));
 	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, temp, tempBytes, L->stream));
 	L->timer.end(tr, L->stream);
+	const int tk = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 20.0 * static_cast<double>(nMatches));
+	HIPCHECK(launchRankKeys(ms, nMatches, L->stream));
+	L->timer.end(tk, L->stream);
 	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, L->stream));
 	HIPCHECK(segSortReset(sb, L->stream));
 	HIPCHECK(segPushOffsets(sb, ms.qOff, nq, L->stream));
@@ -727,7 +731,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	int64_t* nBig = ensureT<int64_t>(c, L->bCounters, 16);
 	zero32(L, bins, 96);
 	zero32(L, queues, 16);
-	HIPCHECK(hipMemsetAsync(nBig, 0, 8 * sizeof(int64_t), L->stream));
+	HIPCHECK(hipMemsetAsync(nBig, 0, 9 * sizeof(int64_t), L->stream));  // nBig[8]: DP cells
 	SegSortBufs sb = segBufs(L, nHits, ng);
 	sb.key = ca.sortKey;
 	sb.val = ca.sortVal;
@@ -806,6 +810,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	const double deferHits = static_cast<double>(split[3]);
 	ca.passOrder = readyOrder;
 	ca.nPass = nReady;
+	ca.dpCells = reinterpret_cast<unsigned long long*>(nBig + 8);
 	ca.queue = queues + 1;
 	HIPCHECK(launchChainGather(ca, blocks, L->stream));
 	HIPCHECK(hipEventRecord(L->ev[3], L->stream));
@@ -833,8 +838,11 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	HIPCHECK(segSortReset(sb, L->stream));
 	sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
 	ca.queue = queues + 3;
+	// score keys: score and backtrack read, key and value written (16 bytes per hit)
+	const int tsk = L->timer.begin(FLYEHIP_K_SCORE_KEYS, L->stream, 16.0 * static_cast<double>(nHits));
 	HIPCHECK(launchChainScoreKeys(ca, sb, ensureT<int64_t>(c, L->bGroupStart, static_cast<size_t>(ng) + 1), nHits,
 								  L->stream));
+	L->timer.end(tsk, L->stream);
 	runSegSort(L, sb, nullptr);
 	ca.queue = queues + 4;
 	// walk: score order and backtrack pointers read, 8 bytes per hit
@@ -849,8 +857,11 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	ca.queue = queues + 5;
 	HIPCHECK(launchChainWalk(ca, nBigPass, ca.nPass, 0, blocks, L->stream));
 	L->timer.end(tw, L->stream);
+	int64_t dpCells = 0;
+	download(L, &dpCells, nBig + 8, sizeof(int64_t));
 	HIPCHECK(hipEventRecord(L->ev[5], L->stream));
 	HIPCHECK(hipEventSynchronize(L->ev[5]));
+	L->stats.dp_cells += dpCells;
 	float a = 0, b = 0, d = 0;
 	HIPCHECK(hipEventElapsedTime(&a, L->ev[2], L->ev[3]));
 	HIPCHECK(hipEventElapsedTime(&b, L->ev[3], L->ev[4]));
@@ -876,8 +887,12 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nMatches) + 64);
 	const int64_t* qMatchOff = static_cast<const int64_t*>(L->bQMatchOff.p);
 	const uint64_t* skey = static_cast<const uint64_t*>(L->bSKey.p);
+	// gather: sorted key, permutation and extPos read, hits and group flags
+	// written (8 + 4 + 4 + 8 + 4 bytes per match)
+	const int tgk = L->timer.begin(FLYEHIP_K_GATHER, L->stream, 28.0 * static_cast<double>(nMatches));
 	HIPCHECK(launchGatherHits(qMatchOff, nq, skey, static_cast<const uint32_t*>(L->bVal.p), ja.mext, hcur, hext, gflag,
 							  nMatches, L->stream));
+	L->timer.end(tgk, L->stream);
 	HIPCHECK(exclusiveScan(gflag, goff, nMatches, tmp, L->stream));
 	int64_t nGroups = 0;
 	download(L, &nGroups, goff + nMatches, sizeof(int64_t));
@@ -981,7 +996,8 @@ void foldStats(flyehip_ctx* c)
 		c->stats.overlaps += s.overlaps;
 		c->stats.batches += s.batches;
 		c->stats.tie_queries += s.tie_queries;
-		for (int f = 0; f < 8; ++f)
+		c->stats.dp_cells += s.dp_cells;
+		for (int f = 0; f < FLYEHIP_K_COUNT; ++f)
 		{
 			c->stats.kernel_ms[f] += s.kernel_ms[f];
 			c->stats.kernel_launches[f] += s.kernel_launches[f];

@@ -614,9 +614,13 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* e
 		}
 		const SegRange* rs = b.big[cur];
 		const int32_t r32 = static_cast<int32_t>(nR);
-		// round bytes: keys read by the count and the rank passes, swapped
-		// pairs read and written (at most half the range): 16 per element
-		const int tround = b.timer ? b.timer->begin(FLYEHIP_K_ROUNDS, s, 16.0 * static_cast<double>(b.hostCounters[kSegBigElems])) : -1;
+		// round bytes per element of the level's big ranges: keys read by the
+		// count pass (4) and by the rank pass (4), swapped pairs read and
+		// written (at most half the range: 8); the median, map and split
+		// passes and the chunk scans move per-range and per-chunk words only
+		const double bigElems = static_cast<double>(b.hostCounters[kSegBigElems]);
+		KTimer* const tm = b.timer;
+		int tround = tm ? tm->begin(FLYEHIP_K_ROUND_OTHER, s) : -1;
 		hipLaunchKernelGGL(segMedianKernel, dim3(blocksFor(nR + 1, 256)), dim3(256), 0, s, b, rs, r32);
 		size_t tb = b.scanTempBytes;
 		// chunk counts were written to chunkOff; scan them into chunkPre, then
@@ -628,17 +632,27 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* e
 						   hipMemcpyDeviceToDevice, s);
 		if (e != hipSuccess) return e;
 		hipLaunchKernelGGL(segMapKernel, dim3(blocksFor(nR, kWavesPerBlock)), dim3(tpb), 0, s, b, r32);
+		if (tm) tm->end(tround, s);
+		tround = tm ? tm->begin(FLYEHIP_K_ROUND_COUNT, s, 4.0 * bigElems) : -1;
 		hipLaunchKernelGGL(segCountKernel, dim3(blocksFor(nChunks + 1, kWavesPerBlock)), dim3(tpb), 0, s, b, rs,
 						   nChunks);
+		if (tm) tm->end(tround, s);
+		tround = tm ? tm->begin(FLYEHIP_K_ROUND_OTHER, s) : -1;
 		tb = b.scanTempBytes;
 		e = hipcub::DeviceScan::ExclusiveSum(b.scanTemp, tb, b.chunkCnt, b.chunkPre, static_cast<int>(nChunks + 1), s);
 		if (e != hipSuccess) return e;
+		if (tm) tm->end(tround, s);
+		tround = tm ? tm->begin(FLYEHIP_K_ROUND_RANK, s, 4.0 * bigElems) : -1;
 		hipLaunchKernelGGL(segRankKernel, dim3(blocksFor(nChunks, kWavesPerBlock)), dim3(tpb), 0, s, b, rs, nChunks);
+		if (tm) tm->end(tround, s);
+		tround = tm ? tm->begin(FLYEHIP_K_ROUND_SWAP, s, 8.0 * bigElems) : -1;
 		hipLaunchKernelGGL(segSwapKernel, dim3(blocksFor(nChunks, kWavesPerBlock)), dim3(tpb), 0, s, b, rs, nChunks);
+		if (tm) tm->end(tround, s);
+		tround = tm ? tm->begin(FLYEHIP_K_ROUND_OTHER, s) : -1;
 		e = hipMemsetAsync(b.counters + kSegNBig, 0, 3 * sizeof(int64_t), s);
 		if (e != hipSuccess) return e;
 		hipLaunchKernelGGL(segSplitKernel, dim3(blocksFor(nR, 256)), dim3(256), 0, s, b, rs, r32, b.big[cur ^ 1]);
-		if (b.timer) b.timer->end(tround, s);
+		if (tm) tm->end(tround, s);
 		e = hipGetLastError();
 		if (e != hipSuccess) return e;
 		e = download(b, s);

@@ -70,8 +70,8 @@ class Stats(C.Structure):
                 ("queries", C.c_int64), ("windows", C.c_int64), ("matches", C.c_int64),
                 ("groups", C.c_int64), ("overlaps", C.c_int64), ("batches", C.c_int64),
                 ("device_bytes_peak", C.c_int64), ("tie_queries", C.c_int64),
-                ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_int64 * 8),
-                ("kernel_bytes", C.c_double * 8)]
+                ("kernel_ms", C.c_double * 16), ("kernel_launches", C.c_int64 * 16),
+                ("kernel_bytes", C.c_double * 16), ("dp_cells", C.c_int64), ("reserved_stats", C.c_int64)]
 
     def as_dict(self) -> dict:
         out = {}
@@ -81,7 +81,30 @@ class Stats(C.Structure):
         return out
 
 
-KERNEL_FAMILIES = ("join", "radix", "leaf", "heap", "prefilter", "dp", "walk", "rounds")
+# Kernels of flyehip_stats_t.kernel_* (FLYEHIP_K_* order in include/flyehip.h).
+KERNEL_FAMILIES = ("join_count", "join_fill", "radix", "rank_keys", "leaf", "heap", "prefilter", "dp", "walk",
+                   "round_count", "round_rank", "round_swap", "round_other", "gather", "score_keys", "unused")
+KERNEL_NAMES = {
+    "join_count": "joinCountKernel",
+    "join_fill": "joinFillKernel",
+    "radix": "rocprim segmented radix sort (hipcub DeviceSegmentedRadixSort)",
+    "rank_keys": "rankKeysKernel",
+    "leaf": "segLeafKernel",
+    "heap": "segHeapLdsKernel (+ segHeapGlobalKernel)",
+    "prefilter": "chainPrefilterKernel",
+    "dp": "chainDpKernel<G>",
+    "walk": "chainWalkKernel<size class, onlyMaxExt>",
+    "round_count": "segCountKernel",
+    "round_rank": "segRankKernel",
+    "round_swap": "segSwapKernel",
+    "round_other": "segMedian/Map/SplitKernel + chunk scans",
+    "gather": "gatherHitsKernel",
+    "score_keys": "chainScoreKeysKernel",
+    "unused": "-",
+}
+# Kernels that are one HIP kernel (a roofline names one of these).
+SINGLE_KERNELS = ("join_count", "join_fill", "rank_keys", "leaf", "heap", "prefilter", "dp", "walk", "round_count",
+                  "round_rank", "round_swap", "gather", "score_keys")
 
 
 CANDIDATE_DTYPE = np.dtype([("query_id", "<i8"), ("target_id", "<i8"), ("kmer", "<u8"),

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FLYEHIP_ABI_VERSION 2
+#define FLYEHIP_ABI_VERSION 3
 
 /* Status codes. */
 #define FLYEHIP_OK 0
@@ -162,6 +162,25 @@ typedef struct {
     int32_t reserved;
 } flyehip_chain_segment_t;   /* 24 bytes */
 
+/* Kernels timed in flyehip_stats_t (one HIP kernel each unless noted). */
+#define FLYEHIP_K_JOIN_COUNT 0   /* joinCountKernel: windows per query              */
+#define FLYEHIP_K_JOIN_FILL 1    /* joinFillKernel: k-mer matches in emission order */
+#define FLYEHIP_K_RADIX 2        /* stable segmented radix sort of the matches (the
+                                    library sort's own kernels)                     */
+#define FLYEHIP_K_RANK_KEYS 3    /* rankKeysKernel: tie flags and rank keys         */
+#define FLYEHIP_K_LEAF 4         /* segLeafKernel: exact leaf sorts in LDS          */
+#define FLYEHIP_K_HEAP 5         /* segHeapLdsKernel / segHeapGlobalKernel          */
+#define FLYEHIP_K_PREFILTER 6    /* chainPrefilterKernel                            */
+#define FLYEHIP_K_DP 7           /* chainDpKernel<G>                                */
+#define FLYEHIP_K_WALK 8         /* chainWalkKernel<size class, onlyMaxExt>         */
+#define FLYEHIP_K_ROUND_COUNT 9  /* segCountKernel: partition flags per chunk       */
+#define FLYEHIP_K_ROUND_RANK 10  /* segRankKernel: swap partners                    */
+#define FLYEHIP_K_ROUND_SWAP 11  /* segSwapKernel                                   */
+#define FLYEHIP_K_ROUND_OTHER 12 /* segMedian/Map/SplitKernel and the chunk scans   */
+#define FLYEHIP_K_GATHER 13      /* gatherHitsKernel: group-order hits              */
+#define FLYEHIP_K_SCORE_KEYS 14  /* chainScoreKeysKernel                            */
+#define FLYEHIP_K_COUNT 16
+
 /* Timing and counters of the last call (milliseconds on the library stream). */
 typedef struct {
     double ms_total;
@@ -178,24 +197,17 @@ typedef struct {
     int64_t batches;
     int64_t device_bytes_peak;
     int64_t tie_queries;     /* queries whose match sort needed tie emulation */
-    /* Kernel families (FLYEHIP_K_*): GPU time summed over the launches (HIP
-     * events on the stream each launch runs on; launches on different
-     * streams overlap, so the sum can exceed ms_total), launch count, and
-     * algorithmic bytes (the per-unit figures documented with the roofline
-     * times the units the launches processed). */
-    double kernel_ms[8];
-    int64_t kernel_launches[8];
-    double kernel_bytes[8];
+    /* Kernels (FLYEHIP_K_*): GPU time summed over the launches (HIP events
+     * on the stream each launch runs on; launches on different streams
+     * overlap, so the sum can exceed ms_total), launch count, and algorithmic
+     * bytes (the per-unit figures documented with the roofline times the
+     * units the launches processed). */
+    double kernel_ms[FLYEHIP_K_COUNT];
+    int64_t kernel_launches[FLYEHIP_K_COUNT];
+    double kernel_bytes[FLYEHIP_K_COUNT];
+    int64_t dp_cells;        /* (row, predecessor) pairs the chaining DP evaluated */
+    int64_t reserved_stats;
 } flyehip_stats_t;
-
-#define FLYEHIP_K_JOIN 0       /* window count + match fill                 */
-#define FLYEHIP_K_RADIX 1      /* stable segmented radix sort of the matches */
-#define FLYEHIP_K_LEAF 2       /* exact leaf sorts (LDS tiles), all sorts    */
-#define FLYEHIP_K_HEAP 3       /* heap-sort fallbacks, all sorts             */
-#define FLYEHIP_K_PREFILTER 4  /* group prefilters                           */
-#define FLYEHIP_K_DP 5         /* chaining DP                                */
-#define FLYEHIP_K_WALK 6       /* backtrack, overlapTest, selection          */
-#define FLYEHIP_K_ROUNDS 7     /* partition rounds of the segmented sorts    */
 
 /* ---- context ---------------------------------------------------------- */
 
