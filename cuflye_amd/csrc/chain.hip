@@ -30,6 +30,10 @@ namespace
 
 constexpr int kLaneMax = 48;
 constexpr uint32_t kWalkRelevant = 0x80000000u;  // score-order value bit: row has a backtrack pointer
+
+// std::sort(indices, score[a] > score[b]) as an ascending 32-bit key (scores
+// are non-negative).
+__device__ inline uint32_t scoreOrderKey(int32_t score) { return static_cast<uint32_t>(0x7fffffff - score); }
 // LDS ring entries per wave (two rings), split over groups: 4 KiB of LDS per
 // wave keeps the DP's occupancy at its VGPR limit (a 16 KiB ring held it to
 // about 2.5 waves per SIMD, and the lookback scan is latency-bound)
@@ -454,6 +458,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 	int32_t* sc = nullptr;
 	int32_t* bt = nullptr;
 	int32_t* pmG = nullptr;
+	uint32_t* sk = nullptr;  // score-order sort keys and values of the group's rows
+	uint32_t* sv = nullptr;
 	int pm = 0;
 	// rows are stored G at a time: lane (r mod G) of the subgroup holds row r's
 	// score, backtrack and prefix maximum until its G-row block is complete,
@@ -498,6 +504,10 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			sc = a.score + gd.hitStart;
 			bt = a.bt + gd.hitStart;
 			pmG = a.posR + gd.hitStart;  // free during the DP: holds max(score[0..j])
+			// the extPos sort's keys and permutation are dead once the group's
+			// hits are gathered: they receive the score-order sort's input
+			sk = a.sortKey + gd.hitStart;
+			sv = a.sortVal + gd.hitStart;
 			pm = 0;
 			winC = hc[0];  // row 0, for lane 0 of row 1's first tile
 			winE = he[0];
@@ -515,6 +525,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 				sc[0] = 0;
 				bt[0] = -1;
 				pmG[0] = 0;
+				sk[0] = scoreOrderKey(0);
+				sv[0] = 0;
 			}
 			hs = 0;  // row 0, held by lane 0
 			hbt = -1;
@@ -642,6 +654,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 					sc[r] = hs;
 					bt[r] = hbt;
 					pmG[r] = hpm;
+					sk[r] = scoreOrderKey(hs);
+					sv[r] = static_cast<uint32_t>(r) | (hbt != -1 ? kWalkRelevant : 0u);
 				}
 			}
 			const int32_t nc = i + 1 < n ? hc[i + 1] : 0;
@@ -679,29 +693,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 
 // ---- finish ----------------------------------------------------------------------
 
-// start[g] = first hit of group g, start[nGroups] = nHits.
-__global__ void groupStartKernel(const GroupDesc* groups, int32_t nGroups, int64_t nHits, int64_t* start)
-{
-	const int g = blockIdx.x * blockDim.x + threadIdx.x;
-	if (g < nGroups) start[g] = groups[g].hitStart;
-	if (g == nGroups) start[g] = nHits;
-}
-
-// Score-order sort keys of every passing group (std::sort(indices,
-// score[a] > score[b]) as an ascending key).  The walk skips every start
-// without a backtrack pointer, so among equal scores only the order of rows
-// that have one can matter: those carry kWalkRelevant in their value.  One
-// thread per hit (the groups' hits tile [0, nHits) in group order, so a hit
-// finds its group in the group starts); streaming, unlike a wave per group.
-__global__ void chainScoreKeysKernel(ChainArgs a, const int64_t* start, int64_t nHits)
-{
-	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (i >= nHits) return;
-	const int g = findSegWave(start, a.nGroups, i);
-	if (!a.groupPass[g]) return;
-	a.sortKey[i] = static_cast<uint32_t>(0x7fffffff - a.score[i]);
-	a.sortVal[i] = static_cast<uint32_t>(i - start[g]) | (a.bt[i] != -1 ? kWalkRelevant : 0u);
-}
 
 // Pushes every passing group's range to the score sort (one lane per group).
 __global__ void chainScorePushKernel(ChainArgs a, SegSortBufs sb)
@@ -1085,15 +1076,9 @@ hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream)
 	return hipGetLastError();
 }
 
-hipError_t launchChainScoreKeys(const ChainArgs& a, const SegSortBufs& sb, int64_t* start, int64_t nHits,
-								hipStream_t stream)
+hipError_t launchChainScorePush(const ChainArgs& a, const SegSortBufs& sb, hipStream_t stream)
 {
 	const int tpb = 256;
-	hipLaunchKernelGGL(groupStartKernel, dim3((a.nGroups + 1 + tpb - 1) / tpb), dim3(tpb), 0, stream, a.groups,
-					   a.nGroups, nHits, start);
-	if (nHits > 0)
-		hipLaunchKernelGGL(chainScoreKeysKernel, dim3(static_cast<unsigned>((nHits + tpb - 1) / tpb)), dim3(tpb), 0,
-						   stream, a, static_cast<const int64_t*>(start), nHits);
 	if (a.nPass > 0)
 		hipLaunchKernelGGL(chainScorePushKernel, dim3((a.nPass + tpb - 1) / tpb), dim3(tpb), 0, stream, a, sb);
 	return hipGetLastError();

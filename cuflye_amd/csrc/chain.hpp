@@ -114,11 +114,9 @@ hipError_t launchSplitPass(const ChainArgs& a, const int32_t* deferred, int32_t*
 						   hipStream_t stream);
 hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream);
 hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t stream);
-// score-order keys of the passing groups and their pushes to sb; start
-// receives the group starts (nGroups + 1 words; the groups' hits tile
-// [0, nHits) in group order)
-hipError_t launchChainScoreKeys(const ChainArgs& a, const SegSortBufs& sb, int64_t* start, int64_t nHits,
-								hipStream_t stream);
+// pushes the passing groups' score-order ranges to sb (the DP writes their
+// keys and values into sortKey / sortVal with its rows)
+hipError_t launchChainScorePush(const ChainArgs& a, const SegSortBufs& sb, hipStream_t stream);
 // backtrack .. selection for passOrder[first, last)
 // tier: 0 below kWalkSmall hits, 1 below kWalkMid, 2 below kWalkBig, 3 above
 hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, int tier, int blocks,

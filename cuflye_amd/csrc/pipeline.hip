@@ -76,26 +76,49 @@ __global__ void __launch_bounds__(1024) scanBlockSumsKernel(int64_t* sums, int64
 	if (threadIdx.x == 0) sums[nb] = total;
 }
 
+// LDS slot of tile position p: one pad word per 16, so a thread reading its
+// kScanIT consecutive positions and the block reading a row of kScanNT
+// positions both spread over the banks.
+__device__ inline int scanPad(int p) { return p + (p >> 4); }
+
+// Tile-wise exclusive scan with the tile's offset from scanBlockSumsKernel.
+// Loads and stores are row-major across the block (each wave touches
+// consecutive words); the per-thread runs of kScanIT consecutive values the
+// scan works on are transposed through LDS.
 __global__ void __launch_bounds__(kScanNT) scanDownKernel(const int32_t* in, int64_t* out, int64_t n,
 														  const int64_t* blockSums, int64_t nb)
 {
 	__shared__ int64_t sh[kScanNT / FH_WAVE];
-	const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kScanIT;
-	int32_t v[kScanIT];
+	__shared__ int64_t tile[kScanTile + kScanTile / 16];
+	const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+	const int t = static_cast<int>(threadIdx.x);
+	for (int i = 0; i < kScanIT; ++i)
+	{
+		const int p = i * kScanNT + t;
+		const int64_t idx = base + p;
+		tile[scanPad(p)] = idx < n ? in[idx] : 0;
+	}
+	__syncthreads();
+	int64_t v[kScanIT];
 	int64_t s = 0;
 	for (int i = 0; i < kScanIT; ++i)
 	{
-		const int64_t idx = b0 + i;
-		v[i] = idx < n ? in[idx] : 0;
+		v[i] = tile[scanPad(t * kScanIT + i)];
 		s += v[i];
 	}
 	int64_t total;
 	int64_t ex = blockExclusiveScan64(s, total, sh) + blockSums[blockIdx.x];
 	for (int i = 0; i < kScanIT; ++i)
 	{
-		const int64_t idx = b0 + i;
-		if (idx < n) out[idx] = ex;
+		tile[scanPad(t * kScanIT + i)] = ex;
 		ex += v[i];
+	}
+	__syncthreads();
+	for (int i = 0; i < kScanIT; ++i)
+	{
+		const int p = i * kScanNT + t;
+		const int64_t idx = base + p;
+		if (idx < n) out[idx] = tile[scanPad(p)];
 	}
 	if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = blockSums[nb];
 }

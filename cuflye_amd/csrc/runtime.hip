@@ -815,7 +815,8 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	HIPCHECK(launchChainGather(ca, blocks, L->stream));
 	HIPCHECK(hipEventRecord(L->ev[3], L->stream));
 	ca.queue = queues + 2;
-	// DP: hits read (hcur, hext) and score / backtrack written, 16 bytes per hit
+	// DP: hits read (hcur, hext) and score / backtrack written, 16 bytes per
+	// hit (plus the score-order keys it writes for the sort that follows)
 	const int td = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * readyHits);
 	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, L->stream));
 	L->timer.end(td, L->stream);
@@ -838,10 +839,9 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	HIPCHECK(segSortReset(sb, L->stream));
 	sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
 	ca.queue = queues + 3;
-	// score keys: score and backtrack read, key and value written (16 bytes per hit)
-	const int tsk = L->timer.begin(FLYEHIP_K_SCORE_KEYS, L->stream, 16.0 * static_cast<double>(nHits));
-	HIPCHECK(launchChainScoreKeys(ca, sb, ensureT<int64_t>(c, L->bGroupStart, static_cast<size_t>(ng) + 1), nHits,
-								  L->stream));
+	// the DP wrote the score-order keys with its rows: push the ranges
+	const int tsk = L->timer.begin(FLYEHIP_K_SCORE_KEYS, L->stream);
+	HIPCHECK(launchChainScorePush(ca, sb, L->stream));
 	L->timer.end(tsk, L->stream);
 	runSegSort(L, sb, nullptr);
 	ca.queue = queues + 4;
