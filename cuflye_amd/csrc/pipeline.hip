@@ -222,6 +222,31 @@ __global__ void assignKernel(HashTable t, const int32_t* sizes, const int64_t* s
 	if (s > 0) t.vals[h] = (static_cast<uint64_t>(starts[h]) << kCountBits) | static_cast<uint64_t>(s);
 }
 
+__global__ void loadKeysKernel(HashTable t, const uint64_t* keys, const uint64_t* vals, int64_t n, uint32_t* err)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const uint64_t c = keys[i];
+	uint64_t h = hashKmer(c) & t.mask;
+	for (;;)
+	{
+		const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&t.keys[h]),
+										static_cast<unsigned long long>(kEmptyKey), static_cast<unsigned long long>(c));
+		if (prev == kEmptyKey)
+		{
+			t.vals[h] = vals[i];
+			t.cursor[h] = 0;
+			return;
+		}
+		if (prev == c)
+		{
+			atomicOr(err, 1u);
+			return;
+		}
+		h = (h + 1) & t.mask;
+	}
+}
+
 __device__ inline bool indexed(uint64_t v) { return v != 0 && v != kRepMark; }
 
 __global__ void fillKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
@@ -736,6 +761,14 @@ hipError_t launchFill(const PackedSeqs& reads, const int64_t* winOff, int64_t nW
 {
 	if (nWin == 0) return hipSuccess;
 	hipLaunchKernelGGL(fillKernel, gridFor(nWin, 256), dim3(256), 0, s, reads, winOff, nWin, k, t, ordKeys);
+	return hipGetLastError();
+}
+
+hipError_t launchLoadKeys(HashTable t, const uint64_t* keys, const uint64_t* vals, int64_t n, uint32_t* err,
+						  hipStream_t s)
+{
+	if (n == 0) return hipSuccess;
+	hipLaunchKernelGGL(loadKeysKernel, gridFor(n, 256), dim3(256), 0, s, t, keys, vals, n, err);
 	return hipGetLastError();
 }
 

@@ -44,6 +44,10 @@ hipError_t launchHistogram(HashTable t, uint64_t* hist, int hmax, uint64_t* acc,
 hipError_t launchClassify(HashTable t, int solidMin, int repMax, int32_t* sizes, uint64_t* counters,
 						  hipStream_t s);
 hipError_t launchAssign(HashTable t, const int32_t* sizes, const int64_t* starts, hipStream_t s);
+// Inserts n given keys with their slot values (an index loaded from the
+// caller); *err is set when a key is inserted twice.
+hipError_t launchLoadKeys(HashTable t, const uint64_t* keys, const uint64_t* vals, int64_t n, uint32_t* err,
+						  hipStream_t s);
 hipError_t launchFill(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
 					  uint64_t* ordKeys, hipStream_t s);
 // Puts every bucket in (read, pos) order and writes its entries: buckets of at

@@ -24,6 +24,8 @@
 #include <new>
 #include <string>
 #include <thread>
+#include <deque>
+#include <unordered_map>
 #include <vector>
 
 #include "chain.hpp"
@@ -184,6 +186,37 @@ struct Lane
 	DevBuf bGroupStart;                   // group starts of a batch's hits (score keys)
 };
 
+// One flyehip_overlaps_submit call: its query records and, as batches finish,
+// their rows in query order (a prefix of `done` queries is complete).
+struct AsyncJob
+{
+	std::vector<int64_t> ids;
+	flyehip_overlap_params_t p{};
+	bool withDiv = false;
+	int32_t done = 0;              // queries whose results are in place
+	int code = FLYEHIP_OK;         // first error (then every query is done)
+	std::string msg;
+	std::vector<flyehip_overlap_t> rows;
+	std::vector<int64_t> rowOff{0};
+	std::vector<float> divs;
+	std::vector<int64_t> divOff{0};
+};
+
+// Submitted jobs, run in submission order by one worker thread; collectors
+// wait on the condition variable for their query's batch.  Lock order: the
+// worker takes the context mutex for a chunk and this mutex afterwards,
+// never both at once.
+struct AsyncQueue
+{
+	std::mutex m;
+	std::condition_variable cv;
+	std::deque<std::shared_ptr<AsyncJob>> pending;
+	std::unordered_map<uint64_t, std::shared_ptr<AsyncJob>> jobs;
+	std::thread worker;
+	bool stop = false;
+	uint64_t nextTicket = 1;
+};
+
 } // namespace
 } // namespace fh
 
@@ -214,6 +247,7 @@ struct flyehip_ctx
 
 	// index
 	bool haveIndex = false;
+	bool indexLoaded = false;  // buckets given by the caller (flyehip_load_index), in the caller's order
 	int k = 0;
 	flyehip_index_info_t info{};
 	fh::DevBuf dKeys, dVals, dCursor, dEntries;
@@ -225,6 +259,9 @@ struct flyehip_ctx
 
 	// device address of this device's no-allocation scratch array
 	unsigned char* scratch = nullptr;
+
+	// flyehip_overlaps_submit / _collect (started on the first submit)
+	std::unique_ptr<fh::AsyncQueue> async;
 
 	flyehip_stats_t stats{};
 };
@@ -1145,14 +1182,10 @@ void runBatch(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& all
 // host thread on its own stream, so one batch's long tails (the last groups
 // of a kernel, a heap-sort fallback) overlap the next batch's work; results
 // are assembled in query order afterwards.
-void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQuery>& allQ,
-				  const int64_t* queryIds, const flyehip_overlap_params_t* p, flyehip_overlap_t* rows,
-				  int64_t rowCap, int64_t* rowOffsets, int64_t* rowsNeeded, float* divStats, int64_t divCap,
-				  int64_t* divOffsets, int batchQueries, int maxLanes)
+std::vector<BatchResult> runBatches(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQuery>& allQ,
+									const int64_t* queryIds, const flyehip_overlap_params_t* p, bool withDiv,
+									int batchQueries, int maxLanes)
 {
-	checkParams(p, c->k);
-	resetStats(c);
-	Timer total;
 	const int32_t n = static_cast<int32_t>(allQ.size());
 	const int32_t nBatches = (n + batchQueries - 1) / batchQueries;
 	const int nl = std::max(1, std::min(std::min(c->batchesInFlight, maxLanes), nBatches));
@@ -1190,7 +1223,7 @@ void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<Join
 						}
 					}
 				} hold(c, L);
-				runBatch(L, seqs, allQ, queryIds, *p, b0, std::min(n, b0 + batchQueries), divOffsets != nullptr,
+				runBatch(L, seqs, allQ, queryIds, *p, b0, std::min(n, b0 + batchQueries), withDiv,
 						 res[static_cast<size_t>(b)]);
 			}
 		}
@@ -1208,6 +1241,23 @@ void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<Join
 	c->lanesRunning = 0;
 	for (auto& e : errs)
 		if (e) std::rethrow_exception(e);
+	return res;
+}
+
+// Batched overlap pipeline over JoinQuery lists (resident or external) with
+// the rows copied into the caller's arrays.
+void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<JoinQuery>& allQ,
+				  const int64_t* queryIds, const flyehip_overlap_params_t* p, flyehip_overlap_t* rows,
+				  int64_t rowCap, int64_t* rowOffsets, int64_t* rowsNeeded, float* divStats, int64_t divCap,
+				  int64_t* divOffsets, int batchQueries, int maxLanes)
+{
+	checkParams(p, c->k);
+	resetStats(c);
+	Timer total;
+	const int32_t n = static_cast<int32_t>(allQ.size());
+	const int32_t nBatches = (n + batchQueries - 1) / batchQueries;
+	std::vector<BatchResult> res =
+		runBatches(c, seqs, allQ, queryIds, p, divOffsets != nullptr, batchQueries, maxLanes);
 
 	int64_t written = 0, divWritten = 0;
 	bool capacityShort = false;
@@ -1255,6 +1305,131 @@ void overlapsImpl(flyehip_ctx* c, const PackedSeqs& seqs, const std::vector<Join
 			 static_cast<long long>(divCap));
 }
 
+// ---- submitted jobs -----------------------------------------------------------
+
+std::vector<JoinQuery> residentQueries(flyehip_ctx* c, const int64_t* ids, int32_t n);
+
+// One chunk of a job under the context mutex: the batched pipeline over the
+// chunk's query records.
+std::vector<BatchResult> asyncRunChunk(flyehip_ctx* c, const AsyncJob& job, int32_t b0, int32_t b1)
+{
+	std::lock_guard<std::mutex> lock(c->mu);
+	HIPCHECK(hipSetDevice(c->device));
+	std::vector<JoinQuery> qs = residentQueries(c, job.ids.data() + b0, b1 - b0);
+	checkParams(&job.p, c->k);
+	resetStats(c);
+	std::vector<BatchResult> res =
+		runBatches(c, storeSeqs(c), qs, job.ids.data() + b0, &job.p, job.withDiv, c->queriesPerBatch, 4);
+	foldStats(c);
+	return res;
+}
+
+// Runs submitted jobs in order, a chunk of queries_per_batch x
+// batches_in_flight query records at a time, and publishes each chunk's rows.
+void asyncWorker(flyehip_ctx* c)
+{
+	AsyncQueue& q = *c->async;
+	for (;;)
+	{
+		std::shared_ptr<AsyncJob> job;
+		{
+			std::unique_lock<std::mutex> l(q.m);
+			q.cv.wait(l, [&] { return q.stop || !q.pending.empty(); });
+			if (q.stop) return;
+			job = q.pending.front();
+		}
+		const int32_t n = static_cast<int32_t>(job->ids.size());
+		for (int32_t b0 = 0; b0 < n;)
+		{
+			int32_t chunk;
+			{
+				std::lock_guard<std::mutex> l(c->mu);
+				chunk = std::max(1, c->queriesPerBatch * std::max(1, c->batchesInFlight));
+			}
+			const int32_t b1 = std::min(n, b0 + chunk);
+			std::vector<BatchResult> res;
+			int code = FLYEHIP_OK;
+			std::string msg;
+			try
+			{
+				res = asyncRunChunk(c, *job, b0, b1);
+			}
+			catch (const FhError& e)
+			{
+				code = e.code;
+				msg = e.msg;
+			}
+			catch (const std::bad_alloc&)
+			{
+				code = FLYEHIP_ERR_BUDGET;
+				msg = "host allocation failed";
+			}
+			catch (...)
+			{
+				code = FLYEHIP_ERR_STATE;
+				msg = "unexpected exception in the job worker";
+			}
+			std::lock_guard<std::mutex> l(q.m);
+			if (code != FLYEHIP_OK)
+			{
+				job->code = code;
+				job->msg = msg;
+				job->done = n;
+			}
+			else
+			{
+				for (const BatchResult& r : res)
+				{
+					job->rows.insert(job->rows.end(), r.rows.begin(), r.rows.end());
+					job->divs.insert(job->divs.end(), r.divs.begin(), r.divs.end());
+					for (size_t i = 0; i < r.count.size(); ++i)
+					{
+						job->rowOff.push_back(job->rowOff.back() + r.count[i]);
+						job->divOff.push_back(job->divOff.back() + (job->withDiv ? r.divCount[i] : 0));
+					}
+				}
+				job->done = b1;
+			}
+			q.cv.notify_all();
+			if (q.stop) return;
+			if (code != FLYEHIP_OK) break;
+			b0 = b1;
+		}
+		std::lock_guard<std::mutex> l(q.m);
+		q.pending.pop_front();
+		q.cv.notify_all();
+	}
+}
+
+// Stops the worker; jobs not finished fail, so no collector waits forever.
+void asyncStop(flyehip_ctx* c)
+{
+	if (!c->async) return;
+	AsyncQueue& q = *c->async;
+	{
+		std::lock_guard<std::mutex> l(q.m);
+		q.stop = true;
+		for (auto& kv : q.jobs)
+		{
+			AsyncJob& j = *kv.second;
+			if (j.done < static_cast<int32_t>(j.ids.size()))
+			{
+				j.code = FLYEHIP_ERR_STATE;
+				j.msg = "context closed before the job finished";
+				j.done = static_cast<int32_t>(j.ids.size());
+			}
+		}
+		q.cv.notify_all();
+	}
+	if (q.worker.joinable()) q.worker.join();
+}
+
+void setError(flyehip_ctx* c, const std::string& msg)
+{
+	std::lock_guard<std::mutex> lock(c->mu);
+	c->err = msg;
+}
+
 // ---- no-allocation read-to-graph chaining (the scratch array is at the top) ----
 constexpr int kMaxDevices = 64;
 std::mutex gScratchMu;
@@ -1288,7 +1463,9 @@ std::vector<JoinQuery> residentQueries(flyehip_ctx* c, const int64_t* ids, int32
 		q.seq = static_cast<int32_t>(r - 1);
 		q.strand = id > 0 ? 1 : 0;
 		q.iid = toInternal(id);
-		q.selfMode = 1;
+		// a loaded index need not hold the query's own window, nor keep
+		// buckets in (read, position) order: scan the bucket for it
+		q.selfMode = c->indexLoaded ? 2 : 1;
 		q.len = c->readLen[static_cast<size_t>(r - 1)];
 		q.reserved = 0;
 		qs[static_cast<size_t>(i)] = q;
@@ -1341,6 +1518,7 @@ int flyehip_open(int device, uint64_t memory_budget_bytes, flyehip_ctx** out_ctx
 int flyehip_close(flyehip_ctx* c)
 {
 	if (!c) return FLYEHIP_OK;
+	asyncStop(c);
 	(void)hipSetDevice(c->device);
 	for (auto& l : c->lanes) destroyLane(c, *l);
 	for (DevBuf* b : {&c->dWords, &c->dWordOff, &c->dReadLen, &c->dReadWinOff, &c->dKeys, &c->dVals, &c->dCursor,
@@ -1411,6 +1589,7 @@ int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehi
 		if (k < 1 || k > 31) fail(FLYEHIP_ERR_ARG, "kmer_size %d out of range 1..31", k);
 		c->k = k;
 		c->haveIndex = false;
+		c->indexLoaded = false;
 		std::vector<int64_t> winOff(static_cast<size_t>(c->nReads) + 1, 0);
 		for (int32_t i = 0; i < c->nReads; ++i)
 			winOff[static_cast<size_t>(i) + 1] = winOff[static_cast<size_t>(i)] + std::max(0, c->readLen[static_cast<size_t>(i)] - k + 1);
@@ -1480,6 +1659,107 @@ int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehi
 		c->info.hash_slots = static_cast<int64_t>(slots);
 		c->info.device_bytes = static_cast<int64_t>(c->dKeys.bytes + c->dVals.bytes + c->dCursor.bytes + c->dEntries.bytes + c->dWords.bytes);
 		c->haveIndex = true;
+		if (out) *out = c->info;
+	});
+}
+
+int flyehip_load_index(flyehip_ctx* c, int32_t kmer_size, const uint64_t* kmers, int64_t n_kmers,
+					   const int64_t* bucket_offsets, const int64_t* entry_ids, const int32_t* entry_pos,
+					   const uint64_t* repetitive, int64_t n_repetitive, flyehip_index_info_t* out)
+{
+	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
+		if (c->nReads <= 0) fail(FLYEHIP_ERR_STATE, "no reads stored");
+		const int k = kmer_size;
+		if (k < 1 || k > 31) fail(FLYEHIP_ERR_ARG, "kmer_size %d out of range 1..31", k);
+		if (n_kmers < 0 || n_repetitive < 0 || (n_kmers > 0 && (!kmers || !bucket_offsets)) ||
+			(n_repetitive > 0 && !repetitive))
+			fail(FLYEHIP_ERR_ARG, "invalid index tables");
+		c->haveIndex = false;
+		c->indexLoaded = false;
+		const int64_t nEntries = n_kmers > 0 ? bucket_offsets[n_kmers] - bucket_offsets[0] : 0;
+		if (nEntries > 0 && (!entry_ids || !entry_pos)) fail(FLYEHIP_ERR_ARG, "null bucket entries");
+		const uint64_t kmask = (uint64_t{1} << (2 * k)) - 1;
+		auto rc = [&](uint64_t x) {
+			uint64_t r = 0;
+			for (int i = 0; i < k; ++i)
+			{
+				r = (r << 2) | (~x & 3ULL);
+				x >>= 2;
+			}
+			return r;
+		};
+		// slot values: (start << 24 | count) for a bucket, kRepMark for a
+		// repetitive k-mer; every k-mer must be in standard form
+		const int64_t nKeys = n_kmers + n_repetitive;
+		std::vector<uint64_t> keys(static_cast<size_t>(nKeys)), vals(static_cast<size_t>(nKeys));
+		for (int64_t i = 0; i < n_kmers; ++i)
+		{
+			const uint64_t x = kmers[i];
+			if (x > kmask || rc(x) < x) fail(FLYEHIP_ERR_ARG, "k-mer %lld (%llu) is not a canonical %d-mer",
+										   static_cast<long long>(i), static_cast<unsigned long long>(x), k);
+			const int64_t b = bucket_offsets[i] - bucket_offsets[0], e = bucket_offsets[i + 1] - bucket_offsets[0];
+			if (e <= b || e - b > static_cast<int64_t>(kCountMask))
+				fail(FLYEHIP_ERR_ARG, "bucket %lld has %lld entries (1 .. %llu)", static_cast<long long>(i),
+					 static_cast<long long>(e - b), static_cast<unsigned long long>(kCountMask));
+			keys[static_cast<size_t>(i)] = x;
+			vals[static_cast<size_t>(i)] = (static_cast<uint64_t>(b) << kCountBits) | static_cast<uint64_t>(e - b);
+		}
+		for (int64_t i = 0; i < n_repetitive; ++i)
+		{
+			const uint64_t x = repetitive[i];
+			if (x > kmask || rc(x) < x) fail(FLYEHIP_ERR_ARG, "repetitive k-mer %lld is not a canonical %d-mer",
+										   static_cast<long long>(i), k);
+			keys[static_cast<size_t>(n_kmers + i)] = x;
+			vals[static_cast<size_t>(n_kmers + i)] = kRepMark;
+		}
+		// entries: (internal id << 32 | position) where the canonical k-mer
+		// is spelled on that strand
+		std::vector<uint64_t> ent(static_cast<size_t>(nEntries));
+		for (int64_t j = 0; j < nEntries; ++j)
+		{
+			const int64_t id = entry_ids[j];
+			const int64_t r = id > 0 ? id : -id;
+			if (id == 0 || r > c->nReads) fail(FLYEHIP_ERR_ARG, "entry %lld names read %lld (1..%d stored)",
+											   static_cast<long long>(j), static_cast<long long>(id), c->nReads);
+			const int32_t p = entry_pos[j];
+			if (p < 0 || p > c->readLen[static_cast<size_t>(r - 1)] - k)
+				fail(FLYEHIP_ERR_ARG, "entry %lld: position %d outside read %lld (length %d)", static_cast<long long>(j),
+					 p, static_cast<long long>(id), c->readLen[static_cast<size_t>(r - 1)]);
+			ent[static_cast<size_t>(j)] = (static_cast<uint64_t>(toInternal(id)) << 32) | static_cast<uint32_t>(p);
+		}
+		uint64_t slots = 1024;
+		while (slots < static_cast<uint64_t>(nKeys) * 3 / 2 + 1024) slots <<= 1;
+		c->k = k;
+		c->tableMask = slots - 1;
+		ensure(c, c->dKeys, slots * 8);
+		ensure(c, c->dVals, slots * 8);
+		ensure(c, c->dCursor, slots * 4);
+		ensure(c, c->dEntries, static_cast<size_t>(nEntries) * 8 + 8);
+		fill64(L, static_cast<uint64_t*>(c->dKeys.p), static_cast<int64_t>(slots), kEmptyKey);
+		fill64(L, static_cast<uint64_t*>(c->dVals.p), static_cast<int64_t>(slots), 0);
+		uint64_t* dk = ensureT<uint64_t>(c, L->bCKey, static_cast<size_t>(nKeys) + 1);
+		uint64_t* dv = ensureT<uint64_t>(c, L->bSKey, static_cast<size_t>(nKeys) + 1);
+		uint32_t* dErr = ensureT<uint32_t>(c, L->bPackErr, 1);
+		upload(L, dk, keys.data(), keys.size() * 8);
+		upload(L, dv, vals.data(), vals.size() * 8);
+		upload(L, c->dEntries.p, ent.data(), ent.size() * 8);
+		HIPCHECK(hipMemsetAsync(dErr, 0, 4, L->stream));
+		HIPCHECK(launchLoadKeys(table(c), dk, dv, nKeys, dErr, L->stream));
+		uint32_t err = 0;
+		download(L, &err, dErr, 4);
+		sync(L);
+		if (err) fail(FLYEHIP_ERR_ARG, "a k-mer appears twice in the index tables");
+		c->info = flyehip_index_info_t{};
+		c->info.kmer_size = k;
+		c->info.distinct_kmers = nKeys;
+		c->info.solid_kmers = n_kmers;
+		c->info.repetitive_kmers = n_repetitive;
+		c->info.indexed_positions = nEntries;
+		c->info.hash_slots = static_cast<int64_t>(slots);
+		c->info.device_bytes = static_cast<int64_t>(c->dKeys.bytes + c->dVals.bytes + c->dCursor.bytes + c->dEntries.bytes + c->dWords.bytes);
+		c->haveIndex = true;
+		c->indexLoaded = true;
 		if (out) *out = c->info;
 	});
 }
@@ -1555,6 +1835,110 @@ int flyehip_overlaps(flyehip_ctx* c, const int64_t* query_ids, int32_t n, const 
 		overlapsImpl(c, storeSeqs(c), qs, query_ids, p, rows, row_cap, row_offsets, rows_needed, div_stats, div_cap,
 					 div_offsets, c->queriesPerBatch, 4);
 	});
+}
+
+int flyehip_overlaps_submit(flyehip_ctx* c, const int64_t* query_ids, int32_t n, const flyehip_overlap_params_t* p,
+							int32_t with_div_stats, uint64_t* ticket)
+{
+	return guarded(c, [&]() {
+		if (!ticket) fail(FLYEHIP_ERR_ARG, "null ticket");
+		if (n < 0 || (n > 0 && !query_ids)) fail(FLYEHIP_ERR_ARG, "invalid query list");
+		checkParams(p, c->k);
+		(void)residentQueries(c, query_ids, n);  // the ids name resident reads
+		if (!c->async)
+		{
+			c->async = std::make_unique<AsyncQueue>();
+			c->async->worker = std::thread(asyncWorker, c);
+		}
+		auto job = std::make_shared<AsyncJob>();
+		job->ids.assign(query_ids, query_ids + n);
+		job->p = *p;
+		job->withDiv = with_div_stats != 0;
+		AsyncQueue& q = *c->async;
+		std::lock_guard<std::mutex> l(q.m);
+		const uint64_t t = q.nextTicket++;
+		q.jobs.emplace(t, job);
+		q.pending.push_back(job);
+		q.cv.notify_all();
+		*ticket = t;
+	});
+}
+
+int flyehip_overlaps_collect(flyehip_ctx* c, uint64_t ticket, int32_t query_index, flyehip_overlap_t* rows,
+							 int64_t row_cap, int64_t* rows_needed, float* div_stats, int64_t div_cap,
+							 int64_t* divs_needed)
+{
+	if (!c) return FLYEHIP_ERR_ARG;
+	if (!c->async)
+	{
+		setError(c, "no job submitted");
+		return FLYEHIP_ERR_STATE;
+	}
+	AsyncQueue& q = *c->async;
+	int code = FLYEHIP_OK;
+	std::string msg;
+	{
+		std::unique_lock<std::mutex> l(q.m);
+		auto it = q.jobs.find(ticket);
+		if (it == q.jobs.end())
+		{
+			code = FLYEHIP_ERR_ARG;
+			msg = "unknown or released ticket";
+		}
+		else
+		{
+			std::shared_ptr<AsyncJob> job = it->second;
+			const int32_t n = static_cast<int32_t>(job->ids.size());
+			if (query_index < 0 || query_index >= n)
+			{
+				code = FLYEHIP_ERR_ARG;
+				msg = "query index outside the submitted list";
+			}
+			else
+			{
+				q.cv.wait(l, [&] { return job->done > query_index; });
+				if (static_cast<int32_t>(job->rowOff.size()) <= query_index + 1)
+				{
+					code = job->code;
+					msg = job->msg;
+				}
+				else
+				{
+					const size_t i = static_cast<size_t>(query_index);
+					const int64_t r0 = job->rowOff[i], nr = job->rowOff[i + 1] - r0;
+					const int64_t d0 = job->divOff[i], nd = job->divOff[i + 1] - d0;
+					if (rows_needed) *rows_needed = nr;
+					if (divs_needed) *divs_needed = nd;
+					if (nr > row_cap || (div_stats && nd > div_cap))
+					{
+						code = FLYEHIP_ERR_CAPACITY;
+						msg = "capacity too small for the query's rows or divStats values";
+					}
+					else
+					{
+						if (nr) std::memcpy(rows, job->rows.data() + r0, static_cast<size_t>(nr) * sizeof(flyehip_overlap_t));
+						if (div_stats && nd) std::memcpy(div_stats, job->divs.data() + d0, static_cast<size_t>(nd) * sizeof(float));
+					}
+				}
+			}
+		}
+	}
+	if (code != FLYEHIP_OK) setError(c, msg);
+	return code;
+}
+
+int flyehip_overlaps_release(flyehip_ctx* c, uint64_t ticket)
+{
+	if (!c) return FLYEHIP_ERR_ARG;
+	if (!c->async) return FLYEHIP_OK;
+	AsyncQueue& q = *c->async;
+	std::unique_lock<std::mutex> l(q.m);
+	auto it = q.jobs.find(ticket);
+	if (it == q.jobs.end()) return FLYEHIP_OK;
+	std::shared_ptr<AsyncJob> job = it->second;
+	q.cv.wait(l, [&] { return job->done >= static_cast<int32_t>(job->ids.size()); });
+	q.jobs.erase(ticket);
+	return FLYEHIP_OK;
 }
 
 int flyehip_overlaps_per_query(flyehip_ctx* c, const int64_t* query_ids, int32_t n, const flyehip_overlap_params_t* p,

@@ -30,7 +30,8 @@ EXPORTED = (
     "flyehip_collect_sorted_candidates", "flyehip_overlaps",
     "flyehip_overlaps_external", "flyehip_overlaps_per_query",
     "flyehip_chain_lists", "flyehip_read_align_chains", "flyehip_divergence_logf",
-    "flyehip_read_align_chains_noalloc",
+    "flyehip_read_align_chains_noalloc", "flyehip_load_index", "flyehip_overlaps_submit",
+    "flyehip_overlaps_collect", "flyehip_overlaps_release",
 )
 
 
@@ -167,6 +168,10 @@ def load_flyehip(path: Path | None = None) -> C.CDLL:
         "flyehip_divergence_logf": ([_P, _P, i64, _P], C.c_int),
         "flyehip_read_align_chains_noalloc": ([_P, _P, _P, i32, i32, i32, C.POINTER(ChainParams), _P, _P,
                                                _P, i64, _P, _P], C.c_int),
+        "flyehip_load_index": ([_P, i32, _P, i64, _P, _P, _P, _P, i64, C.POINTER(IndexInfo)], C.c_int),
+        "flyehip_overlaps_submit": ([_P, _P, i32, C.POINTER(OverlapParams), i32, C.POINTER(u64)], C.c_int),
+        "flyehip_overlaps_collect": ([_P, u64, i32, _P, i64, _P, _P, i64, _P], C.c_int),
+        "flyehip_overlaps_release": ([_P, u64], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

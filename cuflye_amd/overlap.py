@@ -158,6 +158,22 @@ class FlyeHip:
         self.index_info = info
         return info
 
+    def load_index(self, kmer_size: int, kmers, bucket_offsets, entry_ids, entry_pos,
+                   repetitive=()) -> N.IndexInfo:
+        """Flye's own VertexIndex (flyehip_load_index): canonical k-mers,
+        bucket offsets, entries (signed id, position) in Flye's bucket order,
+        repetitive canonical k-mers."""
+        km = np.ascontiguousarray(kmers, np.uint64)
+        bo = np.ascontiguousarray(bucket_offsets, np.int64)
+        ei = np.ascontiguousarray(entry_ids, np.int64)
+        ep = np.ascontiguousarray(entry_pos, np.int32)
+        rp = np.ascontiguousarray(repetitive, np.uint64)
+        info = N.IndexInfo()
+        self._check(self.lib.flyehip_load_index(self.ctx, kmer_size, N.ptr(km), len(km), N.ptr(bo), N.ptr(ei),
+                                                N.ptr(ep), N.ptr(rp), len(rp), C.byref(info)))
+        self.index_info = info
+        return info
+
     # -- candidate collection ---------------------------------------------------
     def collect_candidates(self, query_ids) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
         """Emission-order matches + filtered positions (collectCandidateMatches)."""
@@ -228,6 +244,37 @@ class FlyeHip:
             if with_div_stats and not per_query:
                 return rows, roff, divs[:doff[-1]], doff
             return rows, roff
+
+    # -- submitted jobs (flyehip_overlaps_submit / _collect / _release) ---------
+    def submit_overlaps(self, query_ids, config: OverlapConfig, with_div_stats: bool = False) -> int:
+        """Queues getSeqOverlaps for the query records; returns a ticket."""
+        q = np.ascontiguousarray(query_ids, dtype=np.int64)
+        self._pending_params = config.to_c()
+        t = C.c_uint64(0)
+        self._check(self.lib.flyehip_overlaps_submit(self.ctx, N.ptr(q), len(q), C.byref(self._pending_params),
+                                                     int(with_div_stats), C.byref(t)))
+        return t.value
+
+    def collect_overlaps(self, ticket: int, query_index: int, with_div_stats: bool = False):
+        """Rows (and divStats values) of one submitted query record; blocks
+        until its batch is done.  Safe to call from several threads."""
+        cap, dcap = 64, 16
+        need, dneed = C.c_int64(0), C.c_int64(0)
+        while True:
+            rows = np.zeros(cap, N.OVERLAP_DTYPE)
+            divs = np.zeros(dcap, np.float32) if with_div_stats else None
+            rc = self.lib.flyehip_overlaps_collect(self.ctx, ticket, query_index, N.ptr(rows), cap, C.byref(need),
+                                                   N.ptr(divs), dcap, C.byref(dneed))
+            if rc == N.FLYEHIP_ERR_CAPACITY:
+                cap, dcap = max(cap, need.value), max(dcap, dneed.value)
+                continue
+            self._check(rc)
+            if with_div_stats:
+                return rows[:need.value], divs[:dneed.value]
+            return rows[:need.value]
+
+    def release_overlaps(self, ticket: int) -> None:
+        self._check(self.lib.flyehip_overlaps_release(self.ctx, ticket))
 
     def get_seq_overlaps_external(self, seqs_bases: np.ndarray, seqs_offsets: np.ndarray, query_ids,
                                   config: OverlapConfig):

@@ -246,6 +246,28 @@ int flyehip_store_reads(flyehip_ctx* ctx, const uint8_t* bases,
 int flyehip_build_index(flyehip_ctx* ctx, const flyehip_index_params_t* params,
                         flyehip_index_info_t* out_info);
 
+/* Uploads Flye's own VertexIndex instead of building one: the backend of
+ * patch 0005:245-266 reads Flye's `const VertexIndex&` (iterKmerPos,
+ * isRepetitive, kmerFreq; patch 0002 file lines 52-95), so bucket order - and
+ * with it the emission order and std::sort's tie order - is Flye's.
+ *   kmers[i]            canonical k-mer (Kmer::standardForm, 2 bits per base,
+ *                       MSB first) of bucket i, any order, no repeats;
+ *   bucket_offsets      n_kmers + 1 offsets into the entries;
+ *   entry_ids/entry_pos bucket entries in Flye's order: the signed read id and
+ *                       position where the canonical k-mer is spelled (what
+ *                       iterKmerPos returns for the canonical k-mer itself);
+ *   repetitive          canonical k-mers VertexIndex::isRepetitive holds.
+ * The reads must be stored first (flyehip_store_reads: query sequences and
+ * the read lengths that orient entries for non-canonical query windows).
+ * Every entry must lie inside its read; a k-mer that is not canonical or
+ * appears twice is FLYEHIP_ERR_ARG. */
+int flyehip_load_index(flyehip_ctx* ctx, int32_t kmer_size,
+                       const uint64_t* kmers, int64_t n_kmers,
+                       const int64_t* bucket_offsets,
+                       const int64_t* entry_ids, const int32_t* entry_pos,
+                       const uint64_t* repetitive, int64_t n_repetitive,
+                       flyehip_index_info_t* out_info);
+
 /* ---- candidate collection (collectCandidateMatches, patch 0005:245-266) -- */
 
 /* For each query record query_ids[i] (a signed id of a resident read), writes
@@ -286,6 +308,35 @@ int flyehip_overlaps(flyehip_ctx* ctx, const int64_t* query_ids, int32_t n_queri
                      flyehip_overlap_t* rows, int64_t row_capacity,
                      int64_t* row_offsets, int64_t* rows_needed,
                      float* div_stats, int64_t div_capacity, int64_t* div_offsets);
+
+/* Flye's getSeqOverlaps runs once per query record on a pool of threads
+ * (OverlapContainer::findAllOverlaps), while the GPU wants thousands of
+ * query records per batch; the cuFlye backend seam is thread-safe behind a
+ * mutex (patch 0004:105-109, 252).  The submit / collect pair batches those
+ * threads' work:
+ *   flyehip_overlaps_submit  queues query records (resident ids, typically
+ *                            every read strand the thread pool will ask for)
+ *                            and returns at once with a ticket; one library
+ *                            thread runs the queued jobs in submission order,
+ *                            queries_per_batch x batches_in_flight query
+ *                            records at a time, as flyehip_overlaps would;
+ *   flyehip_overlaps_collect any thread, any order: blocks until the batch
+ *                            holding query_index of the ticket's list is done
+ *                            and copies that query's rows (and, with
+ *                            with_div_stats, its divStats values).  A small
+ *                            capacity returns FLYEHIP_ERR_CAPACITY with the
+ *                            counts needed; a failed batch returns its error
+ *                            for every query it held;
+ *   flyehip_overlaps_release waits for the job to finish and frees its rows.
+ * Rows are those flyehip_overlaps returns for the same list.  Close the
+ * context only after every collect has returned. */
+int flyehip_overlaps_submit(flyehip_ctx* ctx, const int64_t* query_ids, int32_t n_queries,
+                            const flyehip_overlap_params_t* params, int32_t with_div_stats,
+                            uint64_t* ticket);
+int flyehip_overlaps_collect(flyehip_ctx* ctx, uint64_t ticket, int32_t query_index,
+                             flyehip_overlap_t* rows, int64_t row_capacity, int64_t* rows_needed,
+                             float* div_stats, int64_t div_capacity, int64_t* divs_needed);
+int flyehip_overlaps_release(flyehip_ctx* ctx, uint64_t ticket);
 
 /* Same for query sequences that are not resident reads (the serving path and
  * read-to-graph alignment): bases/offsets as in flyehip_store_reads,

@@ -731,6 +731,38 @@ int orc_build_index(void* h, int32_t k, int32_t solid_min, int32_t rep_max, floa
 	return 0;
 }
 
+// An index given as tables (flyehip_load_index's arguments): canonical
+// k-mers with their buckets (entries in the given order: signed id and the
+// position where the canonical k-mer is spelled) and the repetitive k-mers.
+int orc_load_index(void* h, int32_t k, const uint64_t* kmers, int64_t n_kmers, const int64_t* offsets,
+				   const int64_t* ids, const int32_t* pos, const uint64_t* rep, int64_t n_rep)
+{
+	Oracle* o = static_cast<Oracle*>(h);
+	delete o->index;
+	Index* ix = new Index();
+	ix->k = k;
+	std::vector<int64_t> order(static_cast<size_t>(n_kmers));
+	for (int64_t i = 0; i < n_kmers; ++i) order[static_cast<size_t>(i)] = i;
+	std::sort(order.begin(), order.end(), [kmers](int64_t a, int64_t b) { return kmers[a] < kmers[b]; });
+	ix->start.push_back(0);
+	for (int64_t i : order)
+	{
+		ix->kmers.push_back(kmers[i]);
+		for (int64_t e = offsets[i]; e < offsets[i + 1]; ++e)
+		{
+			ix->entryId.push_back(toInternal(ids[e]));
+			ix->entryPos.push_back(pos[e]);
+		}
+		ix->start.push_back(static_cast<int64_t>(ix->entryId.size()));
+	}
+	ix->repetitive.assign(rep, rep + n_rep);
+	std::sort(ix->repetitive.begin(), ix->repetitive.end());
+	for (const auto& s : o->targets.fwd) ix->readLen.push_back(static_cast<int32_t>(s.size()));
+	ix->distinct = n_kmers + n_rep;
+	o->index = ix;
+	return 0;
+}
+
 int64_t orc_histogram(void* h, int64_t* out, int64_t cap)
 {
 	const Index& ix = *static_cast<Oracle*>(h)->index;

@@ -88,6 +88,8 @@ def lib() -> C.CDLL:
     L.orc_div_stats.restype = i64
     L.orc_index_tables.argtypes = [P, P, P, P, P, P, P]
     L.orc_index_tables.restype = None
+    L.orc_load_index.argtypes = [P, i32, P, i64, P, P, P, P, i64]
+    L.orc_load_index.restype = C.c_int
     _lib = L
     return L
 
@@ -209,6 +211,15 @@ class Oracle:
         self.info = dict(solid_min=int(out[0]), rep_max=int(out[1]), distinct=int(out[2]),
                          solid=int(out[3]), repetitive=int(out[4]), positions=int(out[5]))
         return self.info
+
+    def load_index(self, k: int, kmers, offsets, ids, pos, repetitive=()):
+        """An index given as tables (the arguments of flyehip_load_index)."""
+        km = np.ascontiguousarray(kmers, np.uint64)
+        off = np.ascontiguousarray(offsets, np.int64)
+        ei = np.ascontiguousarray(ids, np.int64)
+        ep = np.ascontiguousarray(pos, np.int32)
+        rp = np.ascontiguousarray(repetitive, np.uint64)
+        lib().orc_load_index(self.h, k, _p(km), len(km), _p(off), _p(ei), _p(ep), _p(rp), len(rp))
 
     def index_tables(self) -> dict:
         """kmers (ascending), bucket starts, entry ids / positions in bucket
