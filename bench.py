@@ -99,6 +99,8 @@ def parse():
     ap.add_argument("--dp-stress-cpu-lists", type=int, default=100_000, help="DP-stress lists timed on the oracle")
     ap.add_argument("--r2g-scale", type=float, default=0.25, help="read-to-graph graph scale (0 = skip the leg)")
     ap.add_argument("--r2g-cpu-reads", type=int, default=300, help="reads whose quickSeqOverlaps the oracle times")
+    ap.add_argument("--dump-hash-queries", type=int, default=2048,
+                    help="query records whose candidate-record-v1 canonical SHA-256 is streamed (0 = skip)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
     return ap.parse_args()
@@ -280,6 +282,15 @@ def read_to_graph_leg(args, cores: int) -> dict:
     t = time.perf_counter()
     gi, gk, gs, go = ctx.read_align_chains(aln, aoff, R.LONG_EDGE, R.BIG_ALN, prm.chain())
     ch_s = time.perf_counter() - t
+    # the chains as the compact binary payload (CUFRALB0), validated
+    from cuflye_amd import wire as W
+    t = time.perf_counter()
+    payload = W.ralb_encode(gs, go, gi, gk, aln, aoff)
+    enc_s = time.perf_counter() - t
+    ralb = W.validate_ralb(payload, expected_fixtures=reads.n_reads, expected_records=len(gs),
+                           expected_inputs=len(aln))
+    out_ralb = {"bytes": len(payload), "records": int(ralb["records"]), "sha256": ralb["sha256"],
+                "encode_ms": enc_s * 1e3}
     out = {"workload": f"synthetic repeat graph, config 4 at scale {args.r2g_scale} ({len(g4.left)} edges, "
                        f"{reads.n_reads} reads, {len(reads.bases)} bases)",
            "reads": int(reads.n_reads), "alignments": int(len(aln)), "segments": int(len(gs)),
@@ -291,7 +302,8 @@ def read_to_graph_leg(args, cores: int) -> dict:
                                                                   "ms_extract", "ms_total")},
                                   "counts": {k: st[k] for k in ("matches", "groups", "overlaps", "batches")},
                                   "kernels_ms": dict(zip(KERNEL_FAMILIES, [float(x) for x in st["kernel_ms"]]))},
-           "chain": {"ms": ch_s * 1e3, "reads_per_s": reads.n_reads / ch_s, "alignments_per_s": len(aln) / ch_s}}
+           "chain": {"ms": ch_s * 1e3, "reads_per_s": reads.n_reads / ch_s, "alignments_per_s": len(aln) / ch_s},
+           "compact_binary": out_ralb}
     if not args.skip_cpu:
         from oracle import oracle as O
         t = time.perf_counter()
@@ -451,6 +463,27 @@ def main() -> int:
                "single_thread": {"value": len(one) / one_s, "sample": f"first {len(one)} query records"},
                "host": {"nproc": os.cpu_count(), "cpu_model": cpu_model()}}
         log(f"cpu sample {len(sample)} in {cpu_s:.2f}s on {cores} threads; parity={parity}")
+    # candidate-record-v1 canonical SHA-256 of the same workload's candidates,
+    # streamed by the library (join in batches, per-window order, SHA-256);
+    # the oracle's hash of the first query records checks it
+    dump_hash = None
+    if args.dump_hash_queries > 0:
+        dq = ids[: min(args.dump_hash_queries, len(ids))]
+        t = time.perf_counter()
+        hx, nrec = g.candidates_canonical_sha256(dq)
+        dt = time.perf_counter() - t
+        dump_hash = {"query_records": int(len(dq)), "records": int(nrec), "sha256": hx, "seconds": dt,
+                     "records_per_s": nrec / dt, "text_bytes_per_s_estimate": 35.0 * nrec / dt}
+        if not args.skip_cpu:
+            from oracle import oracle as O
+            few = dq[:64]
+            gh, _ = g.candidates_canonical_sha256(few)
+            orc = O.Oracle(reads.bases, reads.offsets)
+            orc.build_index(15)
+            oh, _ = orc.candidates_canonical_sha256(few, threads=cores)
+            dump_hash["parity_sample"] = bool(gh == oh)
+            dump_hash["parity_sample_query_records"] = int(len(few))
+        log(f"dump hash: {dump_hash}")
     g.close()
 
     serving = serving_leg(args, cores) if args.serve_seconds > 0 else None
@@ -497,6 +530,7 @@ def main() -> int:
         "serving": serving,
         "dp_stress": dp_stress,
         "read_to_graph": r2g,
+        "candidate_dump_hash": dump_hash,
         "stage_ms": {k: stats[k] for k in ("ms_join", "ms_sort", "ms_group", "ms_chain", "ms_extract", "ms_total")},
         "counts": {k: stats[k] for k in ("windows", "matches", "groups", "overlaps", "batches", "tie_queries",
                                          "dp_cells")},
@@ -507,7 +541,8 @@ def main() -> int:
                                     ("dp_stress", (dp_stress or {}).get("parity_sample")),
                                     ("read_to_graph.chain", ((r2g or {}).get("parity") or {}).get("chain_all_reads")),
                                     ("read_to_graph.quick_seq_overlaps",
-                                     ((r2g or {}).get("parity") or {}).get("quick_seq_overlaps_sample")))
+                                     ((r2g or {}).get("parity") or {}).get("quick_seq_overlaps_sample")),
+                                    ("candidate_dump_hash", (dump_hash or {}).get("parity_sample")))
               if ok is False]
     if failed:
         out["value_withheld"] = out["value"]

@@ -20,6 +20,7 @@ LIB = PKG / "lib"
 OBJ = LIB / "obj"
 
 HIP_SOURCES = ["pipeline.hip", "msort.hip", "segsort.hip", "chain.hip", "readalign.hip", "runtime.hip"]
+HOST_SOURCES = ["wire.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -70,9 +71,15 @@ def build(verbose: bool = False) -> None:
 
     with ThreadPoolExecutor(max_workers=len(HIP_SOURCES)) as pool:
         objs = list(pool.map(compile_one, HIP_SOURCES))
+    # host-only parts (the canonical dump hash) with the host compiler
+    host_objs = []
+    for name in HOST_SOURCES:
+        obj = OBJ / (name + ".o")
+        _run(["g++", "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", "-o", str(obj), str(CSRC / name)])
+        host_objs.append(str(obj))
     tmp = LIB / "libflyehip.so.tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
-          *[str(o) for o in objs], str(blob)])
+          *[str(o) for o in objs], *host_objs, str(blob), "-lcrypto"])
     os.replace(tmp, LIB / "libflyehip.so")
     stmp = LIB / "libflyesynth.so.tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", str(stmp),

@@ -35,6 +35,7 @@
 #include "readalign.hpp"
 #include "ktimer.hpp"
 #include "segsort.hpp"
+#include "wire.hpp"
 
 extern "C" const unsigned char fh_logf_table[];
 extern "C" const unsigned char fh_logf_table_end[];
@@ -1794,6 +1795,47 @@ int flyehip_collect_candidates(flyehip_ctx* c, const int64_t* query_ids, int32_t
 		download(L, matches, dc, static_cast<size_t>(bo.matches) * sizeof(flyehip_candidate_t));
 		download(L, filtered, ja.filtered, static_cast<size_t>(bo.filtered) * 4);
 		sync(L);
+	});
+}
+
+int flyehip_candidates_canonical_sha256(flyehip_ctx* c, const int64_t* query_ids, int32_t n, uint8_t* sha256,
+										int64_t* records)
+{
+	return guarded(c, [&]() {
+		Lane* L = c->lanes[0].get();
+		if (!sha256 || n < 0 || (n > 0 && !query_ids)) fail(FLYEHIP_ERR_ARG, "invalid arguments");
+		resetStats(c);
+		std::vector<int64_t> ids(query_ids, query_ids + n);
+		std::sort(ids.begin(), ids.end());
+		(void)residentQueries(c, ids.data(), n);
+		CanonSha256 h;
+		// query records per join batch: about 30 M matches (1 GB of records) at
+		// the 20 kb ONT-like reads' 115 k matches per record
+		const int32_t kBatch = 256;
+		const int threads = std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
+		std::vector<flyehip_candidate_t> host;
+		std::vector<int64_t> qmo;
+		for (int32_t b0 = 0; b0 < n; b0 += kBatch)
+		{
+			const int32_t nb = std::min(kBatch, n - b0);
+			std::vector<JoinQuery> qs = residentQueries(c, ids.data() + b0, nb);
+			JoinArgs ja;
+			const BatchOut bo = runJoin(L, storeSeqs(c), qs, ja);
+			qmo.resize(static_cast<size_t>(nb) + 1);
+			download(L, qmo.data(), L->bQMatchOff.p, qmo.size() * sizeof(int64_t));
+			int64_t* dIds = ensureT<int64_t>(c, L->bQueryIds, static_cast<size_t>(nb) + 1);
+			upload(L, dIds, ids.data() + b0, static_cast<size_t>(nb) * sizeof(int64_t));
+			flyehip_candidate_t* dc = ensureT<flyehip_candidate_t>(c, L->bCand, static_cast<size_t>(bo.matches) + 1);
+			HIPCHECK(launchWriteCandidates(ja, static_cast<const int64_t*>(L->bQMatchOff.p), dIds, nullptr, dc,
+										   bo.matches, L->stream));
+			host.resize(static_cast<size_t>(bo.matches));
+			download(L, host.data(), dc, host.size() * sizeof(flyehip_candidate_t));
+			sync(L);
+			h.addQueries(host.data(), qmo.data(), nb, threads);
+		}
+		if (!h.finish(sha256)) fail(FLYEHIP_ERR_STATE, "canonical hash: %s", h.error().c_str());
+		if (records) *records = h.records();
+		c->stats.queries = n;
 	});
 }
 

@@ -31,7 +31,7 @@ EXPORTED = (
     "flyehip_overlaps_external", "flyehip_overlaps_per_query",
     "flyehip_chain_lists", "flyehip_read_align_chains", "flyehip_divergence_logf",
     "flyehip_read_align_chains_noalloc", "flyehip_load_index", "flyehip_overlaps_submit",
-    "flyehip_overlaps_collect", "flyehip_overlaps_release",
+    "flyehip_overlaps_collect", "flyehip_overlaps_release", "flyehip_candidates_canonical_sha256",
 )
 
 
@@ -172,6 +172,7 @@ def load_flyehip(path: Path | None = None) -> C.CDLL:
         "flyehip_overlaps_submit": ([_P, _P, i32, C.POINTER(OverlapParams), i32, C.POINTER(u64)], C.c_int),
         "flyehip_overlaps_collect": ([_P, u64, i32, _P, i64, _P, _P, i64, _P], C.c_int),
         "flyehip_overlaps_release": ([_P, u64], C.c_int),
+        "flyehip_candidates_canonical_sha256": ([_P, _P, i32, _P, C.POINTER(i64)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -195,6 +195,15 @@ class FlyeHip:
             self._check(rc)
             return m[:need_m.value], moff, f[:need_f.value], foff
 
+    def candidates_canonical_sha256(self, query_ids) -> tuple[str, int]:
+        """candidate-record-v1 canonical SHA-256 (hex) and record count of the
+        query records' matches, streamed (no dump held)."""
+        q = np.ascontiguousarray(query_ids, dtype=np.int64)
+        out = (C.c_uint8 * 32)()
+        n = C.c_int64(0)
+        self._check(self.lib.flyehip_candidates_canonical_sha256(self.ctx, N.ptr(q), len(q), out, C.byref(n)))
+        return bytes(out).hex(), n.value
+
     def collect_sorted_candidates(self, query_ids) -> tuple[np.ndarray, np.ndarray]:
         """Matches after Flye's std::sort by (extId, curPos) (candidate-record-v1 order)."""
         q = np.ascontiguousarray(query_ids, dtype=np.int64)

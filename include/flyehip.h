@@ -284,6 +284,17 @@ int flyehip_collect_candidates(flyehip_ctx* ctx, const int64_t* query_ids,
                                int32_t* filtered, int64_t filtered_capacity,
                                int64_t* filtered_offsets, int64_t* filtered_needed);
 
+/* The candidate-record-v1 canonical SHA-256 of the query records' matches
+ * (docs/abi/candidate-record-v1.md:31-68; the hash
+ * tools/validate_candidate_dump.py:76-134 computes over the dump's records
+ * in ascending tuple order), streamed: the join runs in batches of query
+ * records in ascending id order, each window's matches are ordered by
+ * (target_id, target_pos) and formatted, and the text goes through SHA-256
+ * without the dump being sorted or held.  sha256 receives 32 bytes;
+ * *records the record count. */
+int flyehip_candidates_canonical_sha256(flyehip_ctx* ctx, const int64_t* query_ids,
+                                        int32_t n_queries, uint8_t* sha256, int64_t* records);
+
 /* Same, with the matches in Flye's sorted order: std::sort by
  * (internal extId, curPos) with libstdc++'s tie order
  * (patch 0001:88-91); this is the order candidate-record-v1 dumps. */

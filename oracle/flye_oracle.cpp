@@ -40,6 +40,8 @@
 
 #include "../include/flyehip.h"
 
+#include <openssl/evp.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -932,6 +934,79 @@ int64_t orc_read_align_chains(const flyehip_edge_alignment_t* alns, const int64_
 		seg_offsets[r + 1] = total;
 	}
 	return total;
+}
+
+// candidate-record-v1 canonical SHA-256 of the query records' candidates
+// (tools/validate_candidate_dump.py:76-134: records as decimal TSV lines,
+// sorted as numeric tuples, the text hashed).  Each query's records are
+// sorted whole here; query records are taken in ascending id order and
+// chunks of them are collected on `threads` workers, hashed in order.
+int orc_candidates_canonical_sha256(void* h, const int64_t* query_ids, int32_t n, int32_t threads,
+									uint8_t* out, int64_t* records)
+{
+	Oracle* o = static_cast<Oracle*>(h);
+	std::vector<int64_t> ids(query_ids, query_ids + n);
+	std::sort(ids.begin(), ids.end());
+	EVP_MD_CTX* md = EVP_MD_CTX_new();
+	if (!md || EVP_DigestInit_ex(md, EVP_sha256(), nullptr) != 1) return -1;
+	int64_t total = 0;
+	const int32_t chunk = 64;
+	std::vector<std::string> text(static_cast<size_t>(chunk));
+	std::vector<int64_t> counts(static_cast<size_t>(chunk));
+	for (int32_t c0 = 0; c0 < n; c0 += chunk)
+	{
+		const int32_t m = std::min(chunk, n - c0);
+		parallelFor(m, threads, [&](int32_t i) {
+			const int64_t qid = ids[static_cast<size_t>(c0 + i)];
+			std::string q = querySeq(*o, qid);
+			std::vector<Match> ms;
+			std::vector<int32_t> f;
+			collect(*o->index, toInternal(qid), q, ms, f);
+			struct Rec
+			{
+				int32_t qpos;
+				uint64_t kmer;
+				int64_t tid;
+				int32_t tpos;
+			};
+			std::vector<Rec> rs;
+			rs.reserve(ms.size());
+			for (const auto& m2 : ms) rs.push_back({m2.curPos, m2.kmer, toSigned(m2.extId), m2.extPos});
+			std::sort(rs.begin(), rs.end(), [](const Rec& a, const Rec& b) {
+				if (a.qpos != b.qpos) return a.qpos < b.qpos;
+				if (a.kmer != b.kmer) return a.kmer < b.kmer;
+				if (a.tid != b.tid) return a.tid < b.tid;
+				return a.tpos < b.tpos;
+			});
+			std::string& s = text[static_cast<size_t>(i)];
+			s.clear();
+			for (const auto& r : rs)
+			{
+				s += std::to_string(qid);
+				s += '\t';
+				s += std::to_string(r.qpos);
+				s += '\t';
+				s += std::to_string(r.kmer);
+				s += '\t';
+				s += std::to_string(r.tid);
+				s += '\t';
+				s += std::to_string(r.tpos);
+				s += r.tid > 0 ? "\t+\n" : "\t-\n";
+			}
+			counts[static_cast<size_t>(i)] = static_cast<int64_t>(rs.size());
+		});
+		for (int32_t i = 0; i < m; ++i)
+		{
+			const std::string& s = text[static_cast<size_t>(i)];
+			if (!s.empty()) EVP_DigestUpdate(md, s.data(), s.size());
+			total += counts[static_cast<size_t>(i)];
+		}
+	}
+	unsigned int len = 0;
+	const int ok = EVP_DigestFinal_ex(md, out, &len);
+	EVP_MD_CTX_free(md);
+	if (records) *records = total;
+	return ok == 1 && len == 32 ? 0 : -1;
 }
 
 // Real libstdc++ std::sort of 64-bit keys, returning the permutation: the

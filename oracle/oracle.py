@@ -90,6 +90,8 @@ def lib() -> C.CDLL:
     L.orc_index_tables.restype = None
     L.orc_load_index.argtypes = [P, i32, P, i64, P, P, P, P, i64]
     L.orc_load_index.restype = C.c_int
+    L.orc_candidates_canonical_sha256.argtypes = [P, P, i32, i32, P, P]
+    L.orc_candidates_canonical_sha256.restype = C.c_int
     _lib = L
     return L
 
@@ -220,6 +222,15 @@ class Oracle:
         ep = np.ascontiguousarray(pos, np.int32)
         rp = np.ascontiguousarray(repetitive, np.uint64)
         lib().orc_load_index(self.h, k, _p(km), len(km), _p(off), _p(ei), _p(ep), _p(rp), len(rp))
+
+    def candidates_canonical_sha256(self, query_ids, threads: int = 1) -> tuple[str, int]:
+        """candidate-record-v1 canonical SHA-256 (hex) and record count."""
+        q = np.ascontiguousarray(query_ids, np.int64)
+        out = (C.c_uint8 * 32)()
+        n = C.c_int64(0)
+        if lib().orc_candidates_canonical_sha256(self.h, _p(q), len(q), threads, out, C.byref(n)) != 0:
+            raise RuntimeError("SHA-256 failed")
+        return bytes(out).hex(), n.value
 
     def index_tables(self) -> dict:
         """kmers (ascending), bucket starts, entry ids / positions in bucket
