@@ -17,7 +17,11 @@ Beside the headline number the JSON line carries:
   per_query     - the plain per-query launch path on a sample;
   serving       - p50/p99 request latency and reads/s of the continuous-
                   batching loop over a resident 200 Mb genome index, Poisson
-                  arrivals (disable with --serve-seconds 0).
+                  arrivals at 1, 2 and 4 requests/s of 500-20,000 reads
+                  (disable with --serve-seconds 0);
+  dp_stress, read_to_graph, candidate_dump_hash - the chaining-DP stress
+                  lists, read-to-graph chaining and the streamed
+                  candidate-record-v1 canonical SHA-256.
 Parity: the CPU sample's rows must equal the GPU's bit for bit.
 """
 from __future__ import annotations
@@ -86,15 +90,17 @@ def parse():
     ap.add_argument("--batch", type=int, default=0, help="queries per GPU batch (0 = autotune)")
     ap.add_argument("--dp-lanes", type=int, default=0, help="DP lanes per group (0 = autotune)")
     ap.add_argument("--flights", type=int, default=0, help="batches in flight (0 = autotune)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
-    # 0.5 requests/s (about 2,600 reads/s offered) is what the loop sustains on
-    # the 200 Mb index; 2-4/s exceeds its throughput and only measures queueing
-    ap.add_argument("--serve-rate", type=float, default=0.5, help="Poisson request rate (1/s)")
-    ap.add_argument("--serve-iteration-reads", type=int, default=4096, help="reads per serving iteration")
+    ap.add_argument("--serve-seconds", type=float, default=12.0, help="arrival window per rate (0 = skip the leg)")
+    ap.add_argument("--serve-rates", type=str, default="1,2,4", help="Poisson request rates (1/s), one run each")
+    ap.add_argument("--serve-iteration-reads", type=int, default=8192, help="reads per serving iteration")
     ap.add_argument("--serve-genome", type=int, default=200_000_000)
+    # two contexts (two iterations in flight) measured slower than one on
+    # the 200 Mb index: their kernels share the GPU and each iteration shrinks
+    ap.add_argument("--serve-contexts", type=int, default=1,
+                    help="serving: contexts holding the index (one iteration in flight each)")
     # one GPU batch per serving iteration: per-batch fixed costs are paid once
     ap.add_argument("--serve-batch", type=int, default=8192, help="serving: query records per GPU batch")
-    ap.add_argument("--serve-flights", type=int, default=1, help="serving: batches in flight")
+    ap.add_argument("--serve-dp-lanes", type=int, default=64, help="serving: DP lanes per group")
     ap.add_argument("--dp-stress-lists", type=int, default=2_000_000, help="DP-stress hit lists (0 = skip the leg)")
     ap.add_argument("--dp-stress-cpu-lists", type=int, default=100_000, help="DP-stress lists timed on the oracle")
     ap.add_argument("--r2g-scale", type=float, default=0.25, help="read-to-graph graph scale (0 = skip the leg)")
@@ -133,48 +139,64 @@ def serving_leg(args, cores: int) -> dict:
     pool_b, pool_o = S.reads_from(genome, 20000, seed=55)
     log(f"serve: genome {len(genome)} bp in {n_contigs} contigs, read pool {len(pool_o) - 1} "
         f"({time.time() - t:.1f}s)")
-    ctx = FlyeHip(device=0)
-    ctx.set_tuning(queries_per_batch=args.serve_batch, batches_in_flight=args.serve_flights)
-    t = time.time()
-    ctx.store_reads(genome, offs)
-    info = ctx.build_index(15, solid_min=1, rep_max=64)
-    log(f"serve: index {info.indexed_positions} positions, {info.repetitive_kmers} repetitive k-mers, "
-        f"{time.time() - t:.1f}s")
+    ctxs = []
+    for k in range(max(1, args.serve_contexts)):
+        ctx = FlyeHip(device=0)
+        # one batch per iteration, wide DP lookback tiles: an iteration's
+        # time is its longest chaining DP
+        ctx.set_tuning(dp_lanes_per_group=args.serve_dp_lanes, queries_per_batch=args.serve_batch,
+                       batches_in_flight=1)
+        t = time.time()
+        ctx.store_reads(genome, offs)
+        info = ctx.build_index(15, solid_min=1, rep_max=64)
+        log(f"serve: context {k} index {info.indexed_positions} positions, {info.repetitive_kmers} repetitive "
+            f"k-mers, {info.device_bytes / 2**30:.2f} GiB, {time.time() - t:.1f}s")
+        ctxs.append(ctx)
     del genome
     cfg = OverlapConfig(kmer_size=15, minimum_overlap=1000)
-    srv = OverlapServer(ctx, cfg, admission_bytes=64 << 30, max_iteration_reads=args.serve_iteration_reads)
-    srv.start()
-    rng = np.random.default_rng(7)
-    t0 = time.perf_counter()
-    nid = 1_000_000_001
-    reqs = []
-    next_t = t0
     pool_n = len(pool_o) - 1
-    while True:
-        next_t += rng.exponential(1.0 / args.serve_rate)
-        if next_t - t0 > args.serve_seconds:
-            break
-        while time.perf_counter() < next_t:
-            time.sleep(0.0005)
-        n = int(np.exp(rng.uniform(np.log(500), np.log(20000))))
-        start = int(rng.integers(0, pool_n - n + 1)) if n < pool_n else 0
-        n = min(n, pool_n)
-        b = pool_b[pool_o[start]:pool_o[start + n]]
-        o = pool_o[start:start + n + 1] - pool_o[start]
-        ids = np.arange(nid, nid + n, dtype=np.int64)
-        nid += n
-        reqs.append(srv.submit(Request(len(reqs), b, o, ids)))
-    for r in reqs:
-        r.done_event.wait(600)
-    srv.stop()
-    rep = srv.latency_report()
-    rep["rate_per_s"] = args.serve_rate
-    rep["seconds"] = args.serve_seconds
-    rep["request_reads"] = "log-uniform 500-20000"
-    rep["iteration_reads"] = args.serve_iteration_reads
-    rep["batch"] = {"queries_per_batch": args.serve_batch, "batches_in_flight": args.serve_flights}
-    ctx.close()
-    return rep
+    runs = []
+    nid = 1_000_000_001
+    for rate in [float(x) for x in args.serve_rates.split(",") if x.strip()]:
+        srv = OverlapServer(ctxs, cfg, admission_bytes=64 << 30, max_iteration_reads=args.serve_iteration_reads)
+        srv.start()
+        rng = np.random.default_rng(7)
+        t0 = time.perf_counter()
+        reqs = []
+        next_t = t0
+        while True:
+            next_t += rng.exponential(1.0 / rate)
+            if next_t - t0 > args.serve_seconds:
+                break
+            while time.perf_counter() < next_t:
+                time.sleep(0.0005)
+            n = int(np.exp(rng.uniform(np.log(500), np.log(20000))))
+            n = min(n, pool_n)
+            start = int(rng.integers(0, pool_n - n + 1))
+            b = pool_b[pool_o[start]:pool_o[start + n]]
+            o = pool_o[start:start + n + 1] - pool_o[start]
+            ids = np.arange(nid, nid + n, dtype=np.int64)
+            nid += n
+            reqs.append(srv.submit(Request(len(reqs), b, o, ids)))
+        offered = sum(r.n_reads for r in reqs) / args.serve_seconds
+        for r in reqs:
+            r.done_event.wait(600)
+        srv.stop()
+        rep = srv.latency_report()
+        rep["rate_per_s"] = rate
+        rep["offered_reads_per_s"] = offered
+        # sustained: the reads/s the server delivers while busy reach the
+        # offered reads/s
+        rep["sustained"] = bool(rep.get("capacity_reads_per_s", 0) >= offered)
+        log(f"serve: rate {rate}/s: {rep}")
+        runs.append(rep)
+    for ctx in ctxs:
+        ctx.close()
+    return {"runs": runs, "seconds_per_rate": args.serve_seconds, "request_reads": "log-uniform 500-20000",
+            "iteration_reads": args.serve_iteration_reads, "contexts": len(ctxs),
+            "tuning": {"dp_lanes": args.serve_dp_lanes, "queries_per_batch": args.serve_batch,
+                       "batches_in_flight": 1},
+            "p99_target_ms": 250.0}
 
 
 def dp_stress_leg(args, cores: int, dp_lanes: int) -> dict:
