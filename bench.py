@@ -103,8 +103,8 @@ def parse():
     ap.add_argument("--serve-dp-lanes", type=int, default=64, help="serving: DP lanes per group")
     ap.add_argument("--dp-stress-lists", type=int, default=2_000_000, help="DP-stress hit lists (0 = skip the leg)")
     ap.add_argument("--dp-stress-cpu-lists", type=int, default=100_000, help="DP-stress lists timed on the oracle")
-    ap.add_argument("--r2g-scale", type=float, default=0.25, help="read-to-graph graph scale (0 = skip the leg)")
-    ap.add_argument("--r2g-cpu-reads", type=int, default=300, help="reads whose quickSeqOverlaps the oracle times")
+    ap.add_argument("--r2g-scale", type=float, default=1.0, help="read-to-graph graph scale (0 = skip the leg)")
+    ap.add_argument("--r2g-cpu-reads", type=int, default=3000, help="reads whose quickSeqOverlaps the oracle times")
     ap.add_argument("--dump-hash-queries", type=int, default=2048,
                     help="query records whose candidate-record-v1 canonical SHA-256 is streamed (0 = skip)")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -259,9 +259,9 @@ def dp_stress_leg(args, cores: int, dp_lanes: int) -> dict:
 
 def read_to_graph_leg(args, cores: int) -> dict:
     """Read-to-graph alignment on the synthetic repeat graph (config 4 at
-    --r2g-scale): edge-sequence index, quickSeqOverlaps of every read (the
-    read-to-graph detector shape of getSeqOverlaps, reads handed over from the
-    host), the chain-input filter + sort and chainReadAlignments
+    --r2g-scale): edge-sequence minimizer index, quickSeqOverlaps of every
+    read (every read k-mer joined, the read-to-graph detector shape of
+    getSeqOverlaps, reads handed over from the host), the chain-input filter + sort and chainReadAlignments
     (flyehip_read_align_chains).  The oracle runs the chaining on all reads
     and quickSeqOverlaps on the first reads, as CPU baseline and parity check."""
     import cuflye_amd.synth as S
@@ -279,7 +279,13 @@ def read_to_graph_leg(args, cores: int) -> dict:
     ids = np.arange(1, reads.n_reads + 1, dtype=np.int64)
     ctx = FlyeHip(device=0)
     ctx.store_reads(g4.edges_bases, g4.edges_offsets)
-    ctx.build_index(prm.kmer_size, solid_min=1, rep_max=1 << 20)
+    ctx.build_index_minimizers(prm.kmer_size, prm.minimizer_window, solid_min=1, rep_max=1 << 20)  # warm
+    t = time.perf_counter()
+    info = ctx.build_index_minimizers(prm.kmer_size, prm.minimizer_window, solid_min=1, rep_max=1 << 20)
+    index_s = time.perf_counter() - t
+    edge_bases = int(len(g4.edges_bases))
+    log(f"read-to-graph: minimizer index (window {prm.minimizer_window}) {info.indexed_positions} positions of "
+        f"{edge_bases} edge bases, {index_s * 1e3:.1f} ms")
     ref = ctx.get_seq_overlaps_external(reads.bases, reads.offsets, ids, cfg)  # warm
     # launch knobs for this workload (long read-to-edge chains): batch size,
     # batches in flight, lookback tile width; rows must not change
@@ -313,8 +319,12 @@ def read_to_graph_leg(args, cores: int) -> dict:
                            expected_inputs=len(aln))
     out_ralb = {"bytes": len(payload), "records": int(ralb["records"]), "sha256": ralb["sha256"],
                 "encode_ms": enc_s * 1e3}
-    out = {"workload": f"synthetic repeat graph, config 4 at scale {args.r2g_scale} ({len(g4.left)} edges, "
-                       f"{reads.n_reads} reads, {len(reads.bases)} bases)",
+    elen = np.diff(g4.edges_offsets)
+    out = {"workload": f"synthetic repeat graph, config 4 at scale {args.r2g_scale} ({len(g4.left)} edges of "
+                       f"{int(elen.min())}-{int(elen.max())} bp, {reads.n_reads} reads, {len(reads.bases)} bases)",
+           "edge_index": {"kind": "minimizer", "window": prm.minimizer_window, "kmer_size": prm.kmer_size,
+                          "edge_bases": edge_bases, "indexed_positions": int(info.indexed_positions),
+                          "build_ms": index_s * 1e3},
            "reads": int(reads.n_reads), "alignments": int(len(aln)), "segments": int(len(gs)),
            "quick_seq_overlaps": {"ms": ov_s * 1e3, "reads_per_s": reads.n_reads / ov_s,
                                   "note": "reads handed over from host memory per call",
@@ -334,7 +344,7 @@ def read_to_graph_leg(args, cores: int) -> dict:
         chain_parity = bool(np.array_equal(gk, ek) and np.array_equal(gi, ei) and np.array_equal(go, eo)
                             and gs.tobytes() == es.tobytes())
         orc = O.Oracle(g4.edges_bases, g4.edges_offsets)
-        orc.build_index(prm.kmer_size, solid_min=1, rep_max=1 << 20)
+        orc.build_index(prm.kmer_size, solid_min=1, rep_max=1 << 20, minimizer_window=prm.minimizer_window)
         m = min(reads.n_reads, args.r2g_cpu_reads)
         sample = ids[:m]
         sb = reads.bases[: reads.offsets[m]]

@@ -235,6 +235,10 @@ __host__ __device__ inline uint64_t hashKmer(uint64_t x)
 	return x;
 }
 
+// Minimizer order of a canonical k-mer: the same mixer, so the order is a
+// fixed pseudo-random permutation of k-mers (ties only between equal k-mers).
+__host__ __device__ inline uint64_t minimizerOrder(uint64_t canonicalKmer) { return hashKmer(canonicalKmer); }
+
 constexpr uint64_t kEmptyKey = ~0ULL;
 constexpr uint64_t kRepMark = ~0ULL;
 constexpr int kCountBits = 24;

@@ -214,12 +214,63 @@ __global__ void classifyKernel(HashTable t, int solidMin, int repMax, int32_t* s
 	t.cursor[h] = 0;
 }
 
+__device__ inline bool indexed(uint64_t v) { return v != 0 && v != kRepMark; }
+
+// A solid, non-repetitive k-mer with no bucket entry (a minimizer index where
+// the k-mer is nowhere a minimizer) is not in the index: value 0.
 __global__ void assignKernel(HashTable t, const int32_t* sizes, const int64_t* starts)
 {
 	const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (h > t.mask) return;
 	const int32_t s = sizes[h];
 	if (s > 0) t.vals[h] = (static_cast<uint64_t>(starts[h]) << kCountBits) | static_cast<uint64_t>(s);
+	else if (indexed(t.vals[h])) t.vals[h] = 0;
+}
+
+// Minimizer windows (buildIndexMinimizers' selection, as defined by this
+// library): over every run of w consecutive k-mer positions of a read (one
+// run of all its positions when it has fewer than w), the position whose
+// canonical k-mer has the smallest minimizerOrder, the leftmost on equal
+// values, is a minimizer.  One thread per window start; mask[window] = 1 at
+// every selected position (equal stores from several threads).
+__global__ void minimizerMaskKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin, int k, int w, uint8_t* mask)
+{
+	const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (g >= nWin) return;
+	const int r = findSegWave(winOff, reads.n, g);
+	const int64_t w0 = winOff[r];
+	const int32_t nk = static_cast<int32_t>(winOff[r + 1] - w0);
+	const int32_t s = static_cast<int32_t>(g - w0);
+	const int32_t last = nk >= w ? nk - w : 0;  // last window start
+	if (s > last) return;
+	const int32_t e = min(nk, s + w);
+	int32_t best = s;
+	uint64_t bestOrd = ~0ULL;
+	for (int32_t p = s; p < e; ++p)
+	{
+		bool flip;
+		const uint64_t o = minimizerOrder(canonical(kmerAt(reads.words, reads.wordOff[r], p, k), k, flip));
+		if (p == s || o < bestOrd)
+		{
+			bestOrd = o;
+			best = p;
+		}
+	}
+	mask[w0 + best] = 1;
+}
+
+// Bucket sizes of a minimizer index: selected positions of indexed k-mers.
+__global__ void maskedSizesKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
+								  const uint8_t* mask, int32_t* sizes)
+{
+	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (w >= nWin || !mask[w]) return;
+	const int r = findSegWave(winOff, reads.n, w);
+	const int32_t pos = static_cast<int32_t>(w - winOff[r]);
+	bool flip;
+	const uint64_t c = canonical(kmerAt(reads.words, reads.wordOff[r], pos, k), k, flip);
+	const int64_t h = findSlot(t.keys, t.mask, c);
+	if (h >= 0 && indexed(t.vals[h])) atomicAdd(&sizes[h], 1);
 }
 
 __global__ void loadKeysKernel(HashTable t, const uint64_t* keys, const uint64_t* vals, int64_t n, uint32_t* err)
@@ -247,13 +298,11 @@ __global__ void loadKeysKernel(HashTable t, const uint64_t* keys, const uint64_t
 	}
 }
 
-__device__ inline bool indexed(uint64_t v) { return v != 0 && v != kRepMark; }
-
 __global__ void fillKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
-						   uint64_t* ordKeys)
+						   const uint8_t* mask, uint64_t* ordKeys)
 {
 	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (w >= nWin) return;
+	if (w >= nWin || (mask && !mask[w])) return;
 	const int r = findSegWave(winOff, reads.n, w);
 	const int32_t pos = static_cast<int32_t>(w - winOff[r]);
 	bool flip;
@@ -416,6 +465,46 @@ __device__ inline void transformEntry(uint64_t e, bool g, const int32_t* targetL
 	}
 }
 
+// Self entry of a window inside its bucket: -1 when the window cannot hit
+// itself.  Resident reads: the bucket is in (read, forward position) order, so
+// the entry stored for the query read's own window is found by binary search
+// (absent when the index holds minimizer positions only and this one is
+// not); other queries scan the bucket.
+__device__ inline int selfIndex(const JoinArgs& a, const JoinQuery& Q, int32_t pos, bool g, uint64_t start, int n)
+{
+	if (Q.selfMode == 0) return -1;
+	if (Q.selfMode == 2)
+	{
+		for (int i = 0; i < n; ++i)
+		{
+			uint32_t id;
+			int32_t p;
+			transformEntry(a.entries[start + i], g, a.targetLen, a.k, id, p);
+			if (id == Q.iid && p == pos) return i;
+		}
+		return -1;
+	}
+	const uint32_t r = static_cast<uint32_t>(Q.seq);
+	const int32_t pf = Q.strand ? pos : Q.len - pos - a.k;
+	int lo = 0, hi = n;
+	while (lo < hi)
+	{
+		const int mid = (lo + hi) >> 1;
+		const uint64_t e = a.entries[start + mid];
+		const uint32_t id = static_cast<uint32_t>(e >> 32);
+		const uint32_t rd = id >> 1;
+		const int32_t ep = static_cast<int32_t>(static_cast<uint32_t>(e));
+		const int32_t fp = (id & 1U) ? a.targetLen[rd] - ep - a.k : ep;
+		if (rd < r || (rd == r && fp < pf)) lo = mid + 1;
+		else hi = mid;
+	}
+	if (lo >= n) return -1;
+	uint32_t id;
+	int32_t p;
+	transformEntry(a.entries[start + lo], g, a.targetLen, a.k, id, p);
+	return (id == Q.iid && p == pos) ? lo : -1;
+}
+
 __global__ void joinCountKernel(JoinArgs a)
 {
 	const int64_t w = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -458,6 +547,10 @@ __global__ void joinCountKernel(JoinArgs a)
 					if (id == Q.iid && p == wk.pos) self = 1;
 				}
 			}
+			else if (Q.selfMode == 3)
+			{
+				self = selfIndex(a, Q, wk.pos, wk.g, v >> kCountBits, count) >= 0 ? 1 : 0;
+			}
 			count -= self;
 			info = static_cast<uint32_t>(h) | (wk.g ? 0x80000000U : 0U);
 		}
@@ -465,45 +558,6 @@ __global__ void joinCountKernel(JoinArgs a)
 	a.winCount[w] = count;
 	a.winFilt[w] = filt;
 	a.winInfo[w] = info;
-}
-
-// Self entry of a window inside its bucket: -1 when the window cannot hit
-// itself.  Resident reads: the bucket is in (read, forward position) order, so
-// the entry stored for the query read's own window is found by binary search;
-// other queries scan the bucket.
-__device__ inline int selfIndex(const JoinArgs& a, const JoinQuery& Q, int32_t pos, bool g, uint64_t start, int n)
-{
-	if (Q.selfMode == 0) return -1;
-	if (Q.selfMode == 2)
-	{
-		for (int i = 0; i < n; ++i)
-		{
-			uint32_t id;
-			int32_t p;
-			transformEntry(a.entries[start + i], g, a.targetLen, a.k, id, p);
-			if (id == Q.iid && p == pos) return i;
-		}
-		return -1;
-	}
-	const uint32_t r = static_cast<uint32_t>(Q.seq);
-	const int32_t pf = Q.strand ? pos : Q.len - pos - a.k;
-	int lo = 0, hi = n;
-	while (lo < hi)
-	{
-		const int mid = (lo + hi) >> 1;
-		const uint64_t e = a.entries[start + mid];
-		const uint32_t id = static_cast<uint32_t>(e >> 32);
-		const uint32_t rd = id >> 1;
-		const int32_t ep = static_cast<int32_t>(static_cast<uint32_t>(e));
-		const int32_t fp = (id & 1U) ? a.targetLen[rd] - ep - a.k : ep;
-		if (rd < r || (rd == r && fp < pf)) lo = mid + 1;
-		else hi = mid;
-	}
-	if (lo >= n) return -1;
-	uint32_t id;
-	int32_t p;
-	transformEntry(a.entries[start + lo], g, a.targetLen, a.k, id, p);
-	return (id == Q.iid && p == pos) ? lo : -1;
 }
 
 // One wave fills the matches of 64 consecutive windows together: the raw
@@ -757,10 +811,26 @@ hipError_t launchAssign(HashTable t, const int32_t* sizes, const int64_t* starts
 }
 
 hipError_t launchFill(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
-					  uint64_t* ordKeys, hipStream_t s)
+					  const uint8_t* mask, uint64_t* ordKeys, hipStream_t s)
 {
 	if (nWin == 0) return hipSuccess;
-	hipLaunchKernelGGL(fillKernel, gridFor(nWin, 256), dim3(256), 0, s, reads, winOff, nWin, k, t, ordKeys);
+	hipLaunchKernelGGL(fillKernel, gridFor(nWin, 256), dim3(256), 0, s, reads, winOff, nWin, k, t, mask, ordKeys);
+	return hipGetLastError();
+}
+
+hipError_t launchMinimizerMask(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, int w,
+							   uint8_t* mask, hipStream_t s)
+{
+	if (nWin == 0) return hipSuccess;
+	hipLaunchKernelGGL(minimizerMaskKernel, gridFor(nWin, 256), dim3(256), 0, s, reads, winOff, nWin, k, w, mask);
+	return hipGetLastError();
+}
+
+hipError_t launchMaskedSizes(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
+							 const uint8_t* mask, int32_t* sizes, hipStream_t s)
+{
+	if (nWin == 0) return hipSuccess;
+	hipLaunchKernelGGL(maskedSizesKernel, gridFor(nWin, 256), dim3(256), 0, s, reads, winOff, nWin, k, t, mask, sizes);
 	return hipGetLastError();
 }
 

@@ -48,8 +48,18 @@ hipError_t launchAssign(HashTable t, const int32_t* sizes, const int64_t* starts
 // caller); *err is set when a key is inserted twice.
 hipError_t launchLoadKeys(HashTable t, const uint64_t* keys, const uint64_t* vals, int64_t n, uint32_t* err,
 						  hipStream_t s);
+// Bucket fill of every window, or of the windows with mask[w] set (null:
+// all).
 hipError_t launchFill(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
-					  uint64_t* ordKeys, hipStream_t s);
+					  const uint8_t* mask, uint64_t* ordKeys, hipStream_t s);
+// mask[window] = 1 at every minimizer position (window of w k-mers; mask
+// zeroed by the caller).
+hipError_t launchMinimizerMask(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, int w,
+							   uint8_t* mask, hipStream_t s);
+// sizes[h] += 1 per masked window of an indexed k-mer (sizes zeroed by the
+// caller).
+hipError_t launchMaskedSizes(const PackedSeqs& reads, const int64_t* winOff, int64_t nWin, int k, HashTable t,
+							 const uint8_t* mask, int32_t* sizes, hipStream_t s);
 // Puts every bucket in (read, pos) order and writes its entries: buckets of at
 // most 32 entries by one thread each, larger ones (listed in bigList, at most
 // bigCap; *bigCount zeroed by the caller) by one 1,024-thread block each.
@@ -63,7 +73,8 @@ struct JoinQuery
 	int32_t seq;        // index into the packed sequences it is read from
 	int32_t strand;     // 1: forward, 0: reverse complement
 	uint32_t iid;       // internal id used by the self-hit test
-	int32_t selfMode;   // 0: never, 1: resident read (O(1) test), 2: scan bucket
+	int32_t selfMode;   // 0: never, 1: resident read, every position indexed (O(1) test),
+						// 2: scan bucket, 3: resident read, binary search (minimizer index)
 	int32_t len;
 	int32_t reserved;
 };

@@ -185,6 +185,7 @@ struct Lane
 	DevBuf bAln, bAlnOff, bRA;
 	DevBuf bRawBases, bRawOff, bPackErr;  // ASCII bases being packed (packSeqsDevice)
 	DevBuf bGroupStart;                   // group starts of a batch's hits (score keys)
+	DevBuf bMinMask;                      // minimizer flags of the index build (one byte per window)
 };
 
 // One flyehip_overlaps_submit call: its query records and, as batches finish,
@@ -1466,7 +1467,7 @@ std::vector<JoinQuery> residentQueries(flyehip_ctx* c, const int64_t* ids, int32
 		q.iid = toInternal(id);
 		// a loaded index need not hold the query's own window, nor keep
 		// buckets in (read, position) order: scan the bucket for it
-		q.selfMode = c->indexLoaded ? 2 : 1;
+		q.selfMode = c->indexLoaded ? 2 : (c->info.minimizer_window > 0 ? 3 : 1);
 		q.len = c->readLen[static_cast<size_t>(r - 1)];
 		q.reserved = 0;
 		qs[static_cast<size_t>(i)] = q;
@@ -1580,14 +1581,20 @@ int flyehip_store_reads(flyehip_ctx* c, const uint8_t* bases, const int64_t* off
 	});
 }
 
-int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehip_index_info_t* out)
+namespace
 {
-	return guarded(c, [&]() {
+
+// Counts every canonical k-mer of the stored reads, fixes the thresholds and
+// fills the buckets: with every position, or (window > 0) with the minimizer
+// positions only (minimizerMaskKernel).
+void buildIndexImpl(flyehip_ctx* c, const flyehip_index_params_t* ip, int32_t window, flyehip_index_info_t* out)
+{
 		Lane* L = c->lanes[0].get();
 		if (!ip) fail(FLYEHIP_ERR_ARG, "null index parameters");
 		if (c->nReads <= 0) fail(FLYEHIP_ERR_STATE, "no reads stored");
 		const int k = ip->kmer_size;
 		if (k < 1 || k > 31) fail(FLYEHIP_ERR_ARG, "kmer_size %d out of range 1..31", k);
+		if (window < 0 || window > 256) fail(FLYEHIP_ERR_ARG, "minimizer window %d out of range 1..256", window);
 		c->k = k;
 		c->haveIndex = false;
 		c->indexLoaded = false;
@@ -1629,6 +1636,16 @@ int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehi
 		uint64_t* counters = ensureT<uint64_t>(c, L->bCounters, 16);
 		fill64(L, counters, 16, 0);
 		HIPCHECK(launchClassify(t, solidMin, repMax, sizes, counters, L->stream));
+		uint8_t* mask = nullptr;
+		if (window > 0)
+		{
+			// bucket sizes from the minimizer positions alone
+			mask = ensureT<uint8_t>(c, L->bMinMask, static_cast<size_t>(nWin) + 4);
+			zero32(L, reinterpret_cast<int32_t*>(mask), (nWin + 3) / 4);
+			HIPCHECK(launchMinimizerMask(rs, dWinOff, nWin, k, window, mask, L->stream));
+			zero32(L, sizes, static_cast<int64_t>(slots));
+			HIPCHECK(launchMaskedSizes(rs, dWinOff, nWin, k, t, mask, sizes, L->stream));
+		}
 		int64_t* starts = ensureT<int64_t>(c, L->bMatchOff, slots + 1);
 		int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(static_cast<int64_t>(slots)) + 64);
 		HIPCHECK(exclusiveScan(sizes, starts, static_cast<int64_t>(slots), tmp, L->stream));
@@ -1640,7 +1657,7 @@ int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehi
 		HIPCHECK(launchAssign(t, sizes, starts, L->stream));
 		uint64_t* ord = ensureT<uint64_t>(c, L->bMKey, static_cast<size_t>(nEntries) + 1);
 		ensure(c, c->dEntries, static_cast<size_t>(nEntries) * 8 + 8);
-		HIPCHECK(launchFill(rs, dWinOff, nWin, k, t, ord, L->stream));
+		HIPCHECK(launchFill(rs, dWinOff, nWin, k, t, mask, ord, L->stream));
 		// buckets above 32 entries (at most nEntries / 33 of them) are sorted
 		// by a block each; counters[2] counts them (zeroed above)
 		const uint64_t bigCap = static_cast<uint64_t>(nEntries) / 33 + 1;
@@ -1659,8 +1676,24 @@ int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehi
 		c->info.indexed_positions = nEntries;
 		c->info.hash_slots = static_cast<int64_t>(slots);
 		c->info.device_bytes = static_cast<int64_t>(c->dKeys.bytes + c->dVals.bytes + c->dCursor.bytes + c->dEntries.bytes + c->dWords.bytes);
+		c->info.minimizer_window = window;
 		c->haveIndex = true;
 		if (out) *out = c->info;
+}
+
+} // namespace
+
+int flyehip_build_index(flyehip_ctx* c, const flyehip_index_params_t* ip, flyehip_index_info_t* out)
+{
+	return guarded(c, [&]() { buildIndexImpl(c, ip, 0, out); });
+}
+
+int flyehip_build_index_minimizers(flyehip_ctx* c, const flyehip_index_params_t* ip, int32_t window,
+								   flyehip_index_info_t* out)
+{
+	return guarded(c, [&]() {
+		if (window < 1) fail(FLYEHIP_ERR_ARG, "minimizer window %d out of range 1..256", window);
+		buildIndexImpl(c, ip, window, out);
 	});
 }
 

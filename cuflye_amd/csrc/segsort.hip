@@ -599,8 +599,9 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* e
 	{
 		++rounds;
 		if (std::getenv("FLYEHIP_SEGSORT_TRACE"))
-			std::fprintf(stderr, "[segsort] round %d: big=%lld chunks=%lld leaves=%lld heap=%lld\n", rounds,
+			std::fprintf(stderr, "[segsort] round %d: big=%lld elems=%lld chunks=%lld leaves=%lld heap=%lld\n", rounds,
 						 static_cast<long long>(b.hostCounters[kSegNBig]),
+						 static_cast<long long>(b.hostCounters[kSegBigElems]),
 						 static_cast<long long>(b.hostCounters[kSegBigChunks]),
 						 static_cast<long long>(b.hostCounters[kSegNLeaf]),
 						 static_cast<long long>(b.hostCounters[kSegNHeap]));
@@ -671,8 +672,9 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* e
 	if (e != hipSuccess) return e;
 	const int64_t nHeap = b.hostCounters[kSegNHeap];
 	if (std::getenv("FLYEHIP_SEGSORT_TRACE"))
-		std::fprintf(stderr, "[segsort] rounds=%d verify=%lld free-heap=%lld\n", rounds,
-					 static_cast<long long>(b.hostCounters[kSegNVerify]), static_cast<long long>(nHeap));
+		std::fprintf(stderr, "[segsort] rounds=%d verify=%lld free-heap=%lld free-heap-elems=%lld\n", rounds,
+					 static_cast<long long>(b.hostCounters[kSegNVerify]), static_cast<long long>(nHeap),
+					 static_cast<long long>(b.hostCounters[kSegHeapElems]));
 	// heap bytes: key and value read, value written (12 per element)
 	if (nHeap > 0)
 	{
@@ -703,14 +705,15 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* e
 			SegRange* hv = new SegRange[static_cast<size_t>(nVerify)];
 			(void)hipMemcpyAsync(hv, b.verify, static_cast<size_t>(nVerify) * sizeof(SegRange), hipMemcpyDeviceToHost, s);
 			(void)hipStreamSynchronize(s);
-			long long exact = 0, elems = 0;
+			long long exact = 0, elems = 0, largest = 0;
 			for (int64_t i = 0; i < nVerify; ++i)
 				if (hv[i].d == 1)
 				{
 					++exact;
 					elems += hv[i].l - hv[i].f;
+					largest = std::max<long long>(largest, hv[i].l - hv[i].f);
 				}
-			std::fprintf(stderr, "[segsort] exact heap sorts=%lld elements=%lld\n", exact, elems);
+			std::fprintf(stderr, "[segsort] exact heap sorts=%lld elements=%lld largest=%lld\n", exact, elems, largest);
 			delete[] hv;
 		}
 	}

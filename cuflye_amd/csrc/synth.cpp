@@ -460,17 +460,21 @@ void* syn_hit_lists(int32_t n_lists, double alpha, int32_t xm, int32_t max_hits,
 }
 
 // Repeat graph over a random genome: repeat families are placed as in
-// syn_read_set; the genome is cut at every copy boundary.  Each unique
-// stretch is an edge; all copies of one family collapse into one repeat edge
-// carrying the family unit.  Node ids: the junction before and after each
-// copy of family f are nodes 2f and 2f+1 (shared by all copies), unique
-// stretch ends get nodes of their own.  Edge sequences are the forward
-// strands; edge i has signed id i + 1 and its complement -(i + 1).
+// syn_read_set; the genome is cut at every copy boundary.  All copies of one
+// family collapse into one repeat edge carrying the family unit.  Each unique
+// stretch between copies is one edge, or (seg_alpha > 0) a path of edges
+// whose lengths are drawn from a Pareto law (minimum seg_min, shape
+// seg_alpha, at most seg_max; a remainder below seg_min joins the last
+// piece), standing for the junctions of the many short repeats and bubbles
+// a real repeat graph has.  Node ids: the junction before and after each copy
+// of family f are nodes 2f and 2f+1 (shared by all copies); every other
+// junction gets a node of its own.  Edge sequences are the forward strands;
+// edge i has signed id i + 1 and its complement -(i + 1).
 void* syn_repeat_graph(int64_t genome_len, double gc, double rep_frac, int32_t rep_len_min,
 					   int32_t rep_len_max, int32_t copies_min, int32_t copies_max, double div_min,
 					   double div_max, double coverage, double mu, double sigma, int32_t min_len,
 					   int32_t max_len, double err, double w_ins, double w_del, double w_sub,
-					   double hp_boost, uint64_t seed)
+					   double hp_boost, int32_t seg_min, int32_t seg_max, double seg_alpha, uint64_t seed)
 {
 	Dataset* d = new Dataset();
 	Rng r(seed);
@@ -512,27 +516,47 @@ void* syn_repeat_graph(int64_t genome_len, double gc, double rep_frac, int32_t r
 	}
 	int32_t nextNode = 2 * static_cast<int32_t>(units.size());
 	int32_t prevNode = nextNode++;
+	// one unique stretch [a, b) from node `from` to node `to`, cut into a path
+	auto addStretch = [&](int64_t a, int64_t b, int32_t from, int32_t to) {
+		std::vector<int64_t> cuts{a};
+		if (seg_alpha > 0.0)
+		{
+			int64_t x = a;
+			for (;;)
+			{
+				double u = r.uniform();
+				if (u < 1e-12) u = 1e-12;
+				double len = static_cast<double>(seg_min) * std::pow(u, -1.0 / seg_alpha);
+				if (len > static_cast<double>(seg_max)) len = static_cast<double>(seg_max);
+				const int64_t next = x + static_cast<int64_t>(len);
+				if (next + seg_min > b) break;
+				cuts.push_back(next);
+				x = next;
+			}
+		}
+		cuts.push_back(b);
+		for (size_t i = 0; i + 1 < cuts.size(); ++i)
+		{
+			d->edgeSeqs.push_back(d->genome.substr(static_cast<size_t>(cuts[i]), static_cast<size_t>(cuts[i + 1] - cuts[i])));
+			d->edgeLeft.push_back(i == 0 ? from : nextNode - 1);
+			const bool lastPiece = i + 2 == cuts.size();
+			d->edgeRight.push_back(lastPiece ? to : nextNode++);
+			d->edgeIsRepeat.push_back(0);
+		}
+	};
 	int64_t cursor = 0;
 	for (auto& p : placed)
 	{
 		int32_t inNode = 2 * p.family;
 		int32_t outNode = 2 * p.family + 1;
-		if (p.start > cursor)
-		{
-			d->edgeSeqs.push_back(d->genome.substr(static_cast<size_t>(cursor), static_cast<size_t>(p.start - cursor)));
-			d->edgeLeft.push_back(prevNode);
-			d->edgeRight.push_back(inNode);
-			d->edgeIsRepeat.push_back(0);
-		}
+		if (p.start > cursor) addStretch(cursor, p.start, prevNode, inNode);
 		prevNode = outNode;
 		cursor = p.start + p.len;
 	}
 	if (cursor < genome_len)
 	{
-		d->edgeSeqs.push_back(d->genome.substr(static_cast<size_t>(cursor)));
-		d->edgeLeft.push_back(prevNode);
-		d->edgeRight.push_back(nextNode++);
-		d->edgeIsRepeat.push_back(0);
+		const int32_t endNode = nextNode++;
+		addStretch(cursor, genome_len, prevNode, endNode);
 	}
 	for (size_t f = 0; f < units.size(); ++f)
 	{

@@ -26,7 +26,7 @@ FLYEHIP_ERR_UNSUPPORTED = 6
 EXPORTED = (
     "flyehip_open", "flyehip_close", "flyehip_last_error", "flyehip_abi_version",
     "flyehip_get_stats", "flyehip_set_tuning", "flyehip_store_reads",
-    "flyehip_build_index", "flyehip_collect_candidates",
+    "flyehip_build_index", "flyehip_build_index_minimizers", "flyehip_collect_candidates",
     "flyehip_collect_sorted_candidates", "flyehip_overlaps",
     "flyehip_overlaps_external", "flyehip_overlaps_per_query",
     "flyehip_chain_lists", "flyehip_read_align_chains", "flyehip_divergence_logf",
@@ -42,7 +42,7 @@ class IndexParams(C.Structure):
 
 class IndexInfo(C.Structure):
     _fields_ = [("kmer_size", C.c_int32), ("solid_min", C.c_int32), ("rep_max", C.c_int32),
-                ("reserved", C.c_int32), ("distinct_kmers", C.c_int64),
+                ("minimizer_window", C.c_int32), ("distinct_kmers", C.c_int64),
                 ("solid_kmers", C.c_int64), ("repetitive_kmers", C.c_int64),
                 ("indexed_positions", C.c_int64), ("hash_slots", C.c_int64),
                 ("device_bytes", C.c_int64)]
@@ -153,6 +153,7 @@ def load_flyehip(path: Path | None = None) -> C.CDLL:
         "flyehip_set_tuning": ([_P, i32, i32, i32], C.c_int),
         "flyehip_store_reads": ([_P, _P, _P, i32], C.c_int),
         "flyehip_build_index": ([_P, C.POINTER(IndexParams), C.POINTER(IndexInfo)], C.c_int),
+        "flyehip_build_index_minimizers": ([_P, C.POINTER(IndexParams), i32, C.POINTER(IndexInfo)], C.c_int),
         "flyehip_collect_candidates": ([_P, _P, i32, _P, i64, _P, _P, _P, i64, _P, _P], C.c_int),
         "flyehip_collect_sorted_candidates": ([_P, _P, i32, _P, i64, _P, _P], C.c_int),
         "flyehip_overlaps": ([_P, _P, i32, C.POINTER(OverlapParams), _P, i64, _P, _P, _P, i64, _P],
@@ -195,7 +196,7 @@ def load_synth(path: Path | None = None) -> C.CDLL:
     lib.syn_hit_lists.argtypes = [i32, d, i32, i32, i32, d, d, u64]
     lib.syn_hit_lists.restype = _P
     lib.syn_repeat_graph.argtypes = [i64, d, d, i32, i32, i32, i32, d, d, d, d, d, i32, i32, d, d, d,
-                                     d, d, u64]
+                                     d, d, i32, i32, d, u64]
     lib.syn_repeat_graph.restype = _P
     lib.syn_destroy.argtypes = [_P]
     lib.syn_bases_size.argtypes = [_P, i32]

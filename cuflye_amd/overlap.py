@@ -158,6 +158,16 @@ class FlyeHip:
         self.index_info = info
         return info
 
+    def build_index_minimizers(self, kmer_size: int, window: int, solid_min: int = 0, rep_max: int = 0,
+                               repeat_kmer_rate: float = 10.0) -> N.IndexInfo:
+        """VertexIndex::buildIndexMinimizers: the thresholds of build_index,
+        buckets holding the minimizer positions (window `window` k-mers) only."""
+        p = N.IndexParams(kmer_size, solid_min, rep_max, repeat_kmer_rate)
+        info = N.IndexInfo()
+        self._check(self.lib.flyehip_build_index_minimizers(self.ctx, C.byref(p), window, C.byref(info)))
+        self.index_info = info
+        return info
+
     def load_index(self, kmer_size: int, kmers, bucket_offsets, entry_ids, entry_pos,
                    repetitive=()) -> N.IndexInfo:
         """Flye's own VertexIndex (flyehip_load_index): canonical k-mers,

@@ -1,9 +1,13 @@
 """Read-to-graph alignment (subsystem c): the host-side loop of
 ReadAligner::alignReads over the C ABI.
 
-For each read: quickSeqOverlaps against the edge-sequence index (the
-read-to-graph detector shape of getSeqOverlaps: no overhang check, min overlap
-= small alignment threshold, containment primary selection;
+The edge sequences are indexed at their minimizer positions
+(flyehip_build_index_minimizers, window MINIMIZER_WINDOW).  For each read:
+quickSeqOverlaps against that index - every read k-mer joined, as Flye's
+IterKmers loop does (docs/abi/read-to-graph-minimizer-source-pack-v0.md, "Full
+Query-Hit TSV") - in the read-to-graph detector shape of getSeqOverlaps (no
+overhang check, min overlap = small alignment threshold, containment primary
+selection;
 tools/replay_read_to_graph_source_pack.py:25-31, 324-349), the chain-input
 filter and std::sort by curBegin (patch 0039:72-77, 241-251), then
 chainReadAlignments (tools/replay_read_alignment_chains.py:64-139).  The
@@ -26,12 +30,19 @@ from .overlap import OverlapConfig
 
 LONG_EDGE = 900   # patch 0025 context: static const int LONG_EDGE = 900
 BIG_ALN = 500     # patch 0039:215
+SMALL_ALN = 100   # patch 0039:214 (ReadAligner::alignReads: static const int SMALL_ALN = 100)
+# Minimizer window of the edge-sequence index: Flye recorded 7,725 minimizers
+# over the 8 toy-hifi queries of its read-to-graph source pack (density 0.332,
+# tests/golden/cuflye-m6d-read-to-graph-minimizer-source-pack-dgx-aarch64.json),
+# the 2 / (w + 1) density of window 5.
+MINIMIZER_WINDOW = 5
 
 
 @dataclass
 class ReadToGraphParams:
     kmer_size: int = 15
-    small_alignment_threshold: int = 500
+    small_alignment_threshold: int = SMALL_ALN
+    minimizer_window: int = MINIMIZER_WINDOW
     maximum_jump: int = 1500
     max_read_overlap: int = 50
     minimum_overlap: int = 1000

@@ -134,11 +134,13 @@ class RepeatGraph:
 
 def config4(seed: int = 4, scale: float = 1.0, coverage: float = 30.0) -> RepeatGraph:
     """20 Mb genome with 3% repeats (10-25 kb copies, 2-6 per family, 0.5-2%
-    divergence) cut into a repeat graph; 30x ONT-like reads as in config2."""
+    divergence) cut into a repeat graph whose unique stretches are paths of
+    edges with Pareto lengths (minimum 1 kb, shape 1.02, at most 2 Mb): about
+    3,000 edges of 1 kb - 2 Mb at scale 1; 30x ONT-like reads as in config2."""
     lib = _lib()
     mu, sigma = lognormal_mean_n50(20000.0, 30000.0)
     h = lib.syn_repeat_graph(int(20_000_000 * scale), 0.5, 0.03, 10000, 25000, 2, 6, 0.005, 0.02, coverage,
-                             mu, sigma, 500, 150_000, 0.07, 2.0, 2.0, 1.0, 3.0, seed)
+                             mu, sigma, 500, 150_000, 0.07, 2.0, 2.0, 1.0, 3.0, 1000, 2_000_000, 1.02, seed)
     try:
         reads = _take_reads(h)
         ne = lib.syn_count(h, 1)

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define FLYEHIP_ABI_VERSION 3
+#define FLYEHIP_ABI_VERSION 4
 
 /* Status codes. */
 #define FLYEHIP_OK 0
@@ -67,9 +67,10 @@ typedef struct {
     int32_t kmer_size;
     int32_t solid_min;          /* threshold actually used */
     int32_t rep_max;            /* threshold actually used */
-    int32_t reserved;
+    int32_t minimizer_window;   /* 0: every position indexed */
     int64_t distinct_kmers;     /* distinct canonical k-mers counted */
-    int64_t solid_kmers;        /* solid and not repetitive (indexed) */
+    int64_t solid_kmers;        /* solid and not repetitive (a minimizer index
+                                   buckets only those with a minimizer position) */
     int64_t repetitive_kmers;
     int64_t indexed_positions;  /* total bucket entries */
     int64_t hash_slots;         /* open-addressed table size */
@@ -245,6 +246,24 @@ int flyehip_store_reads(flyehip_ctx* ctx, const uint8_t* bases,
  * in HBM (VertexIndex::countKmers / buildIndex; CUDA_FLYE_DESIGN.md:81-86). */
 int flyehip_build_index(flyehip_ctx* ctx, const flyehip_index_params_t* params,
                         flyehip_index_info_t* out_info);
+
+/* The edge-sequence index of read-to-graph alignment
+ * (VertexIndex::buildIndexMinimizers(minCoverage, wndLen) as ReadAligner
+ * builds pathsIndex; its parameters are recorded by
+ * patches/flye/2.9.6/0040-cuflye-read-to-graph-minimizer-source-pack.patch
+ * file lines 119-124 and 264-266, and Flye's quickSeqOverlaps then joins every
+ * query k-mer against it, docs/abi/read-to-graph-minimizer-source-pack-v0.md
+ * "Full Query-Hit TSV").  Counting and thresholds are flyehip_build_index's;
+ * buckets hold only minimizer positions.  Flye's yieldMinimizers is not in
+ * the reference, so the selection is this library's: in every run of
+ * `window` consecutive k-mer positions of a stored read (all of its
+ * positions when it has fewer), the position whose canonical k-mer has the
+ * smallest 64-bit mix (the table's splitmix64 finaliser) is a minimizer, the
+ * leftmost on equal values; density about 2 / (window + 1).  A k-mer with no
+ * minimizer position is not in the index (no hits, not filtered).  For
+ * Flye's exact pathsIndex upload it with flyehip_load_index. */
+int flyehip_build_index_minimizers(flyehip_ctx* ctx, const flyehip_index_params_t* params,
+                                   int32_t window, flyehip_index_info_t* out_info);
 
 /* Uploads Flye's own VertexIndex instead of building one: the backend of
  * patch 0005:245-266 reads Flye's `const VertexIndex&` (iterKmerPos,

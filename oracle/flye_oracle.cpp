@@ -165,7 +165,43 @@ int autoRepMax(const std::vector<int64_t>& hist, int solidMin, float rate)
 	return std::max(rep, solidMin);
 }
 
-Index* buildIndex(const Store& st, int k, int solidMin, int repMax, float rate)
+// Minimizer positions of one stored read (flyehip_build_index_minimizers'
+// definition): in every run of w consecutive k-mer positions (all of them
+// when there are fewer), the position with the smallest splitmix64 mix of its
+// canonical k-mer, the leftmost on equal values.
+uint64_t mix64(uint64_t x)
+{
+	x ^= x >> 30;
+	x *= 0xbf58476d1ce4e5b9ULL;
+	x ^= x >> 27;
+	x *= 0x94d049bb133111ebULL;
+	x ^= x >> 31;
+	return x;
+}
+
+std::vector<char> minimizerFlags(const std::string& s, int k, int w)
+{
+	std::vector<uint64_t> ord;
+	forEachWindow(s, k, [&](int32_t, uint64_t, uint64_t c, bool) { ord.push_back(mix64(c)); });
+	const int64_t nk = static_cast<int64_t>(ord.size());
+	std::vector<char> flag(static_cast<size_t>(nk), 0);
+	if (nk == 0) return flag;
+	const int64_t last = nk >= w ? nk - w : 0;
+	for (int64_t st = 0; st <= last; ++st)
+	{
+		const int64_t e = std::min<int64_t>(nk, st + w);
+		int64_t best = st;
+		for (int64_t p = st + 1; p < e; ++p)
+			if (ord[static_cast<size_t>(p)] < ord[static_cast<size_t>(best)]) best = p;
+		flag[static_cast<size_t>(best)] = 1;
+	}
+	return flag;
+}
+
+// VertexIndex::countKmers + buildIndex, or buildIndexMinimizers when
+// minWindow > 0 (same counts and thresholds; buckets of minimizer positions,
+// and a k-mer without one is left out of the index).
+Index* buildIndex(const Store& st, int k, int solidMin, int repMax, float rate, int minWindow = 0)
 {
 	Index* ix = new Index();
 	ix->k = k;
@@ -196,6 +232,28 @@ Index* buildIndex(const Store& st, int k, int solidMin, int repMax, float rate)
 	ix->solidMin = solidMin > 0 ? solidMin : autoSolidMin(ix->hist);
 	ix->repMax = repMax > 0 ? repMax : autoRepMax(ix->hist, ix->solidMin, rate);
 
+	std::vector<std::vector<char>> flags;
+	std::vector<std::pair<uint64_t, int64_t>> minCounts;  // k-mer, minimizer positions
+	if (minWindow > 0)
+	{
+		std::vector<uint64_t> sel;
+		for (const auto& s : st.fwd)
+		{
+			flags.push_back(minimizerFlags(s, k, minWindow));
+			const std::vector<char>& f = flags.back();
+			forEachWindow(s, k, [&](int32_t pos, uint64_t, uint64_t c, bool) {
+				if (f[static_cast<size_t>(pos)]) sel.push_back(c);
+			});
+		}
+		std::sort(sel.begin(), sel.end());
+		for (size_t i = 0; i < sel.size();)
+		{
+			size_t j = i;
+			while (j < sel.size() && sel[j] == sel[i]) ++j;
+			minCounts.emplace_back(sel[i], static_cast<int64_t>(j - i));
+			i = j;
+		}
+	}
 	std::vector<int64_t> bucketSize;
 	for (auto& c : counts)
 	{
@@ -205,8 +263,15 @@ Index* buildIndex(const Store& st, int k, int solidMin, int repMax, float rate)
 			ix->repetitive.push_back(c.first);
 			continue;
 		}
+		int64_t size = c.second;
+		if (minWindow > 0)
+		{
+			auto it = std::lower_bound(minCounts.begin(), minCounts.end(), std::make_pair(c.first, int64_t{0}));
+			size = (it != minCounts.end() && it->first == c.first) ? it->second : 0;
+			if (size == 0) continue;
+		}
 		ix->kmers.push_back(c.first);
-		bucketSize.push_back(c.second);
+		bucketSize.push_back(size);
 	}
 	ix->start.assign(ix->kmers.size() + 1, 0);
 	for (size_t b = 0; b < bucketSize.size(); ++b) ix->start[b + 1] = ix->start[b] + bucketSize[b];
@@ -217,7 +282,9 @@ Index* buildIndex(const Store& st, int k, int solidMin, int repMax, float rate)
 	{
 		const std::string& s = st.fwd[n];
 		ix->readLen.push_back(static_cast<int32_t>(s.size()));
+		const std::vector<char>* f = minWindow > 0 ? &flags[n] : nullptr;
 		forEachWindow(s, k, [&](int32_t pos, uint64_t, uint64_t c, bool flip) {
+			if (f && !(*f)[static_cast<size_t>(pos)]) return;
 			auto it = std::lower_bound(ix->kmers.begin(), ix->kmers.end(), c);
 			if (it == ix->kmers.end() || *it != c) return;
 			size_t b = static_cast<size_t>(it - ix->kmers.begin());
@@ -716,11 +783,20 @@ void* orc_create(const uint8_t* bases, const int64_t* offsets, int32_t n_reads)
 void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
 
 // out[0..7] = solid_min, rep_max, distinct, solid, repetitive, positions, hist_len, 0
+int orc_build_index_minimizers(void* h, int32_t k, int32_t window, int32_t solid_min, int32_t rep_max, float rate,
+							   int64_t* out);
 int orc_build_index(void* h, int32_t k, int32_t solid_min, int32_t rep_max, float rate, int64_t* out)
+{
+	return orc_build_index_minimizers(h, k, 0, solid_min, rep_max, rate, out);
+}
+
+// window 0: every position; else minimizer positions only
+int orc_build_index_minimizers(void* h, int32_t k, int32_t window, int32_t solid_min, int32_t rep_max, float rate,
+							   int64_t* out)
 {
 	Oracle* o = static_cast<Oracle*>(h);
 	delete o->index;
-	o->index = buildIndex(o->targets, k, solid_min, rep_max, rate);
+	o->index = buildIndex(o->targets, k, solid_min, rep_max, rate, window);
 	const Index& ix = *o->index;
 	out[0] = ix.solidMin;
 	out[1] = ix.repMax;
@@ -731,6 +807,15 @@ int orc_build_index(void* h, int32_t k, int32_t solid_min, int32_t rep_max, floa
 	out[6] = static_cast<int64_t>(ix.hist.size());
 	out[7] = 0;
 	return 0;
+}
+
+// Minimizer flags of one sequence (one byte per k-mer position).
+int64_t orc_minimizer_flags(const uint8_t* bases, int64_t len, int32_t k, int32_t window, uint8_t* out)
+{
+	const std::string s(reinterpret_cast<const char*>(bases), static_cast<size_t>(len));
+	const std::vector<char> f = minimizerFlags(s, k, window);
+	for (size_t i = 0; i < f.size(); ++i) out[i] = static_cast<uint8_t>(f[i]);
+	return static_cast<int64_t>(f.size());
 }
 
 // An index given as tables (flyehip_load_index's arguments): canonical

@@ -65,6 +65,10 @@ def lib() -> C.CDLL:
     L.orc_destroy.argtypes = [P]
     L.orc_build_index.argtypes = [P, i32, i32, i32, f32, P]
     L.orc_build_index.restype = C.c_int
+    L.orc_build_index_minimizers.argtypes = [P, i32, i32, i32, i32, f32, P]
+    L.orc_build_index_minimizers.restype = C.c_int
+    L.orc_minimizer_flags.argtypes = [P, i64, i32, i32, P]
+    L.orc_minimizer_flags.restype = i64
     L.orc_histogram.argtypes = [P, P, i64]
     L.orc_histogram.restype = i64
     L.orc_candidates.argtypes = [P, i64, P, i32, i32, P, i64, P, i64, P]
@@ -142,6 +146,14 @@ def _params(params):
     return params.to_c() if hasattr(params, "to_c") else params
 
 
+def minimizer_flags(seq, k: int, window: int) -> np.ndarray:
+    """One byte per k-mer position of an ASCII sequence: 1 at its minimizers."""
+    b = np.ascontiguousarray(np.frombuffer(seq, np.uint8) if isinstance(seq, (bytes, bytearray)) else seq, np.uint8)
+    out = np.zeros(max(0, len(b) - k + 1), np.uint8)
+    n = lib().orc_minimizer_flags(_p(b), len(b), k, window, _p(out))
+    return out[:n]
+
+
 def chain_dp(cur, ext, params, ext_sorted: bool = False, early_break: bool = True):
     """Score and backtrack tables of one group's DP, hits in their final order."""
     c = np.ascontiguousarray(cur, np.int32)
@@ -207,9 +219,12 @@ class Oracle:
         except Exception:
             pass
 
-    def build_index(self, k: int, solid_min: int = 0, rep_max: int = 0, rate: float = 10.0) -> dict:
+    def build_index(self, k: int, solid_min: int = 0, rep_max: int = 0, rate: float = 10.0,
+                    minimizer_window: int = 0) -> dict:
+        """countKmers + buildIndex, or buildIndexMinimizers with a window
+        (buckets of minimizer positions only)."""
         out = np.zeros(8, np.int64)
-        lib().orc_build_index(self.h, k, solid_min, rep_max, rate, _p(out))
+        lib().orc_build_index_minimizers(self.h, k, minimizer_window, solid_min, rep_max, rate, _p(out))
         self.info = dict(solid_min=int(out[0]), rep_max=int(out[1]), distinct=int(out[2]),
                          solid=int(out[3]), repetitive=int(out[4]), positions=int(out[5]))
         return self.info

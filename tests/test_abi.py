@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(N.EXPORTED)
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.flyehip_abi_version() == 3
+    assert lib.flyehip_abi_version() == 4
 
 
 def test_open_without_gpu_fails_closed():
