@@ -243,16 +243,23 @@ constexpr uint64_t kEmptyKey = ~0ULL;
 constexpr uint64_t kRepMark = ~0ULL;
 constexpr int kCountBits = 24;
 constexpr uint64_t kCountMask = (1ULL << kCountBits) - 1;
+constexpr uint64_t kWinFlipBit = 1ULL << 63;  // join windows: flip flag beside a bucket value
 
-// Probe for `key`; returns the slot or -1 when absent.
-__device__ inline int64_t findSlot(const uint64_t* keys, uint64_t mask, uint64_t key)
+// Probe an interleaved (key, value) slot array for `key`: the slot, or -1
+// when absent; *value receives the slot's value (read with the key, one
+// 16-byte load per probe).
+__device__ inline int64_t findSlot(const uint64_t* slots, uint64_t mask, uint64_t key, uint64_t* value = nullptr)
 {
 	uint64_t h = hashKmer(key) & mask;
 	for (;;)
 	{
-		uint64_t k = keys[h];
-		if (k == key) return static_cast<int64_t>(h);
-		if (k == kEmptyKey) return -1;
+		const ulonglong2 kv = *reinterpret_cast<const ulonglong2*>(slots + 2 * h);
+		if (kv.x == key)
+		{
+			if (value) *value = kv.y;
+			return static_cast<int64_t>(h);
+		}
+		if (kv.x == kEmptyKey) return -1;
 		h = (h + 1) & mask;
 	}
 }

@@ -134,20 +134,20 @@ __global__ void countKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWi
 	uint64_t h = hashKmer(c) & t.mask;
 	for (;;)
 	{
-		const uint64_t key = t.keys[h];
+		const uint64_t key = t.key(h);
 		if (key == c)
 		{
-			atomicAdd(reinterpret_cast<unsigned long long*>(&t.vals[h]), 1ULL);
+			atomicAdd(reinterpret_cast<unsigned long long*>(&t.val(h)), 1ULL);
 			return;
 		}
 		if (key == kEmptyKey)
 		{
-			const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&t.keys[h]),
+			const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&t.key(h)),
 											static_cast<unsigned long long>(kEmptyKey),
 											static_cast<unsigned long long>(c));
 			if (prev == kEmptyKey || prev == c)
 			{
-				atomicAdd(reinterpret_cast<unsigned long long*>(&t.vals[h]), 1ULL);
+				atomicAdd(reinterpret_cast<unsigned long long*>(&t.val(h)), 1ULL);
 				return;
 			}
 		}
@@ -169,9 +169,9 @@ __global__ void __launch_bounds__(256) histKernel(HashTable t, uint64_t* hist, i
 	for (uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; h < slots;
 		 h += static_cast<uint64_t>(gridDim.x) * blockDim.x)
 	{
-		if (t.keys[h] == kEmptyKey) continue;
+		if (t.key(h) == kEmptyKey) continue;
 		++myDistinct;
-		const uint64_t c = t.vals[h];
+		const uint64_t c = t.val(h);
 		if (c < static_cast<uint64_t>(kHistLds)) atomicAdd(&lh[c], 1U);
 		else if (c < static_cast<uint64_t>(hmax)) atomicAdd(reinterpret_cast<unsigned long long*>(&hist[c]), 1ULL);
 		else
@@ -192,9 +192,9 @@ __global__ void classifyKernel(HashTable t, int solidMin, int repMax, int32_t* s
 	const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (h > t.mask) return;
 	int32_t size = 0;
-	if (t.keys[h] != kEmptyKey)
+	if (t.key(h) != kEmptyKey)
 	{
-		const uint64_t c = t.vals[h];
+		const uint64_t c = t.val(h);
 		if (c >= static_cast<uint64_t>(solidMin) && c <= static_cast<uint64_t>(repMax))
 		{
 			size = static_cast<int32_t>(c);
@@ -202,12 +202,12 @@ __global__ void classifyKernel(HashTable t, int solidMin, int repMax, int32_t* s
 		}
 		else if (c >= static_cast<uint64_t>(solidMin))
 		{
-			t.vals[h] = kRepMark;
+			t.val(h) = kRepMark;
 			atomicAdd(reinterpret_cast<unsigned long long*>(&counters[1]), 1ULL);
 		}
 		else
 		{
-			t.vals[h] = 0;
+			t.val(h) = 0;
 		}
 	}
 	sizes[h] = size;
@@ -223,8 +223,8 @@ __global__ void assignKernel(HashTable t, const int32_t* sizes, const int64_t* s
 	const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (h > t.mask) return;
 	const int32_t s = sizes[h];
-	if (s > 0) t.vals[h] = (static_cast<uint64_t>(starts[h]) << kCountBits) | static_cast<uint64_t>(s);
-	else if (indexed(t.vals[h])) t.vals[h] = 0;
+	if (s > 0) t.val(h) = (static_cast<uint64_t>(starts[h]) << kCountBits) | static_cast<uint64_t>(s);
+	else if (indexed(t.val(h))) t.val(h) = 0;
 }
 
 // Minimizer windows (buildIndexMinimizers' selection, as defined by this
@@ -269,8 +269,9 @@ __global__ void maskedSizesKernel(PackedSeqs reads, const int64_t* winOff, int64
 	const int32_t pos = static_cast<int32_t>(w - winOff[r]);
 	bool flip;
 	const uint64_t c = canonical(kmerAt(reads.words, reads.wordOff[r], pos, k), k, flip);
-	const int64_t h = findSlot(t.keys, t.mask, c);
-	if (h >= 0 && indexed(t.vals[h])) atomicAdd(&sizes[h], 1);
+	uint64_t v = 0;
+	const int64_t h = findSlot(t.slots, t.mask, c, &v);
+	if (h >= 0 && indexed(v)) atomicAdd(&sizes[h], 1);
 }
 
 __global__ void loadKeysKernel(HashTable t, const uint64_t* keys, const uint64_t* vals, int64_t n, uint32_t* err)
@@ -281,11 +282,11 @@ __global__ void loadKeysKernel(HashTable t, const uint64_t* keys, const uint64_t
 	uint64_t h = hashKmer(c) & t.mask;
 	for (;;)
 	{
-		const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&t.keys[h]),
+		const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(&t.key(h)),
 										static_cast<unsigned long long>(kEmptyKey), static_cast<unsigned long long>(c));
 		if (prev == kEmptyKey)
 		{
-			t.vals[h] = vals[i];
+			t.val(h) = vals[i];
 			t.cursor[h] = 0;
 			return;
 		}
@@ -307,9 +308,9 @@ __global__ void fillKernel(PackedSeqs reads, const int64_t* winOff, int64_t nWin
 	const int32_t pos = static_cast<int32_t>(w - winOff[r]);
 	bool flip;
 	const uint64_t c = canonical(kmerAt(reads.words, reads.wordOff[r], pos, k), k, flip);
-	const int64_t h = findSlot(t.keys, t.mask, c);
+	uint64_t v = 0;
+	const int64_t h = findSlot(t.slots, t.mask, c, &v);
 	if (h < 0) return;
-	const uint64_t v = t.vals[h];
 	if (!indexed(v)) return;
 	const uint32_t slot = atomicAdd(&t.cursor[h], 1U);
 	const uint64_t start = v >> kCountBits;
@@ -336,7 +337,7 @@ __global__ void bucketFinishKernel(HashTable t, uint64_t* ordKeys, uint64_t* ent
 {
 	const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (h > t.mask) return;
-	const uint64_t v = t.vals[h];
+	const uint64_t v = t.val(h);
 	if (!indexed(v)) return;
 	const uint64_t start = v >> kCountBits;
 	const int n = static_cast<int>(v & kCountMask);
@@ -373,7 +374,7 @@ __global__ void __launch_bounds__(1024) bucketBlockSortKernel(HashTable t, uint6
 	const uint64_t nb = *bigCount;
 	for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x)
 	{
-		const uint64_t v = t.vals[bigList[b]];
+		const uint64_t v = t.val(bigList[b]);
 		const uint64_t start = v >> kCountBits;
 		const int64_t n = static_cast<int64_t>(v & kCountMask);
 		uint64_t* o = ordKeys + start;
@@ -512,11 +513,11 @@ __global__ void joinCountKernel(JoinArgs a)
 	const WindowKmer wk = windowAt(a, w);
 	int32_t count = 0;
 	int32_t filt = 0;
-	uint32_t info = 0xffffffffU;
-	const int64_t h = findSlot(a.table.keys, a.table.mask, wk.c);
+	uint64_t info = 0;
+	uint64_t v = 0;
+	const int64_t h = findSlot(a.table.slots, a.table.mask, wk.c, &v);
 	if (h >= 0)
 	{
-		const uint64_t v = a.table.vals[h];
 		if (v == kRepMark) filt = 1;
 		else if (v != 0)
 		{
@@ -552,12 +553,15 @@ __global__ void joinCountKernel(JoinArgs a)
 				self = selfIndex(a, Q, wk.pos, wk.g, v >> kCountBits, count) >= 0 ? 1 : 0;
 			}
 			count -= self;
-			info = static_cast<uint32_t>(h) | (wk.g ? 0x80000000U : 0U);
+			// the bucket (start << 24 | count) and the window's flip in bit 63
+			// (bucket starts stay below 2^39): the fill reads it here instead
+			// of probing the table again
+			info = v | (wk.g ? kWinFlipBit : 0ULL);
 		}
 	}
 	a.winCount[w] = count;
 	a.winFilt[w] = filt;
-	a.winInfo[w] = info;
+	a.winVal[w] = info;
 }
 
 // One wave fills the matches of 64 consecutive windows together: the raw
@@ -584,9 +588,9 @@ __global__ void __launch_bounds__(256) joinFillKernel(JoinArgs a)
 		if (a.winFilt[w]) a.filtered[a.filtOff[w]] = pos;
 		if (a.winCount[w] > 0)
 		{
-			const uint32_t info = a.winInfo[w];
-			const bool g = (info & 0x80000000U) != 0;
-			const uint64_t v = a.table.vals[info & 0x7fffffffU];
+			const uint64_t info = a.winVal[w];
+			const bool g = (info & kWinFlipBit) != 0;
+			const uint64_t v = info & ~kWinFlipBit;
 			start = v >> kCountBits;
 			raw = static_cast<int>(v & kCountMask);
 			const JoinQuery Q = a.queries[q];

@@ -20,12 +20,17 @@ struct PackedSeqs
 	int32_t n;
 };
 
+// Open-addressed k-mer table.  Slot h is one 16-byte pair: the canonical
+// k-mer at slots[2h] and its value at slots[2h + 1] (the count while
+// counting; then (start << 24 | count), kRepMark or 0), so a probe that hits
+// reads one cache line, not one line per array.
 struct HashTable
 {
-	uint64_t* keys;
-	uint64_t* vals;    // count while counting; then (start << 24 | count), kRepMark or 0
+	uint64_t* slots;
 	uint32_t* cursor;  // fill cursors
 	uint64_t mask;     // slots - 1
+	__host__ __device__ uint64_t& key(uint64_t h) const { return slots[2 * h]; }
+	__host__ __device__ uint64_t& val(uint64_t h) const { return slots[2 * h + 1]; }
 };
 
 // ---- scans ----------------------------------------------------------------
@@ -92,7 +97,7 @@ struct JoinArgs
 	const int32_t* targetLen;  // by read number - 1
 	int32_t* winCount;         // per window: matches
 	int32_t* winFilt;          // per window: 1 if repetitive
-	uint32_t* winInfo;         // slot | flip << 31 (0xffffffff when none)
+	uint64_t* winVal;          // bucket value (start << 24 | count) | flip << 63 (0 when none)
 	const int64_t* matchOff;   // exclusive scan of winCount (nWin + 1)
 	const int64_t* filtOff;    // exclusive scan of winFilt (nWin + 1)
 	uint64_t* mkey;            // (extIid << 32) | curPos, emission order

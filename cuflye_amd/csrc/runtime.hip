@@ -252,7 +252,7 @@ struct flyehip_ctx
 	bool indexLoaded = false;  // buckets given by the caller (flyehip_load_index), in the caller's order
 	int k = 0;
 	flyehip_index_info_t info{};
-	fh::DevBuf dKeys, dVals, dCursor, dEntries;
+	fh::DevBuf dKeys, dCursor, dEntries;  // dKeys: (key, value) slot pairs
 	uint64_t tableMask = 0;
 
 	// logf table
@@ -511,6 +511,13 @@ __global__ void fillU64Kernel(uint64_t* p, int64_t n, uint64_t v)
 	if (i < n) p[i] = v;
 }
 
+// n (key, value) pairs set to (k, v): an empty hash table
+__global__ void fillPairsKernel(uint64_t* p, int64_t n, uint64_t k, uint64_t v)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i < n) *reinterpret_cast<ulonglong2*>(p + 2 * i) = make_ulonglong2(k, v);
+}
+
 dim3 grid(int64_t n, int tpb) { return dim3(static_cast<unsigned>(std::max<int64_t>(1, (n + tpb - 1) / tpb))); }
 
 void zero32(Lane* L, int32_t* p, int64_t n)
@@ -527,6 +534,12 @@ void fill64(Lane* L, uint64_t* p, int64_t n, uint64_t v)
 	HIPCHECK(hipGetLastError());
 }
 
+void emptyTable(Lane* L, uint64_t* slots, int64_t n)
+{
+	hipLaunchKernelGGL(fillPairsKernel, grid(n, 256), dim3(256), 0, L->stream, slots, n, kEmptyKey, uint64_t{0});
+	HIPCHECK(hipGetLastError());
+}
+
 PackedSeqs storeSeqs(flyehip_ctx* c)
 {
 	PackedSeqs s;
@@ -540,8 +553,7 @@ PackedSeqs storeSeqs(flyehip_ctx* c)
 HashTable table(flyehip_ctx* c)
 {
 	HashTable t;
-	t.keys = static_cast<uint64_t*>(c->dKeys.p);
-	t.vals = static_cast<uint64_t*>(c->dVals.p);
+	t.slots = static_cast<uint64_t*>(c->dKeys.p);
 	t.cursor = static_cast<uint32_t*>(c->dCursor.p);
 	t.mask = c->tableMask;
 	return t;
@@ -599,7 +611,7 @@ BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& 
 	ja.targetLen = static_cast<const int32_t*>(c->dReadLen.p);
 	ja.winCount = ensureT<int32_t>(c, L->bWinCount, static_cast<size_t>(nWin) + 1);
 	ja.winFilt = ensureT<int32_t>(c, L->bWinFilt, static_cast<size_t>(nWin) + 1);
-	ja.winInfo = ensureT<uint32_t>(c, L->bWinInfo, static_cast<size_t>(nWin) + 1);
+	ja.winVal = ensureT<uint64_t>(c, L->bWinInfo, static_cast<size_t>(nWin) + 1);
 	int64_t* matchOff = ensureT<int64_t>(c, L->bMatchOff, static_cast<size_t>(nWin) + 1);
 	int64_t* filtOff = ensureT<int64_t>(c, L->bFiltOff, static_cast<size_t>(nWin) + 1);
 	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nWin) + 64);
@@ -1523,7 +1535,7 @@ int flyehip_close(flyehip_ctx* c)
 	asyncStop(c);
 	(void)hipSetDevice(c->device);
 	for (auto& l : c->lanes) destroyLane(c, *l);
-	for (DevBuf* b : {&c->dWords, &c->dWordOff, &c->dReadLen, &c->dReadWinOff, &c->dKeys, &c->dVals, &c->dCursor,
+	for (DevBuf* b : {&c->dWords, &c->dWordOff, &c->dReadLen, &c->dReadWinOff, &c->dKeys, &c->dCursor,
 					  &c->dEntries, &c->dLogX, &c->dLogY})
 		release(c, *b);
 	delete c;
@@ -1607,11 +1619,9 @@ void buildIndexImpl(flyehip_ctx* c, const flyehip_index_params_t* ip, int32_t wi
 		uint64_t slots = 1024;
 		while (slots < static_cast<uint64_t>(nWin) * 3 / 2 + 1024) slots <<= 1;
 		c->tableMask = slots - 1;
-		ensure(c, c->dKeys, slots * 8);
-		ensure(c, c->dVals, slots * 8);
+		ensure(c, c->dKeys, slots * 16);
 		ensure(c, c->dCursor, slots * 4);
-		fill64(L, static_cast<uint64_t*>(c->dKeys.p), static_cast<int64_t>(slots), kEmptyKey);
-		fill64(L, static_cast<uint64_t*>(c->dVals.p), static_cast<int64_t>(slots), 0);
+		emptyTable(L, static_cast<uint64_t*>(c->dKeys.p), static_cast<int64_t>(slots));
 		const PackedSeqs rs = storeSeqs(c);
 		HashTable t = table(c);
 		HIPCHECK(launchCount(rs, dWinOff, nWin, k, t, L->stream));
@@ -1675,7 +1685,7 @@ void buildIndexImpl(flyehip_ctx* c, const flyehip_index_params_t* ip, int32_t wi
 		c->info.repetitive_kmers = static_cast<int64_t>(cnt[1]);
 		c->info.indexed_positions = nEntries;
 		c->info.hash_slots = static_cast<int64_t>(slots);
-		c->info.device_bytes = static_cast<int64_t>(c->dKeys.bytes + c->dVals.bytes + c->dCursor.bytes + c->dEntries.bytes + c->dWords.bytes);
+		c->info.device_bytes = static_cast<int64_t>(c->dKeys.bytes + c->dCursor.bytes + c->dEntries.bytes + c->dWords.bytes);
 		c->info.minimizer_window = window;
 		c->haveIndex = true;
 		if (out) *out = c->info;
@@ -1766,12 +1776,10 @@ int flyehip_load_index(flyehip_ctx* c, int32_t kmer_size, const uint64_t* kmers,
 		while (slots < static_cast<uint64_t>(nKeys) * 3 / 2 + 1024) slots <<= 1;
 		c->k = k;
 		c->tableMask = slots - 1;
-		ensure(c, c->dKeys, slots * 8);
-		ensure(c, c->dVals, slots * 8);
+		ensure(c, c->dKeys, slots * 16);
 		ensure(c, c->dCursor, slots * 4);
 		ensure(c, c->dEntries, static_cast<size_t>(nEntries) * 8 + 8);
-		fill64(L, static_cast<uint64_t*>(c->dKeys.p), static_cast<int64_t>(slots), kEmptyKey);
-		fill64(L, static_cast<uint64_t*>(c->dVals.p), static_cast<int64_t>(slots), 0);
+		emptyTable(L, static_cast<uint64_t*>(c->dKeys.p), static_cast<int64_t>(slots));
 		uint64_t* dk = ensureT<uint64_t>(c, L->bCKey, static_cast<size_t>(nKeys) + 1);
 		uint64_t* dv = ensureT<uint64_t>(c, L->bSKey, static_cast<size_t>(nKeys) + 1);
 		uint32_t* dErr = ensureT<uint32_t>(c, L->bPackErr, 1);
@@ -1791,7 +1799,7 @@ int flyehip_load_index(flyehip_ctx* c, int32_t kmer_size, const uint64_t* kmers,
 		c->info.repetitive_kmers = n_repetitive;
 		c->info.indexed_positions = nEntries;
 		c->info.hash_slots = static_cast<int64_t>(slots);
-		c->info.device_bytes = static_cast<int64_t>(c->dKeys.bytes + c->dVals.bytes + c->dCursor.bytes + c->dEntries.bytes + c->dWords.bytes);
+		c->info.device_bytes = static_cast<int64_t>(c->dKeys.bytes + c->dCursor.bytes + c->dEntries.bytes + c->dWords.bytes);
 		c->haveIndex = true;
 		c->indexLoaded = true;
 		if (out) *out = c->info;

@@ -100,7 +100,7 @@ KERNEL_NAMES = {
     "round_swap": "segSwapKernel",
     "round_other": "segMedian/Map/SplitKernel + chunk scans",
     "gather": "gatherHitsKernel",
-    "score_keys": "chainScoreKeysKernel",
+    "score_keys": "chainScorePushKernel",
     "unused": "-",
 }
 # Kernels that are one HIP kernel (a roofline names one of these).
