@@ -567,6 +567,8 @@ def main() -> int:
         "counts": {k: stats[k] for k in ("windows", "matches", "groups", "overlaps", "batches", "tie_queries",
                                          "dp_cells")},
         "index_build_s": index_s,
+        "timed_region": "getSeqOverlaps of every query record, rows copied back; the read upload and the "
+                        "index build (index_build_s) happen once before it",
     }
     # the metric is bit-exact output: a failed parity check withholds the value
     failed = [name for name, ok in (("parity_sample", parity),

@@ -1107,6 +1107,12 @@ void orc_std_sort_perm(const uint64_t* keys, int64_t n, int32_t* perm)
 // glibc logf, the function Flye's divergence calls.
 float orc_logf(float x) { return logf(x); }
 
+// glibc logf over an array (the exhaustive comparison of the device's logf)
+void orc_logf_array(const float* x, int64_t n, float* out)
+{
+	for (int64_t i = 0; i < n; ++i) out[i] = logf(x[i]);
+}
+
 // ---- pieces of one group, exported for the tests that pin them ------------
 
 // The DP table of one group over hits in their final order (no sort).

@@ -82,6 +82,8 @@ def lib() -> C.CDLL:
     L.orc_std_sort_perm.argtypes = [P, i64, P]
     L.orc_logf.argtypes = [f32]
     L.orc_logf.restype = f32
+    L.orc_logf_array.argtypes = [P, i64, P]
+    L.orc_logf_array.restype = None
     L.orc_chain_dp.argtypes = [P, P, i32, P, i32, i32, P, P]
     L.orc_chain_dp.restype = None
     L.orc_chain_group.argtypes = [P, P, i32, i64, i32, i64, i32, P, i32, P, P, i64]
@@ -140,6 +142,14 @@ def std_sort_perm(keys: np.ndarray) -> np.ndarray:
 
 def logf(x: float) -> float:
     return lib().orc_logf(x)
+
+
+def logf_array(x: np.ndarray) -> np.ndarray:
+    """glibc logf of every element (float32)."""
+    a = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(a)
+    lib().orc_logf_array(_p(a), len(a), _p(out))
+    return out
 
 
 def _params(params):
