@@ -5,23 +5,37 @@ and bench.py's cpu_baseline leg compare the GPU library against.  It is a
 restatement of Flye 2.9.6's overlap stage (see flye_oracle.cpp for the
 reference lines each part follows) and never part of the product path.
 
-Parity pins (what each oracle function is checked against, tests/test_oracle.py):
+Parity pins (what each oracle function is checked against):
   encode / reverse complement / standard form / id mapping
-      - tests/golden/reference_fixtures: kmer-encode-smoke-v0, kmer-join-smoke-v0,
-        read-window-smoke-v0 expected candidates and their SHA-256
-        (c07761a6..., f0ef59da...), closed forms.
+      - the reference fixtures kmer-encode-smoke-v0, kmer-join-smoke-v0,
+        read-window-smoke-v0: expected candidates and their recorded SHA-256
+        (c07761a6..., f0ef59da...); closed forms (tests/test_oracle.py).
   candidate loop + VertexIndex orientation
-      - brute force: every (query window, target strand position) pair with an
-        identical k-mer on tiny inputs; the self-hit rule.
-  std::sort points - real libstdc++ std::sort (by construction).
-  chaining DP, backtrack, overlapTest, divergence, selection
-      - hand-worked cases with closed-form answers, invariants (chain hits
-        strictly increasing on both axes, score arithmetic), the glibc logf path.
+      - brute force over every (query window, target strand position) pair on
+        tiny inputs (test_candidates_brute_force); the k=4 fixtures through
+        Flye-ordered buckets reproduce their hashes (tests/test_load_index.py).
+  candidate-record-v1 canonical hash
+      - the fixtures' recorded SHA-256 (tests/test_canonical_hash.py).
+  VertexIndex thresholds and buckets
+      - a dictionary count of every k-mer on tiny reads, thresholds and bucket
+        order by hand (test_index_brute_force).  Parity unpinned against Flye:
+        the reference does not ship vertex_index.*.
+  minimizer index
+      - a brute force written from the definition, the 2 / (w + 1) density and
+        Flye's recorded read-to-graph minimizer density, the full index
+        filtered by the flags (tests/test_minimizers.py).  Selection parity
+        unpinned against Flye: yieldMinimizers is not in the reference.
+  std::sort points
+      - real libstdc++ std::sort; against the reference's cpp_sort_in_place on
+        tie-heavy keys (tests/golden/cpp_sort_v1.npz).
+  prefilters, chaining DP (gap tiers, max_jump_gap, maxJump, early break),
+  backtrack, overlapTest, divergence, filtered positions, selection, divStats
+      - hand-worked closed forms (tests/test_oracle_closed_form.py); golden
+        vectors of the reference's replay_group, chain_target and overlap_test
+        (tests/golden/*.npz, tests/test_reference_vectors.py).
   chainReadAlignments
-      - tests/fixtures read-alignment-replay-fixture-v0/simple (2 rows, 1 chain).
-  VertexIndex thresholds - parity unpinned against Flye (the reference does not
-      ship vertex_index.*); pinned against the definition in include/flyehip.h
-      by histogram arithmetic tests.
+      - the reference fixture read-alignment-replay-fixture-v0/simple; golden
+        vectors of chain_read_alignments (read_align_v1.npz).
 """
 from __future__ import annotations
 
