@@ -5,11 +5,13 @@
 // Only elements with equal keys can end up in an order other than the sorted
 // one, and equal (extId, curPos) keys are rare (a k-mer occurring twice in one
 // target read).  So:
-//   1. a stable segmented radix sort (one segment per query) gives the sorted
-//      order S, which is std::sort's result wherever keys are distinct.  The
-//      matches of a query are emitted window by window, so curPos never
-//      decreases in emission order and sorting the extId bits alone is
-//      enough;
+//   1. a stable sort of each query's matches by extId gives the sorted order
+//      S, which is std::sort's result wherever keys are distinct.  The matches
+//      of a query are emitted window by window, so curPos never decreases in
+//      emission order and sorting by extId alone is enough.  When the target
+//      ids fit an LDS histogram, one block per query counts them and scatters
+//      every match to its place in emission order (matchCountSortKernel);
+//      otherwise a segmented radix sort over the extId bits does it;
 //   2. every match gets a 32-bit rank key (position in S of the first equal
 //      key), which orders exactly like the 64-bit key, and the positions of S
 //      whose key also sits at a neighbouring position are flagged "tied";
@@ -20,6 +22,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "dev_common.hpp"
+#include "exsort_wave.hpp"
 #include "msort.hpp"
 
 namespace fh
@@ -44,6 +47,138 @@ __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* 
 
 inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n + tpb - 1) / tpb)); }
 
+// Stable counting sort of one query's matches by internal extId (the high 32
+// bits of the key), one 1,024-thread block per query:
+//   1. a histogram of the extIds in LDS (LDS atomics);
+//   2. its exclusive scan: the first slot of every extId in S;
+//   3. the matches in emission order, 4,096 at a time (4 rows of one per
+//      thread): every wave finds, for each lane, the lanes of its row with
+//      the same extId (one ballot per id bit), so a lane knows its rank among
+//      them and each group has a leader; the leaders claim their slots from
+//      the histogram in emission order (one wave row at a time, a block
+//      barrier between turns), and every match is written to its slot.
+// Equal extIds keep their emission order, which is the stable order.
+constexpr int kMsortThreads = 1024;
+constexpr int kMsortWaves = kMsortThreads / FH_WAVE;
+constexpr int kMsortRows = 4;
+
+// Lanes of the wave whose bin equals this lane's (0 for bin < 0, a lane
+// without an element): one ballot per bit of the bin.
+__device__ inline unsigned long long sameBinMask(int bin, int bits)
+{
+	const bool act = bin >= 0;
+	unsigned long long m = __ballot(act);
+	for (int bit = 0; bit < bits; ++bit)
+	{
+		const bool on = act && ((bin >> bit) & 1);
+		const unsigned long long bb = __ballot(on);
+		m &= on ? bb : ~bb;
+	}
+	return act ? m : 0ULL;
+}
+// LDS: histogram words + one base word per (row, lane) + wave totals
+constexpr int kMsortFixedWords = kMsortRows * kMsortThreads + kMsortWaves;
+
+__global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int64_t* qOff, const uint64_t* akey,
+																	  const uint32_t* aval, uint64_t* skey,
+																	  uint32_t* sval, int nBins, int bits)
+{
+	extern __shared__ uint32_t lds[];
+	uint32_t* hist = lds;
+	uint32_t* lead = lds + nBins;                       // kMsortRows * kMsortThreads
+	uint32_t* waveTot = lead + kMsortRows * kMsortThreads;  // kMsortWaves
+	const int t = static_cast<int>(threadIdx.x);
+	const int wv = t / FH_WAVE;
+	const int lane = laneId();
+	const int64_t b0 = qOff[blockIdx.x];
+	const int64_t n = qOff[blockIdx.x + 1] - b0;
+	if (n == 0) return;
+	for (int i = t; i < nBins; i += kMsortThreads) hist[i] = 0;
+	__syncthreads();
+	// (per-run aggregation with the bin ballots measured slower than the LDS
+	// atomics' serialisation on equal bins)
+	for (int64_t i = t; i < n; i += kMsortThreads) atomicAdd(&hist[static_cast<uint32_t>(akey[b0 + i] >> 32)], 1u);
+	__syncthreads();
+	const unsigned long long lt = laneMaskLt();
+	// exclusive scan of the histogram: a contiguous piece per thread
+	{
+		const int per = (nBins + kMsortThreads - 1) / kMsortThreads;
+		const int lo = min(nBins, t * per), hi = min(nBins, lo + per);
+		uint32_t sum = 0;
+		for (int i = lo; i < hi; ++i) sum += hist[i];
+		const int incl = waveInclusiveSum(static_cast<int>(sum));
+		if (lane == FH_WAVE - 1) waveTot[wv] = static_cast<uint32_t>(incl);
+		__syncthreads();
+		uint32_t run = static_cast<uint32_t>(incl) - sum;
+		for (int w = 0; w < wv; ++w) run += waveTot[w];
+		for (int i = lo; i < hi; ++i)
+		{
+			const uint32_t c = hist[i];
+			hist[i] = run;
+			run += c;
+		}
+		__syncthreads();
+	}
+	for (int64_t t0 = 0; t0 < n; t0 += kMsortRows * kMsortThreads)
+	{
+		uint64_t key[kMsortRows];
+		uint32_t val[kMsortRows];
+		int bin[kMsortRows];
+		unsigned long long same[kMsortRows];
+#pragma unroll
+		for (int r = 0; r < kMsortRows; ++r)
+		{
+			const int64_t i = t0 + r * kMsortThreads + t;
+			const bool act = i < n;
+			key[r] = act ? akey[b0 + i] : 0;
+			val[r] = act ? aval[b0 + i] : 0;
+			bin[r] = act ? static_cast<int>(key[r] >> 32) : -1;
+		}
+#pragma unroll
+		for (int r = 0; r < kMsortRows; ++r) same[r] = sameBinMask(bin[r], bits);
+		// leaders claim slots in emission order: element t0 + r * 1024 + t
+		// comes before every element of a later (r, wave) turn
+		for (int r = 0; r < kMsortRows; ++r)
+		{
+			for (int w = 0; w < kMsortWaves; ++w)
+			{
+				if (wv == w && bin[r] >= 0 && (same[r] & lt) == 0)
+				{
+					const uint32_t base = hist[bin[r]];
+					hist[bin[r]] = base + static_cast<uint32_t>(__popcll(same[r]));
+					lead[r * kMsortThreads + t] = base;
+				}
+				__syncthreads();
+			}
+		}
+#pragma unroll
+		for (int r = 0; r < kMsortRows; ++r)
+		{
+			if (bin[r] < 0) continue;
+			const int ld = __ffsll(static_cast<long long>(same[r])) - 1;
+			const uint32_t base = lead[r * kMsortThreads + wv * FH_WAVE + ld];
+			const int64_t d = b0 + base + __popcll(same[r] & lt);
+			skey[d] = key[r];
+			sval[d] = val[r];
+		}
+		__syncthreads();  // lead[] is rewritten by the next tile
+	}
+}
+
+int msortMaxBins()
+{
+	static int maxBins = -1;
+	if (maxBins < 0)
+	{
+		const int maxBytes = 160 * 1024;
+		maxBins = hipFuncSetAttribute(reinterpret_cast<const void*>(matchCountSortKernel),
+									  hipFuncAttributeMaxDynamicSharedMemorySize, maxBytes) == hipSuccess
+					  ? maxBytes / 4 - kMsortFixedWords
+					  : 65536 / 4 - kMsortFixedWords;
+	}
+	return maxBins;
+}
+
 } // namespace
 
 hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t* bytes)
@@ -57,16 +192,22 @@ hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t
 													   static_cast<const int64_t*>(nullptr), 32, 32 + idBits);
 }
 
-hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBits, void* temp, size_t tempBytes,
-								 hipStream_t s)
+bool msortCounting(int nBins) { return nBins <= msortMaxBins(); }
+
+hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBits, int nBins, void* temp,
+								 size_t tempBytes, hipStream_t s)
 {
 	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
-	const int tpb = 256;
-	hipError_t e = hipcub::DeviceSegmentedRadixSort::SortPairs(temp, tempBytes, a.akey, a.skey, a.aval, a.sval,
-															   static_cast<int>(nMatches), a.nQueries, a.qOff,
-															   a.qOff + 1, 32, 32 + idBits, s);
-	(void)tpb;
-	return e;
+	if (msortCounting(nBins))
+	{
+		const size_t ldsBytes = static_cast<size_t>(nBins + kMsortFixedWords) * 4;
+		hipLaunchKernelGGL(matchCountSortKernel, dim3(static_cast<unsigned>(a.nQueries)), dim3(kMsortThreads), ldsBytes, s,
+						   a.qOff, a.akey, a.aval, a.skey, a.sval, nBins, idBits);
+		return hipGetLastError();
+	}
+	return hipcub::DeviceSegmentedRadixSort::SortPairs(temp, tempBytes, a.akey, a.skey, a.aval, a.sval,
+													   static_cast<int>(nMatches), a.nQueries, a.qOff, a.qOff + 1, 32,
+													   32 + idBits, s);
 }
 
 hipError_t launchRankKeys(const MsortArgs& a, int64_t nMatches, hipStream_t s)

@@ -20,10 +20,12 @@ struct MsortArgs
 };
 
 hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t* bytes);
-// Stable order (segmented radix sort on the extId bits: emission order is
-// already ascending in curPos).
-hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBits, void* temp, size_t tempBytes,
-								 hipStream_t s);
+// Stable order by extId (emission order is already ascending in curPos):
+// a per-query counting sort in LDS when the nBins target ids fit it
+// (msortCounting), else a segmented radix sort on the extId bits.
+bool msortCounting(int nBins);
+hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBits, int nBins, void* temp,
+								 size_t tempBytes, hipStream_t s);
 // Rank keys and tied-position flags of the stable order.
 hipError_t launchRankKeys(const MsortArgs& a, int64_t nMatches, hipStream_t s);
 

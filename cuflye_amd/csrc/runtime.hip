@@ -730,9 +730,11 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nMatches) + 64);
 	int bits = 1;
 	while ((int64_t{1} << bits) < 2 * static_cast<int64_t>(std::max(1, c->nReads))) ++bits;
+	const int nBins = 2 * std::max(1, c->nReads);  // internal target ids
+	const bool counting = msortCounting(nBins);
 	size_t tempBytes = 0;
-	HIPCHECK(msortTempBytes(nMatches, nq, bits, &tempBytes));
-	void* temp = ensure(c, L->bMsortTemp, tempBytes + 256);
+	if (!counting) HIPCHECK(msortTempBytes(nMatches, nq, bits, &tempBytes));
+	void* temp = counting ? nullptr : ensure(c, L->bMsortTemp, tempBytes + 256);
 	SegSortBufs sb = segBufs(L, nMatches, nq);
 	sb.key = ms.rkey;
 	sb.val = ja.mval;
@@ -741,11 +743,12 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
 	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
-	// stable order: 2 radix passes x 24 bytes per match (key and value read
-	// and written); rank keys: sorted key and value read, rank key and tie
-	// flag written (20 bytes per match)
-	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, 48.0 * static_cast<double>(nMatches));
-	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, temp, tempBytes, L->stream));
+	// stable order: the counting sort reads the keys twice (histogram, then
+	// with the values) and writes keys and values once (32 bytes per match);
+	// the radix fallback makes 2 passes x 24 bytes.  Rank keys: sorted key and
+	// value read, rank key and tie flag written (20 bytes per match)
+	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, (counting ? 32.0 : 48.0) * static_cast<double>(nMatches));
+	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, nBins, temp, tempBytes, L->stream));
 	L->timer.end(tr, L->stream);
 	const int tk = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 20.0 * static_cast<double>(nMatches));
 	HIPCHECK(launchRankKeys(ms, nMatches, L->stream));
