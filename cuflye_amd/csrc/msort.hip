@@ -51,16 +51,26 @@ inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n +
 // bits of the key), one 1,024-thread block per query:
 //   1. a histogram of the extIds in LDS (LDS atomics);
 //   2. its exclusive scan: the first slot of every extId in S;
-//   3. the matches in emission order, 4,096 at a time (4 rows of one per
-//      thread): every wave finds, for each lane, the lanes of its row with
-//      the same extId (one ballot per id bit), so a lane knows its rank among
-//      them and each group has a leader; the leaders claim their slots from
-//      the histogram in emission order (one wave row at a time, a block
-//      barrier between turns), and every match is written to its slot.
+//   3. the matches in emission order, 4,096 at a time (every wave 256
+//      consecutive ones as 4 rows of 64): every wave finds, for each lane,
+//      the lanes of its row with the same extId (one ballot per id bit), so a
+//      lane knows its rank among them and each group has a leader; the
+//      waves' leaders claim their slots from the histogram in emission order
+//      (one wave at a time, a block barrier between turns), and every match
+//      is written to its slot.
 // Equal extIds keep their emission order, which is the stable order.
 constexpr int kMsortThreads = 1024;
 constexpr int kMsortWaves = kMsortThreads / FH_WAVE;
 constexpr int kMsortRows = 4;
+
+// Block barrier that orders LDS accesses only: global loads issued before it
+// stay in flight.
+__device__ inline void ldsBarrier()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Lanes of the wave whose bin equals this lane's (0 for bin < 0, a lane
 // without an element): one ballot per bit of the bin.
@@ -79,9 +89,9 @@ __device__ inline unsigned long long sameBinMask(int bin, int bits)
 // LDS: histogram words + one base word per (row, lane) + wave totals
 constexpr int kMsortFixedWords = kMsortRows * kMsortThreads + kMsortWaves;
 
-__global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int64_t* qOff, const uint64_t* akey,
-																	  const uint32_t* aval, uint64_t* skey,
-																	  uint32_t* sval, int nBins, int bits)
+__global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int64_t* qOff, const int32_t* order,
+																	  const uint64_t* akey, const uint32_t* aval,
+																	  uint64_t* skey, uint32_t* sval, int nBins, int bits)
 {
 	extern __shared__ uint32_t lds[];
 	uint32_t* hist = lds;
@@ -90,8 +100,11 @@ __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int6
 	const int t = static_cast<int>(threadIdx.x);
 	const int wv = t / FH_WAVE;
 	const int lane = laneId();
-	const int64_t b0 = qOff[blockIdx.x];
-	const int64_t n = qOff[blockIdx.x + 1] - b0;
+	// the largest queries first: a block runs a whole query, so the longest
+	// one bounds the kernel unless it starts early
+	const int q = order ? order[blockIdx.x] : static_cast<int>(blockIdx.x);
+	const int64_t b0 = qOff[q];
+	const int64_t n = qOff[q + 1] - b0;
 	if (n == 0) return;
 	for (int i = t; i < nBins; i += kMsortThreads) hist[i] = 0;
 	__syncthreads();
@@ -119,37 +132,53 @@ __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int6
 		}
 		__syncthreads();
 	}
+	// tile t0 is processed while the next one loads: the barriers of the
+	// claiming turns order LDS only, so the next tile's global loads stay in
+	// flight across them
+	uint64_t key[kMsortRows];
+	uint32_t val[kMsortRows];
+	int bin[kMsortRows];
+	auto load = [&](int64_t base, uint64_t* k, uint32_t* v, int* bn) {
+#pragma unroll
+		for (int r = 0; r < kMsortRows; ++r)
+		{
+			const int64_t i = base + (wv * kMsortRows + r) * FH_WAVE + lane;
+			const bool act = i < n;
+			k[r] = act ? akey[b0 + i] : 0;
+			v[r] = act ? aval[b0 + i] : 0;
+			bn[r] = act ? static_cast<int>(k[r] >> 32) : -1;
+		}
+	};
+	load(0, key, val, bin);
 	for (int64_t t0 = 0; t0 < n; t0 += kMsortRows * kMsortThreads)
 	{
-		uint64_t key[kMsortRows];
-		uint32_t val[kMsortRows];
-		int bin[kMsortRows];
 		unsigned long long same[kMsortRows];
 #pragma unroll
-		for (int r = 0; r < kMsortRows; ++r)
-		{
-			const int64_t i = t0 + r * kMsortThreads + t;
-			const bool act = i < n;
-			key[r] = act ? akey[b0 + i] : 0;
-			val[r] = act ? aval[b0 + i] : 0;
-			bin[r] = act ? static_cast<int>(key[r] >> 32) : -1;
-		}
-#pragma unroll
 		for (int r = 0; r < kMsortRows; ++r) same[r] = sameBinMask(bin[r], bits);
-		// leaders claim slots in emission order: element t0 + r * 1024 + t
-		// comes before every element of a later (r, wave) turn
-		for (int r = 0; r < kMsortRows; ++r)
+		uint64_t nkey[kMsortRows];
+		uint32_t nval[kMsortRows];
+		int nbin[kMsortRows];
+		load(t0 + kMsortRows * kMsortThreads, nkey, nval, nbin);
+		// leaders claim slots in emission order: wave w holds the tile's
+		// elements [w * 256, (w + 1) * 256) as 4 rows of 64, so the waves take
+		// turns in order and each claims its rows in order (LDS operations of
+		// one wave complete in order)
+		for (int w = 0; w < kMsortWaves; ++w)
 		{
-			for (int w = 0; w < kMsortWaves; ++w)
+			if (wv == w)
 			{
-				if (wv == w && bin[r] >= 0 && (same[r] & lt) == 0)
+#pragma unroll
+				for (int r = 0; r < kMsortRows; ++r)
 				{
-					const uint32_t base = hist[bin[r]];
-					hist[bin[r]] = base + static_cast<uint32_t>(__popcll(same[r]));
-					lead[r * kMsortThreads + t] = base;
+					if (bin[r] >= 0 && (same[r] & lt) == 0)
+					{
+						const uint32_t base = hist[bin[r]];
+						hist[bin[r]] = base + static_cast<uint32_t>(__popcll(same[r]));
+						lead[r * kMsortThreads + t] = base;
+					}
 				}
-				__syncthreads();
 			}
+			ldsBarrier();
 		}
 #pragma unroll
 		for (int r = 0; r < kMsortRows; ++r)
@@ -161,7 +190,14 @@ __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int6
 			skey[d] = key[r];
 			sval[d] = val[r];
 		}
-		__syncthreads();  // lead[] is rewritten by the next tile
+		ldsBarrier();  // lead[] is rewritten by the next tile
+#pragma unroll
+		for (int r = 0; r < kMsortRows; ++r)
+		{
+			key[r] = nkey[r];
+			val[r] = nval[r];
+			bin[r] = nbin[r];
+		}
 	}
 }
 
@@ -202,7 +238,7 @@ hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBit
 	{
 		const size_t ldsBytes = static_cast<size_t>(nBins + kMsortFixedWords) * 4;
 		hipLaunchKernelGGL(matchCountSortKernel, dim3(static_cast<unsigned>(a.nQueries)), dim3(kMsortThreads), ldsBytes, s,
-						   a.qOff, a.akey, a.aval, a.skey, a.sval, nBins, idBits);
+						   a.qOff, a.order, a.akey, a.aval, a.skey, a.sval, nBins, idBits);
 		return hipGetLastError();
 	}
 	return hipcub::DeviceSegmentedRadixSort::SortPairs(temp, tempBytes, a.akey, a.skey, a.aval, a.sval,

@@ -17,6 +17,7 @@ struct MsortArgs
 	uint32_t* sval;        // stable sorted permutation (emission index within the query)
 	uint32_t* rkey;        // emission-order rank keys (filled here)
 	int32_t* dupFlag;      // per sorted position: a neighbouring position holds the same key
+	const int32_t* order;  // counting sort: queries by descending match count (null: in order)
 };
 
 hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t* bytes);

@@ -186,6 +186,7 @@ struct Lane
 	DevBuf bRawBases, bRawOff, bPackErr;  // ASCII bases being packed (packSeqsDevice)
 	DevBuf bGroupStart;                   // group starts of a batch's hits (score keys)
 	DevBuf bMinMask;                      // minimizer flags of the index build (one byte per window)
+	DevBuf bQOrder;                       // queries by descending match count (stable match sort)
 };
 
 // One flyehip_overlaps_submit call: its query records and, as batches finish,
@@ -732,6 +733,21 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 	while ((int64_t{1} << bits) < 2 * static_cast<int64_t>(std::max(1, c->nReads))) ++bits;
 	const int nBins = 2 * std::max(1, c->nReads);  // internal target ids
 	const bool counting = msortCounting(nBins);
+	if (counting && nq > 1)
+	{
+		// queries by descending match count for the counting sort's blocks
+		std::vector<int64_t> qo(static_cast<size_t>(nq) + 1);
+		download(L, qo.data(), ms.qOff, qo.size() * sizeof(int64_t));
+		sync(L);
+		std::vector<int32_t> order(static_cast<size_t>(nq));
+		for (int32_t i = 0; i < nq; ++i) order[static_cast<size_t>(i)] = i;
+		std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+			return qo[static_cast<size_t>(x) + 1] - qo[static_cast<size_t>(x)] > qo[static_cast<size_t>(y) + 1] - qo[static_cast<size_t>(y)];
+		});
+		int32_t* dOrder = ensureT<int32_t>(c, L->bQOrder, static_cast<size_t>(nq));
+		upload(L, dOrder, order.data(), order.size() * sizeof(int32_t));
+		ms.order = dOrder;
+	}
 	size_t tempBytes = 0;
 	if (!counting) HIPCHECK(msortTempBytes(nMatches, nq, bits, &tempBytes));
 	void* temp = counting ? nullptr : ensure(c, L->bMsortTemp, tempBytes + 256);
@@ -1090,7 +1106,7 @@ void releaseBatchBuffers(flyehip_ctx* c, Lane& l)
 					  &l.bSegShadowV, &l.bQueue, &l.bCounters, &l.bQRowCount, &l.bCurLen, &l.bDivOff, &l.bDivCount,
 					  &l.bDivOut, &l.bQueryIds, &l.bCand, &l.bExtWords, &l.bExtWordOff, &l.bExtLen, &l.bAln,
 					  &l.bAlnOff, &l.bRA, &l.bGroupDefer, &l.bFlagReady, &l.bFlagDefer, &l.bPosReady, &l.bPosDefer,
-					  &l.bReadyOrder, &l.bDeferOrder, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart})
+					  &l.bReadyOrder, &l.bDeferOrder, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart, &l.bMinMask, &l.bQOrder})
 		release(c, *b);
 }
 
