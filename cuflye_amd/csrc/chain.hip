@@ -16,6 +16,10 @@
 //           serial loop breaks (an improvement with jumpDiv == 0 && dc < k, or
 //           the sorted-axis delta passing maxJump); recent scores stay in an
 //           LDS ring per group;
+//   claim   one wave per group: the backtrack walks of Flye's score order
+//           computed without sorting (subtree maxima of the backtrack forest,
+//           below), overlapTest, divergence, selection; a group where equal
+//           scores could change the walks is left to
 //   finish  one wave per group: exact std::sort of the rows by score, the
 //           backtrack (lane 0, O(n)), overlapTest, divergence, selection.
 #include "chain.hpp"
@@ -711,6 +715,79 @@ __global__ void chainScorePushKernel(ChainArgs a, SegSortBufs sb)
 
 constexpr int kPropTile = 256;  // proposals sorted in LDS up to this count
 
+// Without onlyMaxExt: std::sort(extOverlaps, score desc) of the np proposals
+// props[0, np) (listed in walk order), the containment filter and the
+// divergence filter; the primaries go to outp.  sk / sv / posL / posR are the
+// group's per-hit scratch.  Returns the row count in every lane.
+__device__ int selectPrimaries(const Proposal* props, int np, uint32_t* sk, uint32_t* sv, int32_t* posL,
+							   int32_t* posR, Proposal* outp, const flyehip_overlap_params_t& p,
+							   WaveStdSortLds<uint32_t, kPropTile>& lds, int* red)
+{
+	const int lane = laneId();
+	for (int i = lane; i < np; i += FH_WAVE)
+	{
+		sk[i] = static_cast<uint32_t>(0x7fffffff - props[i].score);
+		sv[i] = static_cast<uint32_t>(i);
+	}
+	waveSync();
+	waveStdSort<uint32_t, kPropTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, np, posL, posR, lds);
+	if (lane == 0)
+	{
+		int32_t* prim = posL;  // free again after the sort
+		int nprim = 0;
+		for (int q = 0; q < np; ++q)
+		{
+			const Proposal& o = props[sv[q]];
+			bool contained = false;
+			for (int r = 0; r < nprim; ++r)
+			{
+				const Proposal& pr = props[prim[r]];
+				if (pr.curBegin <= o.curBegin && o.curEnd <= pr.curEnd && pr.extBegin <= o.extBegin &&
+					o.extEnd <= pr.extEnd && pr.score > o.score)
+				{
+					contained = true;
+					break;
+				}
+			}
+			if (!contained) prim[nprim++] = static_cast<int32_t>(sv[q]);
+		}
+		int w = 0;
+		for (int r = 0; r < nprim; ++r)
+			if (props[prim[r]].div < p.max_divergence) outp[w++] = props[prim[r]];
+		red[0] = w;
+	}
+	waveSync();
+	return red[0];
+}
+
+// The proposal of a walk that starts at row `start`, reaches row f and
+// passes len rows: positions, overlapTest, the filtered-position count and
+// the divergence.  Returns whether the proposal survives.
+__device__ inline bool makeProposal(Proposal& o, int start, int f, int len, const int32_t* hc, const int32_t* he,
+									const int32_t* sc, const QueryDesc& qd, const GroupDesc& gd,
+									const int32_t* filt, int filtN, const ChainArgs& a)
+{
+	const flyehip_overlap_params_t& p = a.params;
+	const int32_t k = p.kmer_size;
+	o.curBegin = hc[f];
+	o.curEnd = hc[start] + k - 1;
+	o.extBegin = he[f];
+	o.extEnd = he[start] + k - 1;
+	o.score = sc[start] - sc[f] + k - 1;
+	o.chainLength = len;
+	o.div = 0.0f;
+	if (!overlapTestDev(o, qd.curId, gd.extId, qd.curLen, gd.extLen, p)) return false;
+	const int fc = upperBound(filt, filtN, o.curEnd) - lowerBound(filt, filtN, o.curBegin);
+	const float normLen = static_cast<float>(max(o.curEnd - o.curBegin, o.extEnd - o.extBegin) - fc);
+	if (!(normLen > 0.0f)) return false;
+	float rate = static_cast<float>(len) * p.sample_rate / normLen;
+	rate = fminf(rate, 1.0f);
+	if (!(rate > 0.0f)) return false;
+	const float inv = 1.0f / rate;
+	o.div = flyeLogf(inv, a.logX, a.logY, a.logN) / static_cast<float>(k);
+	return true;
+}
+
 // Lane 0's walks for the live starts m of one round (start rows in
 // recStart[lane]): each walk follows the backtrack pointers from its start,
 // clearing them, and ends on the first row without one (a chain start, or a
@@ -768,7 +845,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 	__shared__ int red[2];
 	const int lane = laneId();
 	const flyehip_overlap_params_t& p = a.params;
-	const int32_t k = p.kmer_size;
 	for (;;)
 	{
 		const int qi = first + pullGroup(a.queue);
@@ -821,37 +897,8 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 			bool ok = false;
 			Proposal o;
 			if (lane < nrec)
-			{
-				const int start = recStart[lane];
-				const int f = recFirst[lane];
-				const int len = recLen[lane];
-				o.curBegin = hc[f];
-				o.curEnd = hc[start] + k - 1;
-				o.extBegin = he[f];
-				o.extEnd = he[start] + k - 1;
-				o.score = sc[start] - sc[f] + k - 1;
-				o.chainLength = len;
-				o.div = 0.0f;
-				ok = overlapTestDev(o, qd.curId, gd.extId, qd.curLen, gd.extLen, p);
-				if (ok)
-				{
-					const int fc = upperBound(filt, filtN, o.curEnd) - lowerBound(filt, filtN, o.curBegin);
-					const float normLen =
-						static_cast<float>(max(o.curEnd - o.curBegin, o.extEnd - o.extBegin) - fc);
-					ok = normLen > 0.0f;
-					if (ok)
-					{
-						float rate = static_cast<float>(len) * p.sample_rate / normLen;
-						rate = fminf(rate, 1.0f);
-						ok = rate > 0.0f;
-						if (ok)
-						{
-							const float inv = 1.0f / rate;
-							o.div = flyeLogf(inv, a.logX, a.logY, a.logN) / static_cast<float>(k);
-						}
-					}
-				}
-			}
+				ok = makeProposal(o, recStart[lane], recFirst[lane], recLen[lane], hc, he, sc, qd, gd, filt, filtN,
+								  a);
 			const unsigned long long okm = __ballot(ok);
 			if constexpr (kOnlyMax)
 			{
@@ -880,45 +927,242 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		}
 		else
 		{
-			// std::sort(extOverlaps, score desc), containment filter, divergence filter
 			__shared__ WaveStdSortLds<uint32_t, kPropTile> lds;
-			for (int i = lane; i < np; i += FH_WAVE)
-			{
-				sk[i] = static_cast<uint32_t>(0x7fffffff - props[i].score);
-				sv[i] = static_cast<uint32_t>(i);
-			}
-			waveSync();
-			waveStdSort<uint32_t, kPropTile, kLaneMax>(exsort::Span<uint32_t>{sk, sv}, np, a.posL + hb, a.posR + hb,
-													   lds);
-			if (lane == 0)
-			{
-				int32_t* prim = a.posL + hb;  // free again after the sort
-				int nprim = 0;
-				for (int q = 0; q < np; ++q)
-				{
-					const Proposal& o = props[sv[q]];
-					bool contained = false;
-					for (int r = 0; r < nprim; ++r)
-					{
-						const Proposal& pr = props[prim[r]];
-						if (pr.curBegin <= o.curBegin && o.curEnd <= pr.curEnd && pr.extBegin <= o.extBegin &&
-							o.extEnd <= pr.extEnd && pr.score > o.score)
-						{
-							contained = true;
-							break;
-						}
-					}
-					if (!contained) prim[nprim++] = static_cast<int32_t>(sv[q]);
-				}
-				int w = 0;
-				for (int r = 0; r < nprim; ++r)
-					if (props[prim[r]].div < p.max_divergence) outp[w++] = props[prim[r]];
-				red[1] = w;
-			}
-			waveSync();
-			count = red[1];
+			count = selectPrimaries(props, np, sk, sv, a.posL + hb, a.posR + hb, outp, p, lds, &red[1]);
 		}
 		if (lane == 0) a.groupCount[g] = count;
+		waveSync();
+	}
+}
+
+// ---- claim walk ------------------------------------------------------------
+//
+// The walk visits the rows with a backtrack pointer ("relevant" rows; every
+// other row is a chain root) in std::sort's score order, follows each start's
+// pointers and clears them, so the order only matters where two of those rows
+// share a score.  Let D(v) be the largest relevant score in v's subtree of the
+// backtrack forest (v included).  Walks run towards lower rows, so a row is
+// first reached by the highest-priority relevant row of its subtree: when no
+// two relevant rows of equal score can compete (below), v is claimed by the
+// row whose score is D(v), a start survives exactly when no descendant
+// scores at least as high, and its chain is the claimed path up to its
+// lowest row t plus t's parent (reached earlier by a higher walk) unless t is
+// a root.  Ties can change the result only between "weakly alive" rows (no
+// strictly higher descendant): such a row with an equal-scoring descendant,
+// or two surviving starts of equal score, mark the group for the exact
+// std::sort path.  Nothing here needs the order, so the score sort is skipped.
+//
+// D is computed one 64-row chunk at a time from the top: the chunk's rows
+// take their own score and what children in higher chunks pushed (acc, in
+// the group's free per-hit scratch), pointer doubling over the in-chunk
+// parents spreads the maxima up in six LDS rounds, and rows whose parent lies
+// below the chunk push their D there.  acc is read and written with
+// workgroup-scope atomics (the group's one wave is the workgroup), ordered by
+// a workgroup fence between chunks.
+constexpr int kClaimMaxAlive = FH_WAVE;  // surviving starts a group may hold (one proposal per lane)
+
+__device__ inline uint32_t accLoad(uint32_t* p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void accMax(uint32_t* p, uint32_t v)
+{
+	(void)__hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// One wave per group of passOrder[0, nPass); groups it cannot decide get
+// claimFlag[g] = 1 (and no rows), the others groupCount and their proposals.
+template <bool kOnlyMax>
+__global__ void __launch_bounds__(FH_WAVE) chainClaimKernel(ChainArgs a, int32_t* claimFlag)
+{
+	__shared__ uint32_t val[FH_WAVE];   // D of the chunk's rows (doubling)
+	__shared__ int jmp[FH_WAVE];        // 2^r-th in-chunk ancestor, -1 once outside
+	__shared__ uint32_t cmax[FH_WAVE];  // max D over a row's children
+	__shared__ int aIdx[kClaimMaxAlive];
+	__shared__ int aScore[kClaimMaxAlive];
+	__shared__ int aCnt[kClaimMaxAlive];
+	__shared__ int aTop[kClaimMaxAlive];
+	__shared__ int red[2];
+	const int lane = laneId();
+	const flyehip_overlap_params_t& p = a.params;
+	for (;;)
+	{
+		const int qi = pullGroup(a.queue);
+		if (qi >= a.nPass) break;
+		const int g = a.passOrder[qi];
+		const GroupDesc gd = a.groups[g];
+		const QueryDesc qd = a.queries[gd.query];
+		const int n = __builtin_amdgcn_readfirstlane(gd.n);
+		const int64_t hb = gd.hitStart;
+		const bool extSorted = gd.extLen > qd.curLen;
+		const int32_t* hc = groupHits(a.hcur, a.cur, hb, extSorted);
+		const int32_t* he = groupHits(a.hext, a.ext, hb, extSorted);
+		const int32_t* sc = a.score + hb;
+		const int32_t* bt = a.bt + hb;
+		uint32_t* acc = reinterpret_cast<uint32_t*>(a.posL + hb);
+		for (int i = lane; i < n; i += FH_WAVE) acc[i] = 0u;
+		__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+		int nAlive = 0;
+		bool flagged = false;
+		for (int t = (n - 1) & ~(FH_WAVE - 1); t >= 0 && !flagged; t -= FH_WAVE)
+		{
+			const int v = t + lane;
+			const bool valid = v < n;
+			int b = -1, s = 0;
+			uint32_t a0 = 0;
+			if (valid)
+			{
+				b = bt[v];
+				s = sc[v];
+				a0 = accLoad(acc + v);
+			}
+			const bool rel = b != -1;
+			const int j0 = (rel && b >= t) ? b - t : -1;
+			val[lane] = max(rel ? static_cast<uint32_t>(s) : 0u, a0);
+			jmp[lane] = j0;
+			cmax[lane] = a0;
+			waveSync();
+			int jj = j0;
+			for (int r = 0; r < 6; ++r)
+			{
+				if (__ballot(jj >= 0) == 0) break;
+				const uint32_t mine = val[lane];
+				if (jj >= 0) atomicMax(&val[jj], mine);
+				const int nj = jj >= 0 ? jmp[jj] : -1;
+				__builtin_amdgcn_wave_barrier();
+				jmp[lane] = nj;
+				jj = nj;
+				__builtin_amdgcn_wave_barrier();
+			}
+			waveSync();
+			const uint32_t d = valid ? val[lane] : 0u;
+			if (j0 >= 0) atomicMax(&cmax[j0], d);
+			if (rel && b < t) accMax(acc + b, d);
+			waveSync();
+			const uint32_t dex = cmax[lane];
+			// surviving starts (registered before the chunk's claims are counted)
+			const bool tie = rel && dex == static_cast<uint32_t>(s);
+			const bool alive = rel && dex < static_cast<uint32_t>(s);
+			const unsigned long long am = __ballot(alive);
+			const int na = __popcll(am);
+			if (__ballot(tie) != 0 || nAlive + na > kClaimMaxAlive)
+			{
+				flagged = true;
+				break;
+			}
+			if (alive)
+			{
+				const int slot = nAlive + __popcll(am & laneMaskLt());
+				aIdx[slot] = v;
+				aScore[slot] = s;
+				aCnt[slot] = 0;
+				aTop[slot] = v;
+			}
+			nAlive += na;
+			waveSync();
+			// claims: every row with a relevant row in its subtree, grouped by claimer
+			const bool claimed = valid && d != 0u;
+			unsigned long long m = __ballot(claimed);
+			while (m)
+			{
+				const int l0 = __ffsll(static_cast<long long>(m)) - 1;
+				const uint32_t c = __shfl(d, l0, FH_WAVE);
+				const unsigned long long mc = __ballot(claimed && d == c);
+				const unsigned long long hm = __ballot(lane < nAlive && static_cast<uint32_t>(aScore[lane]) == c);
+				if (hm == 0)
+				{
+					flagged = true;
+					break;
+				}
+				const int slot = __ffsll(static_cast<long long>(hm)) - 1;
+				if (lane == 0)
+				{
+					aCnt[slot] += __popcll(mc);
+					aTop[slot] = t + __ffsll(static_cast<long long>(mc)) - 1;
+				}
+				m &= ~mc;
+				waveSync();
+			}
+			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+		}
+		// two surviving starts of equal score
+		if (!flagged && nAlive > 1)
+		{
+			bool dup = false;
+			if (lane < nAlive)
+				for (int r = lane + 1; r < nAlive; ++r) dup |= aScore[r] == aScore[lane];
+			flagged = __ballot(dup) != 0;
+		}
+		if (flagged)
+		{
+			if (lane == 0) claimFlag[g] = 1;
+			continue;
+		}
+		// one proposal per surviving start; the walk order is score-descending
+		const int32_t* filt = a.filtered + qd.filtStart;
+		const int filtN = qd.filtCount;
+		bool ok = false;
+		Proposal o;
+		int start = 0;
+		if (lane < nAlive)
+		{
+			start = aIdx[lane];
+			const int top = aTop[lane];
+			const int pt = bt[top];
+			const int f = pt != -1 ? pt : top;
+			const int len = aCnt[lane] + (pt != -1 ? 1 : 0);
+			ok = makeProposal(o, start, f, len, hc, he, sc, qd, gd, filt, filtN, a);
+		}
+		const int startScore = ok ? sc[start] : -1;
+		int count = 0;
+		if constexpr (kOnlyMax)
+		{
+			// the first proposal of maximal score in walk order: the highest
+			// start score among the maximal ones
+			const unsigned long long key =
+				ok ? ((static_cast<unsigned long long>(static_cast<uint32_t>(o.score) ^ 0x80000000u) << 32) |
+					  static_cast<uint32_t>(startScore))
+				   : 0ull;
+			unsigned long long best = key;
+			for (int off = 32; off > 0; off >>= 1)
+			{
+				const unsigned long long other = __shfl_xor(best, off, FH_WAVE);
+				best = other > best ? other : best;
+			}
+			const unsigned long long bm = __ballot(ok && key == best);
+			if (bm)
+			{
+				const int l = __ffsll(static_cast<long long>(bm)) - 1;
+				if (lane == l)
+				{
+					a.outProps[g] = o;
+					red[0] = o.div < p.max_divergence ? 1 : 0;
+				}
+				waveSync();
+				count = red[0];
+			}
+		}
+		else
+		{
+			// proposals in walk order (start scores are distinct here)
+			aScore[lane] = startScore;
+			waveSync();
+			int rank = 0;
+			if (ok)
+				for (int r = 0; r < nAlive; ++r) rank += aScore[r] > startScore ? 1 : 0;
+			const int np = __popcll(__ballot(ok));
+			Proposal* props = a.props + hb;
+			if (ok) props[rank] = o;
+			waveSync();
+			__shared__ WaveStdSortLds<uint32_t, kPropTile> lds;
+			count = selectPrimaries(props, np, a.sortKey + hb, a.sortVal + hb, a.posL + hb, a.posR + hb,
+									a.outProps + hb, p, lds, &red[1]);
+		}
+		if (lane == 0)
+		{
+			a.groupCount[g] = count;
+			claimFlag[g] = 0;
+		}
 		waveSync();
 	}
 }
@@ -1130,6 +1374,17 @@ hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, int 
 	const unsigned grid = static_cast<unsigned>(min(blocks, last - first));
 	if (a.params.only_max_ext) launchWalkTier<true>(a, first, last, tier, grid, stream);
 	else launchWalkTier<false>(a, first, last, tier, grid, stream);
+	return hipGetLastError();
+}
+
+hipError_t launchChainClaim(const ChainArgs& a, int32_t* claimFlag, int blocks, hipStream_t stream)
+{
+	if (a.nPass <= 0) return hipSuccess;
+	const unsigned grid = static_cast<unsigned>(min(blocks, a.nPass));
+	if (a.params.only_max_ext)
+		hipLaunchKernelGGL(chainClaimKernel<true>, dim3(grid), dim3(FH_WAVE), 0, stream, a, claimFlag);
+	else
+		hipLaunchKernelGGL(chainClaimKernel<false>, dim3(grid), dim3(FH_WAVE), 0, stream, a, claimFlag);
 	return hipGetLastError();
 }
 

@@ -121,6 +121,11 @@ hipError_t launchChainScorePush(const ChainArgs& a, const SegSortBufs& sb, hipSt
 // tier: 0 below kWalkSmall hits, 1 below kWalkMid, 2 below kWalkBig, 3 above
 hipError_t launchChainWalk(const ChainArgs& a, int32_t first, int32_t last, int tier, int blocks,
 						   hipStream_t stream);
+// Backtrack .. selection of passOrder[0, nPass) without the score sort,
+// for groups whose walk does not depend on the order of equal scores;
+// claimFlag[g] = 1 marks the others (no rows written), which take the score
+// sort and launchChainWalk.  Uses posL as scratch.
+hipError_t launchChainClaim(const ChainArgs& a, int32_t* claimFlag, int blocks, hipStream_t stream);
 hipError_t launchWriteRows(const ChainArgs& a, const int64_t* rowStart, flyehip_overlap_t* rows,
 						   hipStream_t stream);
 

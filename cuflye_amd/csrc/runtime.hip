@@ -177,6 +177,7 @@ struct Lane
 	DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
 	DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
 	DevBuf bGroupDefer, bFlagReady, bFlagDefer, bPosReady, bPosDefer, bReadyOrder, bDeferOrder;
+	DevBuf bClaimFlag, bFlagOrder;
 	DevBuf bGroupPass, bOrder, bPassOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
 	DevBuf bSegBig0, bSegBig1, bSegLeaf, bSegHeap, bSegCounters, bSegChunkOff, bSegChunkRange, bSegChunkCnt;
 	DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp, bSegVerify, bSegShadowK, bSegShadowV;
@@ -797,11 +798,12 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	int32_t* flag = ensureT<int32_t>(c, L->bGroupFlag, static_cast<size_t>(ng) + 1);
 	int64_t* pos = ensureT<int64_t>(c, L->bGroupOff, static_cast<size_t>(ng) + 1);
 	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(std::max(ng, nHits)) + 64);
-	int32_t* queues = ensureT<int32_t>(c, L->bQueue, 16);
+	int32_t* queues = ensureT<int32_t>(c, L->bQueue, 16);  // work-queue counters 0-10
 	int64_t* nBig = ensureT<int64_t>(c, L->bCounters, 16);
 	zero32(L, bins, 96);
 	zero32(L, queues, 16);
-	HIPCHECK(hipMemsetAsync(nBig, 0, 9 * sizeof(int64_t), L->stream));  // nBig[8]: DP cells
+	// nBig[8]: DP cells; nBig[9..12]: groups the claim walk leaves to the ordered walk
+	HIPCHECK(hipMemsetAsync(nBig, 0, 13 * sizeof(int64_t), L->stream));
 	SegSortBufs sb = segBufs(L, nHits, ng);
 	sb.key = ca.sortKey;
 	sb.val = ca.sortVal;
@@ -839,7 +841,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	runSegSort(L, sb, nullptr, &exactPending);
 	HIPCHECK(launchPassFlags(ca, flag, L->stream));
 	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
-	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream, nBig + 6));
+	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream));
 	int32_t* readyOrder = passOrder;
 	int32_t* deferOrder = nullptr;
 	int64_t* posReady = nullptr;
@@ -862,10 +864,8 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	}
 	int64_t counts[3] = {0, 0, 0};
 	int64_t split[5] = {0, 0, 0, 0, 0};  // ready (big groups, hits), deferred (big groups, hits), ready groups
-	int64_t walkTiers[2] = {0, 0};       // passing groups of at least kWalkBig / kWalkMid hits
 	download(L, &counts[0], pos + ng, sizeof(int64_t));
 	download(L, &counts[1], nBig, 2 * sizeof(int64_t));
-	download(L, walkTiers, nBig + 6, 2 * sizeof(int64_t));
 	if (exactPending)
 	{
 		download(L, split, nBig + 2, 4 * sizeof(int64_t));
@@ -873,7 +873,6 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	}
 	sync(L);
 	const int32_t nPassAll = static_cast<int32_t>(counts[0]);
-	const int32_t nBigPass = static_cast<int32_t>(counts[1]);
 	const double passHits = static_cast<double>(counts[2]);
 	const int32_t nReady = exactPending ? static_cast<int32_t>(split[4]) : nPassAll;
 	const double readyHits = exactPending ? static_cast<double>(split[1]) : passHits;
@@ -906,27 +905,60 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 		ca.nPass = nPassAll;
 	}
 	HIPCHECK(hipEventRecord(L->ev[4], L->stream));
-	HIPCHECK(segSortReset(sb, L->stream));
-	sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
-	ca.queue = queues + 3;
-	// the DP wrote the score-order keys with its rows: push the ranges
-	const int tsk = L->timer.begin(FLYEHIP_K_SCORE_KEYS, L->stream);
-	HIPCHECK(launchChainScorePush(ca, sb, L->stream));
-	L->timer.end(tsk, L->stream);
-	runSegSort(L, sb, nullptr);
-	ca.queue = queues + 4;
-	// walk: score order and backtrack pointers read, 8 bytes per hit
-	const int tw = L->timer.begin(FLYEHIP_K_WALK, L->stream, 8.0 * passHits);
-	// size classes of the processing order (size-descending by power of two)
-	const int32_t nLarge = static_cast<int32_t>(walkTiers[0]), nMid = static_cast<int32_t>(walkTiers[1]);
-	HIPCHECK(launchChainWalk(ca, 0, nLarge, 3, blocks, L->stream));
-	ca.queue = queues + 8;
-	HIPCHECK(launchChainWalk(ca, nLarge, nMid, 2, blocks, L->stream));
-	ca.queue = queues + 9;
-	HIPCHECK(launchChainWalk(ca, nMid, nBigPass, 1, blocks, L->stream));
-	ca.queue = queues + 5;
-	HIPCHECK(launchChainWalk(ca, nBigPass, ca.nPass, 0, blocks, L->stream));
-	L->timer.end(tw, L->stream);
+	// backtrack .. selection without the score sort for every passing group
+	// whose walk cannot depend on the order of equal scores; the flagged
+	// groups (claimFlag) take the exact score sort and the ordered walk
+	int32_t* claimFlag = ensureT<int32_t>(c, L->bClaimFlag, static_cast<size_t>(ng) + 1);
+	ca.queue = queues + 10;
+	// claim: score and backtrack read, 8 bytes per passing hit
+	const int tcl = L->timer.begin(FLYEHIP_K_CLAIM, L->stream, 8.0 * passHits);
+	HIPCHECK(launchChainClaim(ca, claimFlag, blocks, L->stream));
+	L->timer.end(tcl, L->stream);
+	int32_t* flagOrder = ensureT<int32_t>(c, L->bFlagOrder, static_cast<size_t>(ng) + 1);
+	{
+		ChainArgs fa = ca;
+		fa.order = passOrder;
+		fa.groupPass = claimFlag;
+		fa.nGroups = nPassAll;
+		HIPCHECK(launchPassFlags(fa, flag, L->stream));
+		HIPCHECK(exclusiveScan(flag, pos, nPassAll, tmp, L->stream));
+		HIPCHECK(launchPassScatter(fa, flag, pos, flagOrder, nBig + 9, L->stream, nBig + 11));
+	}
+	int64_t fc[4] = {0, 0, 0, 0};  // flagged groups of at least kWalkSmall hits, their hits, tiers
+	int64_t nFlagged = 0;
+	download(L, fc, nBig + 9, 4 * sizeof(int64_t));
+	download(L, &nFlagged, pos + nPassAll, sizeof(int64_t));
+	sync(L);
+	L->stats.exact_walk_groups += nFlagged;
+	if (nFlagged > 0)
+	{
+		ca.passOrder = flagOrder;
+		ca.nPass = static_cast<int32_t>(nFlagged);
+		HIPCHECK(segSortReset(sb, L->stream));
+		sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
+		ca.queue = queues + 3;
+		// the DP wrote the score-order keys with its rows: push the ranges
+		const int tsk = L->timer.begin(FLYEHIP_K_SCORE_KEYS, L->stream);
+		HIPCHECK(launchChainScorePush(ca, sb, L->stream));
+		L->timer.end(tsk, L->stream);
+		runSegSort(L, sb, nullptr);
+		ca.queue = queues + 4;
+		// walk: score order and backtrack pointers read, 8 bytes per hit
+		const int tw = L->timer.begin(FLYEHIP_K_WALK, L->stream, 8.0 * static_cast<double>(fc[1]));
+		// size classes of the processing order (size-descending by power of two)
+		const int32_t nLarge = static_cast<int32_t>(fc[2]), nMid = static_cast<int32_t>(fc[3]);
+		const int32_t nBigF = static_cast<int32_t>(fc[0]);
+		HIPCHECK(launchChainWalk(ca, 0, nLarge, 3, blocks, L->stream));
+		ca.queue = queues + 8;
+		HIPCHECK(launchChainWalk(ca, nLarge, nMid, 2, blocks, L->stream));
+		ca.queue = queues + 9;
+		HIPCHECK(launchChainWalk(ca, nMid, nBigF, 1, blocks, L->stream));
+		ca.queue = queues + 5;
+		HIPCHECK(launchChainWalk(ca, nBigF, ca.nPass, 0, blocks, L->stream));
+		L->timer.end(tw, L->stream);
+		ca.passOrder = passOrder;
+		ca.nPass = nPassAll;
+	}
 	int64_t dpCells = 0;
 	download(L, &dpCells, nBig + 8, sizeof(int64_t));
 	HIPCHECK(hipEventRecord(L->ev[5], L->stream));
@@ -1066,6 +1098,7 @@ void foldStats(flyehip_ctx* c)
 		c->stats.overlaps += s.overlaps;
 		c->stats.batches += s.batches;
 		c->stats.tie_queries += s.tie_queries;
+		c->stats.exact_walk_groups += s.exact_walk_groups;
 		c->stats.dp_cells += s.dp_cells;
 		for (int f = 0; f < FLYEHIP_K_COUNT; ++f)
 		{
@@ -1106,7 +1139,7 @@ void releaseBatchBuffers(flyehip_ctx* c, Lane& l)
 					  &l.bSegShadowV, &l.bQueue, &l.bCounters, &l.bQRowCount, &l.bCurLen, &l.bDivOff, &l.bDivCount,
 					  &l.bDivOut, &l.bQueryIds, &l.bCand, &l.bExtWords, &l.bExtWordOff, &l.bExtLen, &l.bAln,
 					  &l.bAlnOff, &l.bRA, &l.bGroupDefer, &l.bFlagReady, &l.bFlagDefer, &l.bPosReady, &l.bPosDefer,
-					  &l.bReadyOrder, &l.bDeferOrder, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart, &l.bMinMask, &l.bQOrder})
+					  &l.bReadyOrder, &l.bDeferOrder, &l.bClaimFlag, &l.bFlagOrder, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart, &l.bMinMask, &l.bQOrder})
 		release(c, *b);
 }
 

@@ -72,7 +72,7 @@ class Stats(C.Structure):
                 ("groups", C.c_int64), ("overlaps", C.c_int64), ("batches", C.c_int64),
                 ("device_bytes_peak", C.c_int64), ("tie_queries", C.c_int64),
                 ("kernel_ms", C.c_double * 16), ("kernel_launches", C.c_int64 * 16),
-                ("kernel_bytes", C.c_double * 16), ("dp_cells", C.c_int64), ("reserved_stats", C.c_int64)]
+                ("kernel_bytes", C.c_double * 16), ("dp_cells", C.c_int64), ("exact_walk_groups", C.c_int64)]
 
     def as_dict(self) -> dict:
         out = {}
@@ -84,11 +84,11 @@ class Stats(C.Structure):
 
 # Kernels of flyehip_stats_t.kernel_* (FLYEHIP_K_* order in include/flyehip.h).
 KERNEL_FAMILIES = ("join_count", "join_fill", "radix", "rank_keys", "leaf", "heap", "prefilter", "dp", "walk",
-                   "round_count", "round_rank", "round_swap", "round_other", "gather", "score_keys", "unused")
+                   "round_count", "round_rank", "round_swap", "round_other", "gather", "score_keys", "claim")
 KERNEL_NAMES = {
     "join_count": "joinCountKernel",
     "join_fill": "joinFillKernel",
-    "radix": "rocprim segmented radix sort (hipcub DeviceSegmentedRadixSort)",
+    "radix": "matchCountSortKernel (hipcub DeviceSegmentedRadixSort above its target-id limit)",
     "rank_keys": "rankKeysKernel",
     "leaf": "segLeafKernel",
     "heap": "segHeapLdsKernel (+ segHeapGlobalKernel)",
@@ -101,11 +101,11 @@ KERNEL_NAMES = {
     "round_other": "segMedian/Map/SplitKernel + chunk scans",
     "gather": "gatherHitsKernel",
     "score_keys": "chainScorePushKernel",
-    "unused": "-",
+    "claim": "chainClaimKernel<onlyMaxExt>",
 }
 # Kernels that are one HIP kernel (a roofline names one of these).
 SINGLE_KERNELS = ("join_count", "join_fill", "rank_keys", "leaf", "heap", "prefilter", "dp", "walk", "round_count",
-                  "round_rank", "round_swap", "gather", "score_keys")
+                  "round_rank", "round_swap", "gather", "score_keys", "claim")
 
 
 CANDIDATE_DTYPE = np.dtype([("query_id", "<i8"), ("target_id", "<i8"), ("kmer", "<u8"),

@@ -166,8 +166,9 @@ typedef struct {
 /* Kernels timed in flyehip_stats_t (one HIP kernel each unless noted). */
 #define FLYEHIP_K_JOIN_COUNT 0   /* joinCountKernel: windows per query              */
 #define FLYEHIP_K_JOIN_FILL 1    /* joinFillKernel: k-mer matches in emission order */
-#define FLYEHIP_K_RADIX 2        /* stable segmented radix sort of the matches (the
-                                    library sort's own kernels)                     */
+#define FLYEHIP_K_RADIX 2        /* stable (extId, curPos) order of the matches:
+                                    matchCountSortKernel (the library radix sort
+                                    above its target-id limit)                      */
 #define FLYEHIP_K_RANK_KEYS 3    /* rankKeysKernel: tie flags and rank keys         */
 #define FLYEHIP_K_LEAF 4         /* segLeafKernel: exact leaf sorts in LDS          */
 #define FLYEHIP_K_HEAP 5         /* segHeapLdsKernel / segHeapGlobalKernel          */
@@ -179,7 +180,9 @@ typedef struct {
 #define FLYEHIP_K_ROUND_SWAP 11  /* segSwapKernel                                   */
 #define FLYEHIP_K_ROUND_OTHER 12 /* segMedian/Map/SplitKernel and the chunk scans   */
 #define FLYEHIP_K_GATHER 13      /* gatherHitsKernel: group-order hits              */
-#define FLYEHIP_K_SCORE_KEYS 14  /* chainScoreKeysKernel                            */
+#define FLYEHIP_K_SCORE_KEYS 14  /* chainScorePushKernel                            */
+#define FLYEHIP_K_CLAIM 15       /* chainClaimKernel<onlyMaxExt>: backtrack .. selection
+                                    without the score sort                          */
 #define FLYEHIP_K_COUNT 16
 
 /* Timing and counters of the last call (milliseconds on the library stream). */
@@ -207,7 +210,7 @@ typedef struct {
     int64_t kernel_launches[FLYEHIP_K_COUNT];
     double kernel_bytes[FLYEHIP_K_COUNT];
     int64_t dp_cells;        /* (row, predecessor) pairs the chaining DP evaluated */
-    int64_t reserved_stats;
+    int64_t exact_walk_groups; /* passing groups whose walk needed the exact score sort */
 } flyehip_stats_t;
 
 /* ---- context ---------------------------------------------------------- */
