@@ -462,8 +462,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 	int32_t* sc = nullptr;
 	int32_t* bt = nullptr;
 	int32_t* pmG = nullptr;
-	uint32_t* sk = nullptr;  // score-order sort keys and values of the group's rows
-	uint32_t* sv = nullptr;
 	int pm = 0;
 	// rows are stored G at a time: lane (r mod G) of the subgroup holds row r's
 	// score, backtrack and prefix maximum until its G-row block is complete,
@@ -508,10 +506,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			sc = a.score + gd.hitStart;
 			bt = a.bt + gd.hitStart;
 			pmG = a.posR + gd.hitStart;  // free during the DP: holds max(score[0..j])
-			// the extPos sort's keys and permutation are dead once the group's
-			// hits are gathered: they receive the score-order sort's input
-			sk = a.sortKey + gd.hitStart;
-			sv = a.sortVal + gd.hitStart;
 			pm = 0;
 			winC = hc[0];  // row 0, for lane 0 of row 1's first tile
 			winE = he[0];
@@ -529,8 +523,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 				sc[0] = 0;
 				bt[0] = -1;
 				pmG[0] = 0;
-				sk[0] = scoreOrderKey(0);
-				sv[0] = 0;
 			}
 			hs = 0;  // row 0, held by lane 0
 			hbt = -1;
@@ -658,8 +650,6 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 					sc[r] = hs;
 					bt[r] = hbt;
 					pmG[r] = hpm;
-					sk[r] = scoreOrderKey(hs);
-					sv[r] = static_cast<uint32_t>(r) | (hbt != -1 ? kWalkRelevant : 0u);
 				}
 			}
 			const int32_t nc = i + 1 < n ? hc[i + 1] : 0;
@@ -697,6 +687,27 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 
 // ---- finish ----------------------------------------------------------------------
 
+
+// The score sort's input for the groups of passOrder[0, nPass) (one wave per
+// group): std::sort(indices, score desc) as ascending keys, the row index as
+// value with the relevance bit on rows that have a backtrack pointer (the
+// extPos sort's keys and permutation are dead once the hits are gathered).
+__global__ void __launch_bounds__(FH_WAVE) chainScoreKeysKernel(ChainArgs a)
+{
+	const int qi = blockIdx.x;
+	if (qi >= a.nPass) return;
+	const GroupDesc gd = a.groups[a.passOrder[qi]];
+	const int64_t hb = gd.hitStart;
+	const int32_t* sc = a.score + hb;
+	const int32_t* bt = a.bt + hb;
+	uint32_t* sk = a.sortKey + hb;
+	uint32_t* sv = a.sortVal + hb;
+	for (int i = laneId(); i < gd.n; i += FH_WAVE)
+	{
+		sk[i] = scoreOrderKey(sc[i]);
+		sv[i] = static_cast<uint32_t>(i) | (bt[i] != -1 ? kWalkRelevant : 0u);
+	}
+}
 
 // Pushes every passing group's range to the score sort (one lane per group).
 __global__ void chainScorePushKernel(ChainArgs a, SegSortBufs sb)
@@ -1324,7 +1335,10 @@ hipError_t launchChainScorePush(const ChainArgs& a, const SegSortBufs& sb, hipSt
 {
 	const int tpb = 256;
 	if (a.nPass > 0)
+	{
+		hipLaunchKernelGGL(chainScoreKeysKernel, dim3(static_cast<unsigned>(a.nPass)), dim3(FH_WAVE), 0, stream, a);
 		hipLaunchKernelGGL(chainScorePushKernel, dim3((a.nPass + tpb - 1) / tpb), dim3(tpb), 0, stream, a, sb);
+	}
 	return hipGetLastError();
 }
 

@@ -114,8 +114,8 @@ hipError_t launchSplitPass(const ChainArgs& a, const int32_t* deferred, int32_t*
 						   hipStream_t stream);
 hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream);
 hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t stream);
-// pushes the passing groups' score-order ranges to sb (the DP writes their
-// keys and values into sortKey / sortVal with its rows)
+// writes the score-order keys and values of the groups passOrder[0, nPass)
+// into sortKey / sortVal and pushes their ranges to sb
 hipError_t launchChainScorePush(const ChainArgs& a, const SegSortBufs& sb, hipStream_t stream);
 // backtrack .. selection for passOrder[first, last)
 // tier: 0 below kWalkSmall hits, 1 below kWalkMid, 2 below kWalkBig, 3 above

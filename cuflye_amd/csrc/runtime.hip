@@ -885,7 +885,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	HIPCHECK(hipEventRecord(L->ev[3], L->stream));
 	ca.queue = queues + 2;
 	// DP: hits read (hcur, hext) and score / backtrack written, 16 bytes per
-	// hit (plus the score-order keys it writes for the sort that follows)
+	// hit (plus the prefix maxima it keeps for its far lookbacks)
 	const int td = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * readyHits);
 	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, L->stream));
 	L->timer.end(td, L->stream);
@@ -937,7 +937,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 		HIPCHECK(segSortReset(sb, L->stream));
 		sb.relevantBit = 0x80000000u;  // rows with a backtrack pointer (chain.hip)
 		ca.queue = queues + 3;
-		// the DP wrote the score-order keys with its rows: push the ranges
+		// score-order keys from the DP's rows, then the ranges
 		const int tsk = L->timer.begin(FLYEHIP_K_SCORE_KEYS, L->stream);
 		HIPCHECK(launchChainScorePush(ca, sb, L->stream));
 		L->timer.end(tsk, L->stream);

@@ -100,12 +100,12 @@ KERNEL_NAMES = {
     "round_swap": "segSwapKernel",
     "round_other": "segMedian/Map/SplitKernel + chunk scans",
     "gather": "gatherHitsKernel",
-    "score_keys": "chainScorePushKernel",
+    "score_keys": "chainScoreKeysKernel + chainScorePushKernel",
     "claim": "chainClaimKernel<onlyMaxExt>",
 }
 # Kernels that are one HIP kernel (a roofline names one of these).
 SINGLE_KERNELS = ("join_count", "join_fill", "rank_keys", "leaf", "heap", "prefilter", "dp", "walk", "round_count",
-                  "round_rank", "round_swap", "gather", "score_keys", "claim")
+                  "round_rank", "round_swap", "gather", "claim")
 
 
 CANDIDATE_DTYPE = np.dtype([("query_id", "<i8"), ("target_id", "<i8"), ("kmer", "<u8"),

@@ -180,7 +180,7 @@ typedef struct {
 #define FLYEHIP_K_ROUND_SWAP 11  /* segSwapKernel                                   */
 #define FLYEHIP_K_ROUND_OTHER 12 /* segMedian/Map/SplitKernel and the chunk scans   */
 #define FLYEHIP_K_GATHER 13      /* gatherHitsKernel: group-order hits              */
-#define FLYEHIP_K_SCORE_KEYS 14  /* chainScorePushKernel                            */
+#define FLYEHIP_K_SCORE_KEYS 14  /* chainScoreKeysKernel + chainScorePushKernel     */
 #define FLYEHIP_K_CLAIM 15       /* chainClaimKernel<onlyMaxExt>: backtrack .. selection
                                     without the score sort                          */
 #define FLYEHIP_K_COUNT 16
