@@ -117,6 +117,50 @@ struct Index
 	std::vector<int32_t> readLen;     // by read number - 1
 	int64_t distinct = 0;
 	std::vector<int64_t> hist;        // hist[c] = distinct k-mers with count c
+	// hash lookup (Flye's VertexIndex is a hash table): open addressing,
+	// linear probing, canonical k-mer -> bucket b, or kRepetitive
+	std::vector<uint64_t> hashKey;
+	std::vector<int64_t> hashVal;     // -1 = empty slot
+	uint64_t hashMask = 0;
+	static constexpr int64_t kRepetitive = -2;
+
+	static uint64_t mix(uint64_t x)
+	{
+		x ^= x >> 33;
+		x *= 0xff51afd7ed558ccdULL;
+		x ^= x >> 33;
+		return x;
+	}
+	void buildLookup()
+	{
+		const size_t n = kmers.size() + repetitive.size();
+		size_t cap = 16;
+		while (cap < 2 * n + 1) cap <<= 1;
+		hashKey.assign(cap, 0);
+		hashVal.assign(cap, -1);
+		hashMask = cap - 1;
+		auto put = [&](uint64_t key, int64_t v) {
+			uint64_t h = mix(key) & hashMask;
+			while (hashVal[h] != -1) h = (h + 1) & hashMask;
+			hashKey[h] = key;
+			hashVal[h] = v;
+		};
+		// repetitive first: a k-mer given as both is filtered (probes stop at
+		// the first slot holding the key)
+		for (uint64_t r : repetitive) put(r, kRepetitive);
+		for (size_t b = 0; b < kmers.size(); ++b) put(kmers[b], static_cast<int64_t>(b));
+	}
+	// bucket of canonical k-mer c, kRepetitive, or -1 (absent)
+	int64_t lookup(uint64_t c) const
+	{
+		uint64_t h = mix(c) & hashMask;
+		while (hashVal[h] != -1)
+		{
+			if (hashKey[h] == c) return hashVal[h];
+			h = (h + 1) & hashMask;
+		}
+		return -1;
+	}
 };
 
 // Same window walk for every caller: calls f(pos, kmer, canonical, flip).
@@ -302,6 +346,7 @@ Index* buildIndex(const Store& st, int k, int solidMin, int repMax, float rate, 
 			}
 		});
 	}
+	ix->buildLookup();
 	return ix;
 }
 
@@ -318,14 +363,14 @@ void collect(const Index& ix, uint32_t qInternal, const std::string& q,
 {
 	const int k = ix.k;
 	forEachWindow(q, k, [&](int32_t pos, uint64_t x, uint64_t c, bool flip) {
-		if (std::binary_search(ix.repetitive.begin(), ix.repetitive.end(), c))
+		const int64_t hit = ix.lookup(c);
+		if (hit == Index::kRepetitive)
 		{
 			filtered.push_back(pos);
 			return;
 		}
-		auto it = std::lower_bound(ix.kmers.begin(), ix.kmers.end(), c);
-		if (it == ix.kmers.end() || *it != c) return;
-		size_t b = static_cast<size_t>(it - ix.kmers.begin());
+		if (hit < 0) return;
+		const size_t b = static_cast<size_t>(hit);
 		for (int64_t e = ix.start[b]; e < ix.start[b + 1]; ++e)
 		{
 			uint32_t id = ix.entryId[e];
@@ -846,6 +891,7 @@ int orc_load_index(void* h, int32_t k, const uint64_t* kmers, int64_t n_kmers, c
 	std::sort(ix->repetitive.begin(), ix->repetitive.end());
 	for (const auto& s : o->targets.fwd) ix->readLen.push_back(static_cast<int32_t>(s.size()));
 	ix->distinct = n_kmers + n_rep;
+	ix->buildLookup();
 	o->index = ix;
 	return 0;
 }
