@@ -168,30 +168,38 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 			const int64_t hb = gd.hitStart;
 			const int32_t* cur0 = a.cur + hb;
 			const int32_t* ext0 = a.ext + hb;
-			// unique curPos counted from prevPos = 0 on the (extId, curPos) order
-			int uniq = 0;
-			int mnE = 0x7fffffff, mxE = -0x7fffffff - 1;
-#pragma unroll 4
-			for (int i = lane; i < n; i += FH_WAVE)
+			bool pass;
+			if (a.passKnown)
 			{
-				const int32_t c = cur0[i];
-				const int32_t prev = i == 0 ? 0 : cur0[i - 1];
-				uniq += (c != prev);
-				const int32_t e = ext0[i];
-				mnE = min(mnE, e);
-				mxE = max(mxE, e);
+				pass = a.groupPass[g] != 0;  // chainRelevanceKernel
 			}
-			uniq = waveSum(uniq);
-			mnE = waveMin(mnE);
-			mxE = waveMax(mxE);
-			const int32_t mnC = cur0[0];
-			const int32_t mxC = cur0[n - 1];
-			bool pass = !(static_cast<float>(uniq) < minUniq);
-			if (pass && (mxC - mnC < p.minimum_overlap || mxE - mnE < p.minimum_overlap)) pass = false;
-			if (pass && p.check_overhang && !p.force_local)
+			else
 			{
-				if (min(mnC, mnE) > p.maximum_overhang) pass = false;
-				if (min(qd.curLen - mxC, gd.extLen - mxE) > p.maximum_overhang) pass = false;
+				// unique curPos counted from prevPos = 0 on the (extId, curPos) order
+				int uniq = 0;
+				int mnE = 0x7fffffff, mxE = -0x7fffffff - 1;
+#pragma unroll 4
+				for (int i = lane; i < n; i += FH_WAVE)
+				{
+					const int32_t c = cur0[i];
+					const int32_t prev = i == 0 ? 0 : cur0[i - 1];
+					uniq += (c != prev);
+					const int32_t e = ext0[i];
+					mnE = min(mnE, e);
+					mxE = max(mxE, e);
+				}
+				uniq = waveSum(uniq);
+				mnE = waveMin(mnE);
+				mxE = waveMax(mxE);
+				const int32_t mnC = cur0[0];
+				const int32_t mxC = cur0[n - 1];
+				pass = !(static_cast<float>(uniq) < minUniq);
+				if (pass && (mxC - mnC < p.minimum_overlap || mxE - mnE < p.minimum_overlap)) pass = false;
+				if (pass && p.check_overhang && !p.force_local)
+				{
+					if (min(mnC, mnE) > p.maximum_overhang) pass = false;
+					if (min(qd.curLen - mxC, gd.extLen - mxE) > p.maximum_overhang) pass = false;
+				}
 			}
 			const bool extSorted = pass && gd.extLen > qd.curLen;
 			if (extSorted)
@@ -212,6 +220,128 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 			}
 			segPushWave(sb, sb.big[0], lane == 0 && extSorted, static_cast<int32_t>(hb), static_cast<int32_t>(hb + n),
 						exsort::depthLimit(static_cast<uint32_t>(n)), 0);
+		}
+	}
+}
+
+// Before the match sort's tie replay, on the stable (extId, curPos) order:
+// the prefilters of every group (they do not depend on the order of tied
+// hits) into groupPass, and whether the order of the group's tied hits (equal
+// curPos) can reach a result into groupRel: in a passing group chained in
+// curPos order it can; in a passing group sorted by extPos only if two of its
+// hits share an extPos (otherwise the extPos order is the same whatever order
+// the sort starts from).  That test inserts extPos + 1 into a hash table of 2n
+// slots in the group's posL / posR scratch (workgroup-scope atomics, the
+// group's one wave being the workgroup).  Groups are taken 64 at a time in
+// index order.
+__device__ inline int32_t* hashSlot(int32_t* lo, int32_t* hi, int n, uint32_t s)
+{
+	return static_cast<int>(s) < n ? lo + s : hi + (static_cast<int>(s) - n);
+}
+
+__global__ void __launch_bounds__(FH_WAVE) chainRelevanceKernel(ChainArgs a, int32_t* groupRel)
+{
+	const int lane = laneId();
+	const flyehip_overlap_params_t& p = a.params;
+	const float minUniq = p.min_kmer_survival_rate * static_cast<float>(p.minimum_overlap);
+	for (;;)
+	{
+		int base = 0;
+		if (lane == 0) base = atomicAdd(a.queue, FH_WAVE);
+		base = __shfl(base, 0, FH_WAVE);
+		if (base >= a.nGroups) break;
+		const int myG = base + lane;
+		bool cand = false;
+		if (myG < a.nGroups)
+		{
+			const int n = a.groups[myG].n;
+			cand = n >= 2 && !(static_cast<float>(n) < minUniq);
+			if (!cand)
+			{
+				a.groupPass[myG] = 0;
+				groupRel[myG] = 0;
+			}
+		}
+		unsigned long long todo = __ballot(cand);
+		while (todo)
+		{
+			const int l = __ffsll(static_cast<long long>(todo)) - 1;
+			todo &= todo - 1;
+			const int g = base + l;
+			const GroupDesc gd = a.groups[g];
+			const QueryDesc qd = a.queries[gd.query];
+			const int n = gd.n;
+			const int64_t hb = gd.hitStart;
+			const int32_t* cur0 = a.cur + hb;
+			const int32_t* ext0 = a.ext + hb;
+			int uniq = 0;
+			bool tie = false;
+			int mnE = 0x7fffffff, mxE = -0x7fffffff - 1;
+#pragma unroll 4
+			for (int i = lane; i < n; i += FH_WAVE)
+			{
+				const int32_t c = cur0[i];
+				const int32_t prev = i == 0 ? 0 : cur0[i - 1];
+				uniq += (c != prev);
+				tie |= i > 0 && c == prev;
+				const int32_t e = ext0[i];
+				mnE = min(mnE, e);
+				mxE = max(mxE, e);
+			}
+			const bool anyTie = __ballot(tie) != 0;
+			uniq = waveSum(uniq);
+			mnE = waveMin(mnE);
+			mxE = waveMax(mxE);
+			const int32_t mnC = cur0[0];
+			const int32_t mxC = cur0[n - 1];
+			bool pass = !(static_cast<float>(uniq) < minUniq);
+			if (pass && (mxC - mnC < p.minimum_overlap || mxE - mnE < p.minimum_overlap)) pass = false;
+			if (pass && p.check_overhang && !p.force_local)
+			{
+				if (min(mnC, mnE) > p.maximum_overhang) pass = false;
+				if (min(qd.curLen - mxC, gd.extLen - mxE) > p.maximum_overhang) pass = false;
+			}
+			bool rel = pass && anyTie;
+			if (rel && gd.extLen > qd.curLen)
+			{
+				// two hits with the same extPos?
+				int32_t* lo = a.posL + hb;
+				int32_t* hi = a.posR + hb;
+				const uint32_t m = 2u * static_cast<uint32_t>(n);
+				for (int i = lane; i < n; i += FH_WAVE)
+				{
+					lo[i] = 0;
+					hi[i] = 0;
+				}
+				__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+				__builtin_amdgcn_wave_barrier();
+				bool dup = false;
+				for (int i = lane; i < n; i += FH_WAVE)
+				{
+					const int32_t key = ext0[i] + 1;
+					uint32_t sl = static_cast<uint32_t>((static_cast<uint64_t>(static_cast<uint32_t>(key) * 2654435761u) * m) >> 32);
+					for (;;)
+					{
+						int32_t expected = 0;
+						const bool put = __hip_atomic_compare_exchange_strong(
+							hashSlot(lo, hi, n, sl), &expected, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+							__HIP_MEMORY_SCOPE_WORKGROUP);
+						if (put) break;
+						if (expected == key)
+						{
+							dup = true;
+							break;
+						}
+						sl = sl + 1 == m ? 0 : sl + 1;
+					}
+				}
+				rel = __ballot(dup) != 0;
+			}
+			if (lane == 0)
+			{
+				a.groupPass[g] = pass ? 1 : 0;
+				groupRel[g] = rel ? 1 : 0;
+			}
 		}
 	}
 }
@@ -1281,6 +1411,14 @@ hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* b
 	hipLaunchKernelGGL(binCountKernel, grid, dim3(tpb), 0, stream, groups, nGroups, bins);
 	hipLaunchKernelGGL(binPrefixKernel, dim3(1), dim3(64), 0, stream, bins, bins + 40);
 	hipLaunchKernelGGL(binScatterKernel, grid, dim3(tpb), 0, stream, groups, nGroups, bins + 40, order);
+	return hipGetLastError();
+}
+
+hipError_t launchChainRelevance(const ChainArgs& a, int32_t* groupRel, int blocks, hipStream_t stream)
+{
+	if (a.nGroups <= 0) return hipSuccess;
+	const unsigned grid = static_cast<unsigned>(min(blocks, (a.nGroups + FH_WAVE - 1) / FH_WAVE));
+	hipLaunchKernelGGL(chainRelevanceKernel, dim3(grid), dim3(FH_WAVE), 0, stream, a, groupRel);
 	return hipGetLastError();
 }
 

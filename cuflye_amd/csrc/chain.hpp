@@ -76,6 +76,7 @@ struct ChainArgs
 	const uint32_t* logX;
 	const uint32_t* logY;
 	unsigned long long* dpCells;  // adds the (row, predecessor) pairs the DP evaluates (may be null)
+	int32_t passKnown;            // groupPass already holds the prefilters (chainRelevanceKernel)
 };
 
 // Walk kernel variants by group size (powers of two, so each is a contiguous
@@ -94,7 +95,13 @@ hipError_t launchLogf(const float* x, int64_t n, float* out, const uint32_t* tab
 					  hipStream_t stream);
 hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* bins, int32_t* order,
 							hipStream_t stream);
-// prefilters; pushes the extPos sorts to sb
+// Before the match sort's tie replay, on the stable order: every group's
+// prefilters into groupPass, and groupRel[g] = 1 when the order of the
+// group's tied (equal curPos) hits can reach a result (a passing group in
+// curPos order, or one sorted by extPos with two hits sharing an extPos).
+// Uses posL / posR as scratch.
+hipError_t launchChainRelevance(const ChainArgs& a, int32_t* groupRel, int blocks, hipStream_t stream);
+// prefilters (or groupPass with passKnown); pushes the extPos sorts to sb
 hipError_t launchChainPrefilter(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream);
 // passOrder = order filtered by groupPass (flags, scan by the caller, scatter);
 // nBig[0] counts passing groups of at least kWalkSmall hits, nBig[1] their hits

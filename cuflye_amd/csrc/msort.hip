@@ -31,7 +31,8 @@ namespace
 {
 
 __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, const uint32_t* sval,
-							   uint32_t* rkey, int32_t* dupFlag, int64_t n)
+							   uint32_t* rkey, int32_t* dupFlag, int64_t n, const int64_t* groupOff,
+							   const int32_t* groupFlag, const int32_t* groupRel)
 {
 	// rank keys are scattered inside the query's range: one XCD per range
 	const int64_t i = static_cast<int64_t>(xcdBlockIndex()) * blockDim.x + threadIdx.x;
@@ -42,7 +43,11 @@ __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* 
 	int64_t first = i;
 	while (first > b && skey[first - 1] == k) --first;
 	rkey[b + sval[i]] = static_cast<uint32_t>(first - b);
-	dupFlag[i] = (first < i || (i + 1 < e && skey[i + 1] == k)) ? 1 : 0;
+	bool tied = first < i || (i + 1 < e && skey[i + 1] == k);
+	// only ties whose order can reach a result (groupRel of the position's
+	// group: groups start where groupFlag is set, groupOff counts the starts)
+	if (tied && groupRel) tied = groupRel[groupOff[i] + groupFlag[i] - 1] != 0;
+	dupFlag[i] = tied ? 1 : 0;
 }
 
 inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n + tpb - 1) / tpb)); }
@@ -251,7 +256,7 @@ hipError_t launchRankKeys(const MsortArgs& a, int64_t nMatches, hipStream_t s)
 	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
 	const int tpb = 256;
 	hipLaunchKernelGGL(rankKeysKernel, gridFor(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.sval,
-					   a.rkey, a.dupFlag, nMatches);
+					   a.rkey, a.dupFlag, nMatches, a.groupOff, a.groupFlag, a.groupRel);
 	return hipGetLastError();
 }
 

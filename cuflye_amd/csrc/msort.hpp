@@ -18,6 +18,12 @@ struct MsortArgs
 	uint32_t* rkey;        // emission-order rank keys (filled here)
 	int32_t* dupFlag;      // per sorted position: a neighbouring position holds the same key
 	const int32_t* order;  // counting sort: queries by descending match count (null: in order)
+	// optional (all null: every tie counts): per sorted position, the group
+	// start flags and their exclusive scan, and per group whether the order
+	// of its tied hits can reach a result; only those ties are flagged
+	const int64_t* groupOff = nullptr;
+	const int32_t* groupFlag = nullptr;
+	const int32_t* groupRel = nullptr;
 };
 
 hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t* bytes);
@@ -27,7 +33,8 @@ hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t
 bool msortCounting(int nBins);
 hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBits, int nBins, void* temp,
 								 size_t tempBytes, hipStream_t s);
-// Rank keys and tied-position flags of the stable order.
+// Rank keys and tied-position flags of the stable order (with groupRel set,
+// only the ties of the groups marked relevant).
 hipError_t launchRankKeys(const MsortArgs& a, int64_t nMatches, hipStream_t s);
 
 } // namespace fh

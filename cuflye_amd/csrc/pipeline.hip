@@ -682,6 +682,18 @@ __global__ void gatherHitsKernel(const int64_t* qMatchOff, int32_t nQueries, con
 	groupFlag[i] = (i == qb || (key[i - 1] >> 32) != (k >> 32)) ? 1 : 0;
 }
 
+// After the tie replay: the hits at flagged (tied, relevant) sorted positions
+// take the extPos of the match the replay put there (curPos is the key's, the
+// same for every member of a tie).
+__global__ void fixTiedHitsKernel(const int64_t* qMatchOff, int32_t nQueries, const int32_t* dupFlag,
+								  const uint32_t* val, const int32_t* mext, int32_t* hext, int64_t nMatches)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= nMatches || !dupFlag[i]) return;
+	const int q = findSegWave(qMatchOff, nQueries, i);
+	hext[i] = mext[qMatchOff[q] + val[i]];
+}
+
 __global__ void buildGroupsKernel(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const int32_t* groupFlag,
 								  const int64_t* groupOff, const int32_t* targetLen, GroupDesc* groups, int64_t nMatches)
 {
@@ -899,6 +911,15 @@ hipError_t launchGatherHits(const int64_t* qMatchOff, int32_t nQueries, const ui
 	if (nMatches == 0) return hipSuccess;
 	hipLaunchKernelGGL(gatherHitsKernel, gridFor(nMatches, 256), dim3(256), 0, s, qMatchOff, nQueries, key, val, mext,
 					   hcur, hext, groupFlag, nMatches);
+	return hipGetLastError();
+}
+
+hipError_t launchFixTiedHits(const int64_t* qMatchOff, int32_t nQueries, const int32_t* dupFlag, const uint32_t* val,
+							 const int32_t* mext, int32_t* hext, int64_t nMatches, hipStream_t s)
+{
+	if (nMatches == 0) return hipSuccess;
+	hipLaunchKernelGGL(fixTiedHitsKernel, gridFor(nMatches, 256), dim3(256), 0, s, qMatchOff, nQueries, dupFlag, val,
+					   mext, hext, nMatches);
 	return hipGetLastError();
 }
 

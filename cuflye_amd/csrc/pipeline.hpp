@@ -127,6 +127,9 @@ hipError_t launchWriteCandidates(const JoinArgs& a, const int64_t* qMatchOff, co
 hipError_t launchGatherHits(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const uint32_t* val,
 							const int32_t* mext, int32_t* hcur, int32_t* hext, int32_t* groupFlag,
 							int64_t nMatches, hipStream_t s);
+// hext at the sorted positions flagged in dupFlag from the permutation val.
+hipError_t launchFixTiedHits(const int64_t* qMatchOff, int32_t nQueries, const int32_t* dupFlag, const uint32_t* val,
+							 const int32_t* mext, int32_t* hext, int64_t nMatches, hipStream_t s);
 hipError_t launchBuildGroups(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key,
 							 const int32_t* groupFlag, const int64_t* groupOff, const int32_t* targetLen,
 							 GroupDesc* groups, int64_t nMatches, hipStream_t s);

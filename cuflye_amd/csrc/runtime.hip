@@ -177,7 +177,7 @@ struct Lane
 	DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
 	DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
 	DevBuf bGroupDefer, bFlagReady, bFlagDefer, bPosReady, bPosDefer, bReadyOrder, bDeferOrder;
-	DevBuf bClaimFlag, bFlagOrder;
+	DevBuf bClaimFlag, bFlagOrder, bGroupRel;
 	DevBuf bGroupPass, bOrder, bPassOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
 	DevBuf bSegBig0, bSegBig1, bSegLeaf, bSegHeap, bSegCounters, bSegChunkOff, bSegChunkRange, bSegChunkCnt;
 	DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp, bSegVerify, bSegShadowK, bSegShadowV;
@@ -710,26 +710,76 @@ void runSegSort(Lane* L, SegSortBufs& b, int64_t* tieArrays, bool* exactPending 
 	HIPCHECK(e);
 }
 
-// Exact std::sort of every query's matches: stable order, then the tie replay.
-// Leaves the sorted keys in bSKey and the permutation (emission index within
-// the query) in bVal.
-void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
+// The match sort's tie replay after the stable order (launchMatchSortStable):
+// rank keys and tied-position flags, then the pruned exact introsort of every
+// query holding a flagged position, values into bVal.  Without a relevance
+// mask (ms.groupRel null) every tie counts and bVal becomes Flye's order.
+// With it only the ties of the groups marked relevant are flagged: ranges
+// without a flagged position are dropped, so bVal holds Flye's order at the
+// flagged positions (their groups lie wholly in replayed ranges) and at every
+// position of a distinct key; positions of other ties may hold any member
+// (even one twice, where a tie straddles a dropped and a replayed range), so
+// the caller takes those from the stable order (their order cannot reach a
+// result).
+void runTieReplay(Lane* L, const JoinArgs& ja, int64_t nMatches, MsortArgs& ms)
 {
 	flyehip_ctx* c = L->c;
-	if (nMatches >= 0x7fffffffLL) fail(FLYEHIP_ERR_BUDGET, "batch of %lld matches exceeds 2^31 - 1", static_cast<long long>(nMatches));
 	const size_t nm = static_cast<size_t>(nMatches) + 1;
 	const int32_t nq = ja.nQueries;
+	int64_t* dup = ensureT<int64_t>(c, L->bDup, nm);
+	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nMatches) + 64);
+	SegSortBufs sb = segBufs(L, nMatches, nq);
+	sb.key = ms.rkey;
+	sb.val = ja.mval;
+	sb.outVal = ms.sval;
+	sb.dup = dup;
+	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
+	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
+	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
+	// rank keys: sorted key and value read, rank key and tie flag written (20
+	// bytes per match)
+	const int tk = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 20.0 * static_cast<double>(nMatches));
+	HIPCHECK(launchRankKeys(ms, nMatches, L->stream));
+	L->timer.end(tk, L->stream);
+	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, L->stream));
+	HIPCHECK(segSortReset(sb, L->stream));
+	HIPCHECK(segPushOffsets(sb, ms.qOff, nq, L->stream));
+	int64_t ties = 0;
+	runSegSort(L, sb, &ties);
+	HIPCHECK(hipEventRecord(L->ev[1], L->stream));
+	HIPCHECK(hipEventSynchronize(L->ev[1]));
+	float es = 0;
+	HIPCHECK(hipEventElapsedTime(&es, L->ev[0], L->ev[1]));
+	L->stats.ms_sort += es;
+	L->stats.tie_queries += ties;
+}
+
+// The match sort's arguments for the batch's buffers.
+MsortArgs matchSortArgs(Lane* L, const JoinArgs& ja, int64_t nMatches)
+{
+	flyehip_ctx* c = L->c;
+	const size_t nm = static_cast<size_t>(nMatches) + 1;
 	MsortArgs ms{};
 	ms.qOff = static_cast<const int64_t*>(L->bQMatchOff.p);
-	ms.nQueries = nq;
+	ms.nQueries = ja.nQueries;
 	ms.akey = ja.mkey;
 	ms.aval = ja.mval;
 	ms.skey = ensureT<uint64_t>(c, L->bSKey, nm);
 	ms.sval = ensureT<uint32_t>(c, L->bVal, nm);
 	ms.rkey = ensureT<uint32_t>(c, L->bRKey, nm);
 	ms.dupFlag = ensureT<int32_t>(c, L->bDupFlag, nm);
-	int64_t* dup = ensureT<int64_t>(c, L->bDup, nm);
-	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nMatches) + 64);
+	return ms;
+}
+
+// Stable (extId, curPos) order of every query's matches (bSKey, bVal); with
+// replay the tie replay follows, so bVal becomes std::sort's permutation
+// (emission index within the query).
+void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches, bool replay = true)
+{
+	flyehip_ctx* c = L->c;
+	if (nMatches >= 0x7fffffffLL) fail(FLYEHIP_ERR_BUDGET, "batch of %lld matches exceeds 2^31 - 1", static_cast<long long>(nMatches));
+	const int32_t nq = ja.nQueries;
+	MsortArgs ms = matchSortArgs(L, ja, nMatches);
 	int bits = 1;
 	while ((int64_t{1} << bits) < 2 * static_cast<int64_t>(std::max(1, c->nReads))) ++bits;
 	const int nBins = 2 * std::max(1, c->nReads);  // internal target ids
@@ -752,35 +802,19 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches)
 	size_t tempBytes = 0;
 	if (!counting) HIPCHECK(msortTempBytes(nMatches, nq, bits, &tempBytes));
 	void* temp = counting ? nullptr : ensure(c, L->bMsortTemp, tempBytes + 256);
-	SegSortBufs sb = segBufs(L, nMatches, nq);
-	sb.key = ms.rkey;
-	sb.val = ja.mval;
-	sb.outVal = ms.sval;
-	sb.dup = dup;
-	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
-	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
 	// stable order: the counting sort reads the keys twice (histogram, then
 	// with the values) and writes keys and values once (32 bytes per match);
-	// the radix fallback makes 2 passes x 24 bytes.  Rank keys: sorted key and
-	// value read, rank key and tie flag written (20 bytes per match)
+	// the radix fallback makes 2 passes x 24 bytes
 	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, (counting ? 32.0 : 48.0) * static_cast<double>(nMatches));
 	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, nBins, temp, tempBytes, L->stream));
 	L->timer.end(tr, L->stream);
-	const int tk = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 20.0 * static_cast<double>(nMatches));
-	HIPCHECK(launchRankKeys(ms, nMatches, L->stream));
-	L->timer.end(tk, L->stream);
-	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, L->stream));
-	HIPCHECK(segSortReset(sb, L->stream));
-	HIPCHECK(segPushOffsets(sb, ms.qOff, nq, L->stream));
-	int64_t ties = 0;
-	runSegSort(L, sb, &ties);
 	HIPCHECK(hipEventRecord(L->ev[1], L->stream));
 	HIPCHECK(hipEventSynchronize(L->ev[1]));
 	float es = 0;
 	HIPCHECK(hipEventElapsedTime(&es, L->ev[0], L->ev[1]));
 	L->stats.ms_sort += es;
-	L->stats.tie_queries += ties;
+	if (replay) runTieReplay(L, ja, nMatches, ms);
 }
 
 // Order, prefilters + extPos sort, DP, score sort + walks over ca.nGroups
@@ -975,8 +1009,14 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 
 // Groups + chaining for a batch whose sorted matches sit in bMKey/bVal.
 // Returns rows on the device in bRows and per-query row counts in bQRowCount.
+// With replayTies the matches come in the stable order (runMatchSort without
+// replay): after the groups are built, chainRelevanceKernel marks the groups
+// whose tied hits' order can reach a result, the tie replay runs for those
+// ties alone, and the hits at their positions take the replayed order; every
+// other tie keeps the stable order, which no result can tell from Flye's.
 BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vector<int64_t>& qIds,
-				  const std::vector<int32_t>& qLens, const flyehip_overlap_params_t& p, double* msGroup, double* msChain)
+				  const std::vector<int32_t>& qLens, const flyehip_overlap_params_t& p, double* msGroup, double* msChain,
+				  bool replayTies = false)
 {
 	flyehip_ctx* c = L->c;
 	BatchOut bo;
@@ -1011,6 +1051,31 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 	hipLaunchKernelGGL(makeQueryDescKernel, grid(nq, 256), dim3(256), 0, L->stream, dIds, dLens,
 					   static_cast<const int64_t*>(L->bQFiltOff.p), nq, qd);
 	HIPCHECK(hipGetLastError());
+	if (replayTies)
+	{
+		const size_t nmr = static_cast<size_t>(nMatches) + 1;
+		ChainArgs ra{};
+		ra.params = p;
+		ra.groups = groups;
+		ra.queries = qd;
+		ra.nGroups = static_cast<int32_t>(nGroups);
+		ra.cur = hcur;
+		ra.ext = hext;
+		ra.posL = ensureT<int32_t>(c, L->bPosL, nmr);
+		ra.posR = ensureT<int32_t>(c, L->bPosR, nmr);
+		ra.groupPass = ensureT<int32_t>(c, L->bGroupPass, static_cast<size_t>(nGroups) + 1);
+		int32_t* queues = ensureT<int32_t>(c, L->bQueue, 16);
+		zero32(L, queues, 16);
+		ra.queue = queues + 15;
+		int32_t* groupRel = ensureT<int32_t>(c, L->bGroupRel, static_cast<size_t>(nGroups) + 1);
+		HIPCHECK(launchChainRelevance(ra, groupRel, chainBlocks(c), L->stream));
+		MsortArgs ms = matchSortArgs(L, ja, nMatches);
+		ms.groupOff = goff;
+		ms.groupFlag = gflag;
+		ms.groupRel = groupRel;
+		runTieReplay(L, ja, nMatches, ms);
+		HIPCHECK(launchFixTiedHits(qMatchOff, nq, ms.dupFlag, ms.sval, ja.mext, hext, nMatches, L->stream));
+	}
 	sync(L);
 	(void)msGroup;
 	(void)tg;
@@ -1048,6 +1113,7 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 	ca.groupPass = ensureT<int32_t>(c, L->bGroupPass, static_cast<size_t>(nGroups) + 1);
 	ca.logX = static_cast<const uint32_t*>(c->dLogX.p);
 	ca.logY = static_cast<const uint32_t*>(c->dLogY.p);
+	ca.passKnown = replayTies ? 1 : 0;
 	runChainKernels(L, ca, nMatches);
 	int64_t* rowStart = ensureT<int64_t>(c, L->bRowStart, static_cast<size_t>(nGroups) + 1);
 	HIPCHECK(exclusiveScan(ca.groupCount, rowStart, nGroups, tmp, L->stream));
@@ -1139,7 +1205,7 @@ void releaseBatchBuffers(flyehip_ctx* c, Lane& l)
 					  &l.bSegShadowV, &l.bQueue, &l.bCounters, &l.bQRowCount, &l.bCurLen, &l.bDivOff, &l.bDivCount,
 					  &l.bDivOut, &l.bQueryIds, &l.bCand, &l.bExtWords, &l.bExtWordOff, &l.bExtLen, &l.bAln,
 					  &l.bAlnOff, &l.bRA, &l.bGroupDefer, &l.bFlagReady, &l.bFlagDefer, &l.bPosReady, &l.bPosDefer,
-					  &l.bReadyOrder, &l.bDeferOrder, &l.bClaimFlag, &l.bFlagOrder, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart, &l.bMinMask, &l.bQOrder})
+					  &l.bReadyOrder, &l.bDeferOrder, &l.bClaimFlag, &l.bFlagOrder, &l.bGroupRel, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart, &l.bMinMask, &l.bQOrder})
 		release(c, *b);
 }
 
@@ -1191,7 +1257,8 @@ struct BatchResult
 	std::vector<float> divs;
 };
 
-// One batch allQ[b0, b1) on a lane: join, exact match sort, chaining; rows
+// One batch allQ[b0, b1) on a lane: join, match sort (stable order, then the
+// tie replay for the ties whose order can reach a result), chaining; rows
 // (and divStats windows, computed on the device) copied to the host.
 void runBatch(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& allQ, const int64_t* queryIds,
 			  const flyehip_overlap_params_t& p, int32_t b0, int32_t b1, bool withDiv, BatchResult& out)
@@ -1204,8 +1271,8 @@ void runBatch(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& all
 	JoinArgs ja;
 	BatchOut jo = runJoin(L, seqs, qs, ja);
 	sync(L);
-	runMatchSort(L, ja, jo.matches);
-	BatchOut co = runChain(L, ja, jo.matches, ids, lens, p, nullptr, nullptr);
+	runMatchSort(L, ja, jo.matches, false);
+	BatchOut co = runChain(L, ja, jo.matches, ids, lens, p, nullptr, nullptr, true);
 	const int32_t nq = b1 - b0;
 	out.count.assign(static_cast<size_t>(nq), 0);
 	out.rows.resize(static_cast<size_t>(co.rows));
