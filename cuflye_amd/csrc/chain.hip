@@ -1141,8 +1141,13 @@ __global__ void __launch_bounds__(FH_WAVE) chainClaimKernel(ChainArgs a, int32_t
 		const int32_t* sc = a.score + hb;
 		const int32_t* bt = a.bt + hb;
 		uint32_t* acc = reinterpret_cast<uint32_t*>(a.posL + hb);
-		for (int i = lane; i < n; i += FH_WAVE) acc[i] = 0u;
-		__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+		// a group of one chunk has no row below it: no accumulators
+		const bool oneChunk = n <= FH_WAVE;
+		if (!oneChunk)
+		{
+			for (int i = lane; i < n; i += FH_WAVE) acc[i] = 0u;
+			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+		}
 		int nAlive = 0;
 		bool flagged = false;
 		for (int t = (n - 1) & ~(FH_WAVE - 1); t >= 0 && !flagged; t -= FH_WAVE)
@@ -1155,7 +1160,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainClaimKernel(ChainArgs a, int32_t
 			{
 				b = bt[v];
 				s = sc[v];
-				a0 = accLoad(acc + v);
+				if (!oneChunk) a0 = accLoad(acc + v);
 			}
 			const bool rel = b != -1;
 			const int j0 = (rel && b >= t) ? b - t : -1;
@@ -1224,7 +1229,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainClaimKernel(ChainArgs a, int32_t
 				m &= ~mc;
 				waveSync();
 			}
-			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+			if (!oneChunk) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
 		}
 		// two surviving starts of equal score
 		if (!flagged && nAlive > 1)

@@ -30,24 +30,38 @@ namespace fh
 namespace
 {
 
+// Tied-position flags of the stable order: a neighbouring position holds the
+// same key (with groupRel set, only in the groups marked relevant: groups
+// start where groupFlag is set, groupOff counts the starts).
+__global__ void tieFlagsKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, int32_t* dupFlag, int64_t n,
+							   const int64_t* groupOff, const int32_t* groupFlag, const int32_t* groupRel)
+{
+	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int q = findSegWave(qOff, nq, i);
+	const int64_t b = qOff[q], e = qOff[q + 1];
+	const uint64_t k = skey[i];
+	bool tied = (i > b && skey[i - 1] == k) || (i + 1 < e && skey[i + 1] == k);
+	if (tied && groupRel) tied = groupRel[groupOff[i] + groupFlag[i] - 1] != 0;
+	dupFlag[i] = tied ? 1 : 0;
+}
+
+// Rank keys (first sorted position of the key in its query) in emission
+// order, for the queries that hold a flagged position (dup: exclusive prefix
+// of the flags); the replay reads no other query's.
 __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, const uint32_t* sval,
-							   uint32_t* rkey, int32_t* dupFlag, int64_t n, const int64_t* groupOff,
-							   const int32_t* groupFlag, const int32_t* groupRel)
+							   const int64_t* dup, uint32_t* rkey, int64_t n)
 {
 	// rank keys are scattered inside the query's range: one XCD per range
 	const int64_t i = static_cast<int64_t>(xcdBlockIndex()) * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const int q = findSegWave(qOff, nq, i);
 	const int64_t b = qOff[q], e = qOff[q + 1];
+	if (dup[e] == dup[b]) return;
 	const uint64_t k = skey[i];
 	int64_t first = i;
 	while (first > b && skey[first - 1] == k) --first;
 	rkey[b + sval[i]] = static_cast<uint32_t>(first - b);
-	bool tied = first < i || (i + 1 < e && skey[i + 1] == k);
-	// only ties whose order can reach a result (groupRel of the position's
-	// group: groups start where groupFlag is set, groupOff counts the starts)
-	if (tied && groupRel) tied = groupRel[groupOff[i] + groupFlag[i] - 1] != 0;
-	dupFlag[i] = tied ? 1 : 0;
 }
 
 inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n + tpb - 1) / tpb)); }
@@ -251,12 +265,21 @@ hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBit
 													   32 + idBits, s);
 }
 
-hipError_t launchRankKeys(const MsortArgs& a, int64_t nMatches, hipStream_t s)
+hipError_t launchTieFlags(const MsortArgs& a, int64_t nMatches, hipStream_t s)
+{
+	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
+	const int tpb = 256;
+	hipLaunchKernelGGL(tieFlagsKernel, gridFor(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.dupFlag,
+					   nMatches, a.groupOff, a.groupFlag, a.groupRel);
+	return hipGetLastError();
+}
+
+hipError_t launchRankKeys(const MsortArgs& a, const int64_t* dup, int64_t nMatches, hipStream_t s)
 {
 	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
 	const int tpb = 256;
 	hipLaunchKernelGGL(rankKeysKernel, gridFor(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.sval,
-					   a.rkey, a.dupFlag, nMatches, a.groupOff, a.groupFlag, a.groupRel);
+					   dup, a.rkey, nMatches);
 	return hipGetLastError();
 }
 

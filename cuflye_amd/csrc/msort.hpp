@@ -33,8 +33,11 @@ hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t
 bool msortCounting(int nBins);
 hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBits, int nBins, void* temp,
 								 size_t tempBytes, hipStream_t s);
-// Rank keys and tied-position flags of the stable order (with groupRel set,
-// only the ties of the groups marked relevant).
-hipError_t launchRankKeys(const MsortArgs& a, int64_t nMatches, hipStream_t s);
+// Tied-position flags of the stable order (with groupRel set, only the ties
+// of the groups marked relevant).
+hipError_t launchTieFlags(const MsortArgs& a, int64_t nMatches, hipStream_t s);
+// Rank keys (emission order) of the queries that hold a flagged position;
+// dup is the exclusive prefix of the flags (nMatches + 1 words).
+hipError_t launchRankKeys(const MsortArgs& a, const int64_t* dup, int64_t nMatches, hipStream_t s);
 
 } // namespace fh

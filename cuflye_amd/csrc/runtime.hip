@@ -736,12 +736,16 @@ void runTieReplay(Lane* L, const JoinArgs& ja, int64_t nMatches, MsortArgs& ms)
 	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
 	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
-	// rank keys: sorted key and value read, rank key and tie flag written (20
-	// bytes per match)
-	const int tk = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 20.0 * static_cast<double>(nMatches));
-	HIPCHECK(launchRankKeys(ms, nMatches, L->stream));
+	// tie flags: sorted key read, flag written (12 bytes per match); rank
+	// keys of the queries with a flagged position: sorted key and value read,
+	// rank key written (16 bytes per match of those queries, counted over all)
+	const int tk = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 12.0 * static_cast<double>(nMatches));
+	HIPCHECK(launchTieFlags(ms, nMatches, L->stream));
 	L->timer.end(tk, L->stream);
 	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, L->stream));
+	const int tr = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 0.0);
+	HIPCHECK(launchRankKeys(ms, dup, nMatches, L->stream));
+	L->timer.end(tr, L->stream);
 	HIPCHECK(segSortReset(sb, L->stream));
 	HIPCHECK(segPushOffsets(sb, ms.qOff, nq, L->stream));
 	int64_t ties = 0;
@@ -1068,7 +1072,10 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 		zero32(L, queues, 16);
 		ra.queue = queues + 15;
 		int32_t* groupRel = ensureT<int32_t>(c, L->bGroupRel, static_cast<size_t>(nGroups) + 1);
+		// the prefilters (hits read, 8 bytes each) and the extPos duplicate test
+		const int tp = L->timer.begin(FLYEHIP_K_PREFILTER, L->stream, 8.0 * static_cast<double>(nMatches));
 		HIPCHECK(launchChainRelevance(ra, groupRel, chainBlocks(c), L->stream));
+		L->timer.end(tp, L->stream);
 		MsortArgs ms = matchSortArgs(L, ja, nMatches);
 		ms.groupOff = goff;
 		ms.groupFlag = gflag;

@@ -89,10 +89,10 @@ KERNEL_NAMES = {
     "join_count": "joinCountKernel",
     "join_fill": "joinFillKernel",
     "radix": "matchCountSortKernel (hipcub DeviceSegmentedRadixSort above its target-id limit)",
-    "rank_keys": "rankKeysKernel",
+    "rank_keys": "tieFlagsKernel + rankKeysKernel",
     "leaf": "segLeafKernel",
     "heap": "segHeapLdsKernel (+ segHeapGlobalKernel)",
-    "prefilter": "chainPrefilterKernel",
+    "prefilter": "chainRelevanceKernel (with the tie replay) + chainPrefilterKernel",
     "dp": "chainDpKernel<G>",
     "walk": "chainWalkKernel<size class, onlyMaxExt>",
     "round_count": "segCountKernel",
@@ -104,8 +104,8 @@ KERNEL_NAMES = {
     "claim": "chainClaimKernel<onlyMaxExt>",
 }
 # Kernels that are one HIP kernel (a roofline names one of these).
-SINGLE_KERNELS = ("join_count", "join_fill", "rank_keys", "leaf", "heap", "prefilter", "dp", "walk", "round_count",
-                  "round_rank", "round_swap", "gather", "claim")
+SINGLE_KERNELS = ("join_count", "join_fill", "leaf", "heap", "dp", "walk", "round_count", "round_rank", "round_swap",
+                  "gather", "claim")
 
 
 CANDIDATE_DTYPE = np.dtype([("query_id", "<i8"), ("target_id", "<i8"), ("kmer", "<u8"),

@@ -169,10 +169,11 @@ typedef struct {
 #define FLYEHIP_K_RADIX 2        /* stable (extId, curPos) order of the matches:
                                     matchCountSortKernel (the library radix sort
                                     above its target-id limit)                      */
-#define FLYEHIP_K_RANK_KEYS 3    /* rankKeysKernel: tie flags and rank keys         */
+#define FLYEHIP_K_RANK_KEYS 3    /* tieFlagsKernel + rankKeysKernel                 */
 #define FLYEHIP_K_LEAF 4         /* segLeafKernel: exact leaf sorts in LDS          */
 #define FLYEHIP_K_HEAP 5         /* segHeapLdsKernel / segHeapGlobalKernel          */
-#define FLYEHIP_K_PREFILTER 6    /* chainPrefilterKernel                            */
+#define FLYEHIP_K_PREFILTER 6    /* chainPrefilterKernel (+ chainRelevanceKernel
+                                    where the tie replay runs)                      */
 #define FLYEHIP_K_DP 7           /* chainDpKernel<G>                                */
 #define FLYEHIP_K_WALK 8         /* chainWalkKernel<size class, onlyMaxExt>         */
 #define FLYEHIP_K_ROUND_COUNT 9  /* segCountKernel: partition flags per chunk       */
