@@ -232,14 +232,16 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 // hits share an extPos (otherwise the extPos order is the same whatever order
 // the sort starts from).  That test inserts extPos + 1 into a hash table of 2n
 // slots in the group's posL / posR scratch (workgroup-scope atomics, the
-// group's one wave being the workgroup).  Groups are taken 64 at a time in
-// index order.
+// group's one wave being the workgroup).  In the groups it marks, the tied
+// hits get tieFlag 1 (a group is one extId, so equal keys are equal curPos
+// inside it; the caller clears tieFlag first).  Groups are taken 64 at a time
+// in index order.
 __device__ inline int32_t* hashSlot(int32_t* lo, int32_t* hi, int n, uint32_t s)
 {
 	return static_cast<int>(s) < n ? lo + s : hi + (static_cast<int>(s) - n);
 }
 
-__global__ void __launch_bounds__(FH_WAVE) chainRelevanceKernel(ChainArgs a, int32_t* groupRel)
+__global__ void __launch_bounds__(FH_WAVE) chainRelevanceKernel(ChainArgs a, int32_t* groupRel, int32_t* tieFlag)
 {
 	const int lane = laneId();
 	const flyehip_overlap_params_t& p = a.params;
@@ -336,6 +338,14 @@ __global__ void __launch_bounds__(FH_WAVE) chainRelevanceKernel(ChainArgs a, int
 					}
 				}
 				rel = __ballot(dup) != 0;
+			}
+			if (rel)
+			{
+				for (int i = lane; i < n; i += FH_WAVE)
+				{
+					const int32_t c = cur0[i];
+					if ((i > 0 && cur0[i - 1] == c) || (i + 1 < n && cur0[i + 1] == c)) tieFlag[hb + i] = 1;
+				}
 			}
 			if (lane == 0)
 			{
@@ -1419,11 +1429,12 @@ hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* b
 	return hipGetLastError();
 }
 
-hipError_t launchChainRelevance(const ChainArgs& a, int32_t* groupRel, int blocks, hipStream_t stream)
+hipError_t launchChainRelevance(const ChainArgs& a, int32_t* groupRel, int32_t* tieFlag, int blocks,
+								hipStream_t stream)
 {
 	if (a.nGroups <= 0) return hipSuccess;
 	const unsigned grid = static_cast<unsigned>(min(blocks, (a.nGroups + FH_WAVE - 1) / FH_WAVE));
-	hipLaunchKernelGGL(chainRelevanceKernel, dim3(grid), dim3(FH_WAVE), 0, stream, a, groupRel);
+	hipLaunchKernelGGL(chainRelevanceKernel, dim3(grid), dim3(FH_WAVE), 0, stream, a, groupRel, tieFlag);
 	return hipGetLastError();
 }
 

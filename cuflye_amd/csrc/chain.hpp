@@ -98,9 +98,11 @@ hipError_t launchGroupOrder(const GroupDesc* groups, int32_t nGroups, int32_t* b
 // Before the match sort's tie replay, on the stable order: every group's
 // prefilters into groupPass, and groupRel[g] = 1 when the order of the
 // group's tied (equal curPos) hits can reach a result (a passing group in
-// curPos order, or one sorted by extPos with two hits sharing an extPos).
+// curPos order, or one sorted by extPos with two hits sharing an extPos);
+// tieFlag (cleared by the caller) gets 1 at the tied hits of those groups.
 // Uses posL / posR as scratch.
-hipError_t launchChainRelevance(const ChainArgs& a, int32_t* groupRel, int blocks, hipStream_t stream);
+hipError_t launchChainRelevance(const ChainArgs& a, int32_t* groupRel, int32_t* tieFlag, int blocks,
+								hipStream_t stream);
 // prefilters (or groupPass with passKnown); pushes the extPos sorts to sb
 hipError_t launchChainPrefilter(const ChainArgs& a, const SegSortBufs& sb, int blocks, hipStream_t stream);
 // passOrder = order filtered by groupPass (flags, scan by the caller, scatter);

@@ -77,6 +77,16 @@ __device__ inline int findSegWave(const int64_t* off, int n, int64_t w)
 	return r;
 }
 
+// Segment of w, given the segment r of a position at or before w: steps
+// forward over the segment starts in between (a binary search over the rest
+// after four).
+__device__ inline int stepSeg(const int64_t* off, int n, int r, int64_t w)
+{
+	for (int s = 0; s < 4 && r + 1 < n && off[r + 1] <= w; ++s) ++r;
+	if (r + 1 < n && off[r + 1] <= w) r = findSegFrom(off, r + 1, n, w);
+	return r;
+}
+
 __device__ inline int waveSum(int v)
 {
 	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, FH_WAVE);

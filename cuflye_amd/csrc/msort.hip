@@ -30,20 +30,28 @@ namespace fh
 namespace
 {
 
+// Per-position kernels over a batch's sorted matches: a block takes
+// kPosPerThread x 256 consecutive positions, 256 at a time, so the query of
+// a position is searched once per wave and then stepped forward.
+constexpr int kPosPerThread = 8;
+
 // Tied-position flags of the stable order: a neighbouring position holds the
-// same key (with groupRel set, only in the groups marked relevant: groups
-// start where groupFlag is set, groupOff counts the starts).
-__global__ void tieFlagsKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, int32_t* dupFlag, int64_t n,
-							   const int64_t* groupOff, const int32_t* groupFlag, const int32_t* groupRel)
+// same key.
+__global__ void tieFlagsKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, int32_t* dupFlag, int64_t n)
 {
-	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const int q = findSegWave(qOff, nq, i);
-	const int64_t b = qOff[q], e = qOff[q + 1];
-	const uint64_t k = skey[i];
-	bool tied = (i > b && skey[i - 1] == k) || (i + 1 < e && skey[i + 1] == k);
-	if (tied && groupRel) tied = groupRel[groupOff[i] + groupFlag[i] - 1] != 0;
-	dupFlag[i] = tied ? 1 : 0;
+	const int64_t first = static_cast<int64_t>(blockIdx.x) * blockDim.x * kPosPerThread + threadIdx.x;
+	if (first - static_cast<int64_t>(laneId()) >= n) return;
+	int q = findSegWave(qOff, nq, first < n ? first : n - 1);
+	for (int j = 0; j < kPosPerThread; ++j)
+	{
+		const int64_t i = first + static_cast<int64_t>(j) * blockDim.x;
+		if (i >= n) break;
+		q = stepSeg(qOff, nq, q, i);
+		const int64_t b = qOff[q], e = qOff[q + 1];
+		const uint64_t k = skey[i];
+		const bool tied = (i > b && skey[i - 1] == k) || (i + 1 < e && skey[i + 1] == k);
+		dupFlag[i] = tied ? 1 : 0;
+	}
 }
 
 // Rank keys (first sorted position of the key in its query) in emission
@@ -53,18 +61,28 @@ __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* 
 							   const int64_t* dup, uint32_t* rkey, int64_t n)
 {
 	// rank keys are scattered inside the query's range: one XCD per range
-	const int64_t i = static_cast<int64_t>(xcdBlockIndex()) * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const int q = findSegWave(qOff, nq, i);
-	const int64_t b = qOff[q], e = qOff[q + 1];
-	if (dup[e] == dup[b]) return;
-	const uint64_t k = skey[i];
-	int64_t first = i;
-	while (first > b && skey[first - 1] == k) --first;
-	rkey[b + sval[i]] = static_cast<uint32_t>(first - b);
+	const int64_t first = static_cast<int64_t>(xcdBlockIndex()) * blockDim.x * kPosPerThread + threadIdx.x;
+	if (first - static_cast<int64_t>(laneId()) >= n) return;
+	int q = findSegWave(qOff, nq, first < n ? first : n - 1);
+	for (int j = 0; j < kPosPerThread; ++j)
+	{
+		const int64_t i = first + static_cast<int64_t>(j) * blockDim.x;
+		if (i >= n) break;
+		q = stepSeg(qOff, nq, q, i);
+		const int64_t b = qOff[q], e = qOff[q + 1];
+		if (dup[e] == dup[b]) continue;
+		const uint64_t k = skey[i];
+		int64_t f = i;
+		while (f > b && skey[f - 1] == k) --f;
+		rkey[b + sval[i]] = static_cast<uint32_t>(f - b);
+	}
 }
 
-inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n + tpb - 1) / tpb)); }
+inline dim3 gridForPos(int64_t n, int tpb)
+{
+	const int64_t per = static_cast<int64_t>(tpb) * kPosPerThread;
+	return dim3(static_cast<unsigned>((n + per - 1) / per));
+}
 
 // Stable counting sort of one query's matches by internal extId (the high 32
 // bits of the key), one 1,024-thread block per query:
@@ -77,7 +95,13 @@ inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n +
 //      waves' leaders claim their slots from the histogram in emission order
 //      (one wave at a time, a block barrier between turns), and every match
 //      is written to its slot.
-// Equal extIds keep their emission order, which is the stable order.
+// Equal extIds keep their emission order, which is the stable order.  The
+// join writes every match's emission index, which is its position in the
+// query's emission order, so the values are the positions themselves.  With
+// hcur set the kernel also writes the chain's hits in S order: curPos, the
+// match's extPos (mext in emission order) and the group-start flag (the first
+// slot of an extId; the first claim of a bin sets bit 31 of its histogram
+// word and passes it on in bit 31 of the leader's base).
 constexpr int kMsortThreads = 1024;
 constexpr int kMsortWaves = kMsortThreads / FH_WAVE;
 constexpr int kMsortRows = 4;
@@ -109,9 +133,11 @@ __device__ inline unsigned long long sameBinMask(int bin, int bits)
 constexpr int kMsortFixedWords = kMsortRows * kMsortThreads + kMsortWaves;
 
 __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int64_t* qOff, const int32_t* order,
-																	  const uint64_t* akey, const uint32_t* aval,
-																	  uint64_t* skey, uint32_t* sval, int nBins, int bits)
+																	  const uint64_t* akey, uint64_t* skey,
+																	  uint32_t* sval, int nBins, int bits,
+																	  MsortHits hits)
 {
+	constexpr uint32_t kClaimed = 0x80000000u;
 	extern __shared__ uint32_t lds[];
 	uint32_t* hist = lds;
 	uint32_t* lead = lds + nBins;                       // kMsortRows * kMsortThreads
@@ -154,30 +180,31 @@ __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int6
 	// tile t0 is processed while the next one loads: the barriers of the
 	// claiming turns order LDS only, so the next tile's global loads stay in
 	// flight across them
+	const bool withHits = hits.hcur != nullptr;
 	uint64_t key[kMsortRows];
-	uint32_t val[kMsortRows];
+	int32_t ext[kMsortRows];
 	int bin[kMsortRows];
-	auto load = [&](int64_t base, uint64_t* k, uint32_t* v, int* bn) {
+	auto load = [&](int64_t base, uint64_t* k, int32_t* e, int* bn) {
 #pragma unroll
 		for (int r = 0; r < kMsortRows; ++r)
 		{
 			const int64_t i = base + (wv * kMsortRows + r) * FH_WAVE + lane;
 			const bool act = i < n;
 			k[r] = act ? akey[b0 + i] : 0;
-			v[r] = act ? aval[b0 + i] : 0;
+			e[r] = act && withHits ? hits.mext[b0 + i] : 0;
 			bn[r] = act ? static_cast<int>(k[r] >> 32) : -1;
 		}
 	};
-	load(0, key, val, bin);
+	load(0, key, ext, bin);
 	for (int64_t t0 = 0; t0 < n; t0 += kMsortRows * kMsortThreads)
 	{
 		unsigned long long same[kMsortRows];
 #pragma unroll
 		for (int r = 0; r < kMsortRows; ++r) same[r] = sameBinMask(bin[r], bits);
 		uint64_t nkey[kMsortRows];
-		uint32_t nval[kMsortRows];
+		int32_t next[kMsortRows];
 		int nbin[kMsortRows];
-		load(t0 + kMsortRows * kMsortThreads, nkey, nval, nbin);
+		load(t0 + kMsortRows * kMsortThreads, nkey, next, nbin);
 		// leaders claim slots in emission order: wave w holds the tile's
 		// elements [w * 256, (w + 1) * 256) as 4 rows of 64, so the waves take
 		// turns in order and each claims its rows in order (LDS operations of
@@ -191,9 +218,11 @@ __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int6
 				{
 					if (bin[r] >= 0 && (same[r] & lt) == 0)
 					{
-						const uint32_t base = hist[bin[r]];
-						hist[bin[r]] = base + static_cast<uint32_t>(__popcll(same[r]));
-						lead[r * kMsortThreads + t] = base;
+						const uint32_t h = hist[bin[r]];
+						const uint32_t base = h & ~kClaimed;
+						hist[bin[r]] = (base + static_cast<uint32_t>(__popcll(same[r]))) | kClaimed;
+						// bit 31: this run opens the bin (its first slot starts a group)
+						lead[r * kMsortThreads + t] = base | (h & kClaimed ? 0u : kClaimed);
 					}
 				}
 			}
@@ -204,17 +233,24 @@ __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int6
 		{
 			if (bin[r] < 0) continue;
 			const int ld = __ffsll(static_cast<long long>(same[r])) - 1;
-			const uint32_t base = lead[r * kMsortThreads + wv * FH_WAVE + ld];
-			const int64_t d = b0 + base + __popcll(same[r] & lt);
+			const uint32_t lw = lead[r * kMsortThreads + wv * FH_WAVE + ld];
+			const int rank = __popcll(same[r] & lt);
+			const int64_t d = b0 + (lw & ~kClaimed) + rank;
 			skey[d] = key[r];
-			sval[d] = val[r];
+			sval[d] = static_cast<uint32_t>(t0 + (wv * kMsortRows + r) * FH_WAVE + lane);
+			if (withHits)
+			{
+				hits.hcur[d] = static_cast<int32_t>(static_cast<uint32_t>(key[r]));
+				hits.hext[d] = ext[r];
+				hits.groupFlag[d] = (lw & kClaimed) && rank == 0 ? 1 : 0;
+			}
 		}
 		ldsBarrier();  // lead[] is rewritten by the next tile
 #pragma unroll
 		for (int r = 0; r < kMsortRows; ++r)
 		{
 			key[r] = nkey[r];
-			val[r] = nval[r];
+			ext[r] = next[r];
 			bin[r] = nbin[r];
 		}
 	}
@@ -257,7 +293,7 @@ hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBit
 	{
 		const size_t ldsBytes = static_cast<size_t>(nBins + kMsortFixedWords) * 4;
 		hipLaunchKernelGGL(matchCountSortKernel, dim3(static_cast<unsigned>(a.nQueries)), dim3(kMsortThreads), ldsBytes, s,
-						   a.qOff, a.order, a.akey, a.aval, a.skey, a.sval, nBins, idBits);
+						   a.qOff, a.order, a.akey, a.skey, a.sval, nBins, idBits, a.hits);
 		return hipGetLastError();
 	}
 	return hipcub::DeviceSegmentedRadixSort::SortPairs(temp, tempBytes, a.akey, a.skey, a.aval, a.sval,
@@ -269,8 +305,8 @@ hipError_t launchTieFlags(const MsortArgs& a, int64_t nMatches, hipStream_t s)
 {
 	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
 	const int tpb = 256;
-	hipLaunchKernelGGL(tieFlagsKernel, gridFor(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.dupFlag,
-					   nMatches, a.groupOff, a.groupFlag, a.groupRel);
+	hipLaunchKernelGGL(tieFlagsKernel, gridForPos(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey,
+					   a.dupFlag, nMatches);
 	return hipGetLastError();
 }
 
@@ -278,7 +314,7 @@ hipError_t launchRankKeys(const MsortArgs& a, const int64_t* dup, int64_t nMatch
 {
 	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
 	const int tpb = 256;
-	hipLaunchKernelGGL(rankKeysKernel, gridFor(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.sval,
+	hipLaunchKernelGGL(rankKeysKernel, gridForPos(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.sval,
 					   dup, a.rkey, nMatches);
 	return hipGetLastError();
 }

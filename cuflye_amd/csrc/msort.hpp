@@ -7,6 +7,16 @@
 namespace fh
 {
 
+// The chain's hits in S order, written by the counting sort when hcur is set:
+// curPos, extPos (from mext, emission order) and the group-start flags.
+struct MsortHits
+{
+	const int32_t* mext = nullptr;
+	int32_t* hcur = nullptr;
+	int32_t* hext = nullptr;
+	int32_t* groupFlag = nullptr;
+};
+
 struct MsortArgs
 {
 	const int64_t* qOff;   // per query match offsets (nQueries + 1)
@@ -18,23 +28,19 @@ struct MsortArgs
 	uint32_t* rkey;        // emission-order rank keys (filled here)
 	int32_t* dupFlag;      // per sorted position: a neighbouring position holds the same key
 	const int32_t* order;  // counting sort: queries by descending match count (null: in order)
-	// optional (all null: every tie counts): per sorted position, the group
-	// start flags and their exclusive scan, and per group whether the order
-	// of its tied hits can reach a result; only those ties are flagged
-	const int64_t* groupOff = nullptr;
-	const int32_t* groupFlag = nullptr;
-	const int32_t* groupRel = nullptr;
+	MsortHits hits;        // counting sort only
 };
 
 hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t* bytes);
 // Stable order by extId (emission order is already ascending in curPos):
 // a per-query counting sort in LDS when the nBins target ids fit it
-// (msortCounting), else a segmented radix sort on the extId bits.
+// (msortCounting; it also writes a.hits when set), else a segmented radix
+// sort on the extId bits.
 bool msortCounting(int nBins);
 hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBits, int nBins, void* temp,
 								 size_t tempBytes, hipStream_t s);
-// Tied-position flags of the stable order (with groupRel set, only the ties
-// of the groups marked relevant).
+// Tied-position flags of the stable order: a neighbouring position holds the
+// same key.
 hipError_t launchTieFlags(const MsortArgs& a, int64_t nMatches, hipStream_t s);
 // Rank keys (emission order) of the queries that hold a flagged position;
 // dup is the exclusive prefix of the flags (nMatches + 1 words).

@@ -721,7 +721,7 @@ void runSegSort(Lane* L, SegSortBufs& b, int64_t* tieArrays, bool* exactPending 
 // (even one twice, where a tie straddles a dropped and a replayed range), so
 // the caller takes those from the stable order (their order cannot reach a
 // result).
-void runTieReplay(Lane* L, const JoinArgs& ja, int64_t nMatches, MsortArgs& ms)
+void runTieReplay(Lane* L, const JoinArgs& ja, int64_t nMatches, MsortArgs& ms, bool flagsReady = false)
 {
 	flyehip_ctx* c = L->c;
 	const size_t nm = static_cast<size_t>(nMatches) + 1;
@@ -736,12 +736,16 @@ void runTieReplay(Lane* L, const JoinArgs& ja, int64_t nMatches, MsortArgs& ms)
 	sb.posL = ensureT<int32_t>(c, L->bPosL, nm);
 	sb.posR = ensureT<int32_t>(c, L->bPosR, nm);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
-	// tie flags: sorted key read, flag written (12 bytes per match); rank
-	// keys of the queries with a flagged position: sorted key and value read,
-	// rank key written (16 bytes per match of those queries, counted over all)
-	const int tk = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 12.0 * static_cast<double>(nMatches));
-	HIPCHECK(launchTieFlags(ms, nMatches, L->stream));
-	L->timer.end(tk, L->stream);
+	// tie flags (unless the caller wrote them): sorted key read, flag written
+	// (12 bytes per match); rank keys of the queries with a flagged position:
+	// sorted key and value read, rank key written (16 bytes per match of
+	// those queries, not counted)
+	if (!flagsReady)
+	{
+		const int tk = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 12.0 * static_cast<double>(nMatches));
+		HIPCHECK(launchTieFlags(ms, nMatches, L->stream));
+		L->timer.end(tk, L->stream);
+	}
 	HIPCHECK(exclusiveScan(ms.dupFlag, dup, nMatches, tmp, L->stream));
 	const int tr = L->timer.begin(FLYEHIP_K_RANK_KEYS, L->stream, 0.0);
 	HIPCHECK(launchRankKeys(ms, dup, nMatches, L->stream));
@@ -777,8 +781,10 @@ MsortArgs matchSortArgs(Lane* L, const JoinArgs& ja, int64_t nMatches)
 
 // Stable (extId, curPos) order of every query's matches (bSKey, bVal); with
 // replay the tie replay follows, so bVal becomes std::sort's permutation
-// (emission index within the query).
-void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches, bool replay = true)
+// (emission index within the query).  With withHits the counting sort also
+// writes the chain's hits in that order (bHCur, bHExt, bGroupFlag); returns
+// whether it did (the radix fallback leaves them to gatherHitsKernel).
+bool runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches, bool replay = true, bool withHits = false)
 {
 	flyehip_ctx* c = L->c;
 	if (nMatches >= 0x7fffffffLL) fail(FLYEHIP_ERR_BUDGET, "batch of %lld matches exceeds 2^31 - 1", static_cast<long long>(nMatches));
@@ -803,14 +809,25 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches, bool replay = t
 		upload(L, dOrder, order.data(), order.size() * sizeof(int32_t));
 		ms.order = dOrder;
 	}
+	const bool hits = counting && withHits;
+	if (hits)
+	{
+		const size_t nm = static_cast<size_t>(nMatches) + 1;
+		ms.hits.mext = ja.mext;
+		ms.hits.hcur = ensureT<int32_t>(c, L->bHCur, nm);
+		ms.hits.hext = ensureT<int32_t>(c, L->bHExt, nm);
+		ms.hits.groupFlag = ensureT<int32_t>(c, L->bGroupFlag, nm);
+	}
 	size_t tempBytes = 0;
 	if (!counting) HIPCHECK(msortTempBytes(nMatches, nq, bits, &tempBytes));
 	void* temp = counting ? nullptr : ensure(c, L->bMsortTemp, tempBytes + 256);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
 	// stable order: the counting sort reads the keys twice (histogram, then
-	// with the values) and writes keys and values once (32 bytes per match);
-	// the radix fallback makes 2 passes x 24 bytes
-	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, (counting ? 32.0 : 48.0) * static_cast<double>(nMatches));
+	// to place them) and writes keys and values once (28 bytes per match),
+	// with the hits extPos read and curPos, extPos and group flag written (16
+	// more); the radix fallback makes 2 passes x 24 bytes
+	const double perMatch = counting ? (hits ? 44.0 : 28.0) : 48.0;
+	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, perMatch * static_cast<double>(nMatches));
 	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, nBins, temp, tempBytes, L->stream));
 	L->timer.end(tr, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[1], L->stream));
@@ -819,6 +836,7 @@ void runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches, bool replay = t
 	HIPCHECK(hipEventElapsedTime(&es, L->ev[0], L->ev[1]));
 	L->stats.ms_sort += es;
 	if (replay) runTieReplay(L, ja, nMatches, ms);
+	return hits;
 }
 
 // Order, prefilters + extPos sort, DP, score sort + walks over ca.nGroups
@@ -1020,7 +1038,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 // other tie keeps the stable order, which no result can tell from Flye's.
 BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vector<int64_t>& qIds,
 				  const std::vector<int32_t>& qLens, const flyehip_overlap_params_t& p, double* msGroup, double* msChain,
-				  bool replayTies = false)
+				  bool replayTies = false, bool hitsReady = false)
 {
 	flyehip_ctx* c = L->c;
 	BatchOut bo;
@@ -1033,12 +1051,15 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 	int64_t* tmp = ensureT<int64_t>(c, L->bScanTmp, scanTempWords(nMatches) + 64);
 	const int64_t* qMatchOff = static_cast<const int64_t*>(L->bQMatchOff.p);
 	const uint64_t* skey = static_cast<const uint64_t*>(L->bSKey.p);
-	// gather: sorted key, permutation and extPos read, hits and group flags
-	// written (8 + 4 + 4 + 8 + 4 bytes per match)
-	const int tgk = L->timer.begin(FLYEHIP_K_GATHER, L->stream, 28.0 * static_cast<double>(nMatches));
-	HIPCHECK(launchGatherHits(qMatchOff, nq, skey, static_cast<const uint32_t*>(L->bVal.p), ja.mext, hcur, hext, gflag,
-							  nMatches, L->stream));
-	L->timer.end(tgk, L->stream);
+	if (!hitsReady)
+	{
+		// gather: sorted key, permutation and extPos read, hits and group
+		// flags written (8 + 4 + 4 + 8 + 4 bytes per match)
+		const int tgk = L->timer.begin(FLYEHIP_K_GATHER, L->stream, 28.0 * static_cast<double>(nMatches));
+		HIPCHECK(launchGatherHits(qMatchOff, nq, skey, static_cast<const uint32_t*>(L->bVal.p), ja.mext, hcur, hext,
+								  gflag, nMatches, L->stream));
+		L->timer.end(tgk, L->stream);
+	}
 	HIPCHECK(exclusiveScan(gflag, goff, nMatches, tmp, L->stream));
 	int64_t nGroups = 0;
 	download(L, &nGroups, goff + nMatches, sizeof(int64_t));
@@ -1072,15 +1093,15 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 		zero32(L, queues, 16);
 		ra.queue = queues + 15;
 		int32_t* groupRel = ensureT<int32_t>(c, L->bGroupRel, static_cast<size_t>(nGroups) + 1);
+		MsortArgs ms = matchSortArgs(L, ja, nMatches);
+		// the tie flags start cleared: the relevance pass sets those of the
+		// tied hits of the groups it marks
+		zero32(L, ms.dupFlag, nMatches);
 		// the prefilters (hits read, 8 bytes each) and the extPos duplicate test
 		const int tp = L->timer.begin(FLYEHIP_K_PREFILTER, L->stream, 8.0 * static_cast<double>(nMatches));
-		HIPCHECK(launchChainRelevance(ra, groupRel, chainBlocks(c), L->stream));
+		HIPCHECK(launchChainRelevance(ra, groupRel, ms.dupFlag, chainBlocks(c), L->stream));
 		L->timer.end(tp, L->stream);
-		MsortArgs ms = matchSortArgs(L, ja, nMatches);
-		ms.groupOff = goff;
-		ms.groupFlag = gflag;
-		ms.groupRel = groupRel;
-		runTieReplay(L, ja, nMatches, ms);
+		runTieReplay(L, ja, nMatches, ms, true);
 		HIPCHECK(launchFixTiedHits(qMatchOff, nq, ms.dupFlag, ms.sval, ja.mext, hext, nMatches, L->stream));
 	}
 	sync(L);
@@ -1278,8 +1299,8 @@ void runBatch(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& all
 	JoinArgs ja;
 	BatchOut jo = runJoin(L, seqs, qs, ja);
 	sync(L);
-	runMatchSort(L, ja, jo.matches, false);
-	BatchOut co = runChain(L, ja, jo.matches, ids, lens, p, nullptr, nullptr, true);
+	const bool hitsReady = runMatchSort(L, ja, jo.matches, false, true);
+	BatchOut co = runChain(L, ja, jo.matches, ids, lens, p, nullptr, nullptr, true, hitsReady);
 	const int32_t nq = b1 - b0;
 	out.count.assign(static_cast<size_t>(nq), 0);
 	out.rows.resize(static_cast<size_t>(co.rows));
