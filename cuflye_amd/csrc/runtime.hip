@@ -159,15 +159,15 @@ struct AllocGate
 	}
 };
 
-// One batch pipeline: a stream (plus side streams) and the grow-only batch
+// One batch pipeline: a stream (plus a side stream) and the grow-only batch
 // buffers.  Batches on different lanes run concurrently; they share the read
 // store, the index and the memory budget of the context.
 struct Lane
 {
 	flyehip_ctx* c = nullptr;
 	hipStream_t stream = nullptr;
-	hipStream_t aux[3] = {};    // side streams (default priority)
-	hipEvent_t auxEv[5] = {};   // timing disabled
+	hipStream_t side = nullptr;  // side stream (default priority)
+	hipEvent_t sideEv[2] = {};   // fork, done (timing disabled)
 	hipEvent_t ev[8] = {};
 	int64_t* hostSeg = nullptr;  // pinned counter words of the segmented sort
 	flyehip_stats_t stats{};     // stage timings and counts of this lane's batches
@@ -290,11 +290,11 @@ void release(flyehip_ctx* c, DevBuf& b)
 // The lane the calling host thread drives inside a multi-lane call.
 thread_local Lane* tlsLane = nullptr;
 
-// Waits for everything a lane has enqueued (its stream and side streams).
+// Waits for everything a lane has enqueued (its stream and side stream).
 void quiesce(Lane* L)
 {
 	HIPCHECK(hipStreamSynchronize(L->stream));
-	for (hipStream_t a : L->aux) HIPCHECK(hipStreamSynchronize(a));
+	HIPCHECK(hipStreamSynchronize(L->side));
 }
 
 void* ensure(flyehip_ctx* c, DevBuf& b, size_t bytes)
@@ -697,8 +697,9 @@ SegSortBufs segBufs(Lane* L, int64_t n, int64_t nArrays)
 	b.verifyCap = caps.verifyCap;
 	b.chunkCap = caps.chunkCap;
 	b.hostCounters = L->hostSeg;
-	for (int t = 0; t < 3; ++t) b.aux[t] = L->aux[t];
-	for (int t = 0; t < 5; ++t) b.auxEv[t] = L->auxEv[t];
+	b.side = L->side;
+	b.forkEv = L->sideEv[0];
+	b.doneEv = L->sideEv[1];
 	b.timer = &L->timer;
 	return b;
 }
@@ -878,7 +879,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	HIPCHECK(launchChainPrefilter(ca, sb, blocks, L->stream));
 	L->timer.end(tp, L->stream);
 	// the extPos sort leaves its exact heap sorts (libstdc++'s heapsort
-	// fallback, a dependent chain per range) running on the side streams: the
+	// fallback, a dependent chain per range) running on the side stream: the
 	// groups that hold such a range are deferred, and the gather + DP of all
 	// other passing groups run meanwhile
 	bool exactPending = false;
@@ -893,7 +894,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 		{
 			if (*pending) (void)hipStreamWaitEvent(s, e, 0);
 		}
-	} joinOnExit{L->stream, L->auxEv[4], &exactPending};
+	} joinOnExit{L->stream, L->sideEv[1], &exactPending};
 	runSegSort(L, sb, nullptr, &exactPending);
 	HIPCHECK(launchPassFlags(ca, flag, L->stream));
 	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
@@ -948,7 +949,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	if (deferOrder)
 	{
 		exactPending = false;
-		HIPCHECK(hipStreamWaitEvent(L->stream, L->auxEv[4], 0));
+		HIPCHECK(hipStreamWaitEvent(L->stream, L->sideEv[1], 0));
 		ca.passOrder = deferOrder;
 		ca.nPass = nPassAll - nReady;
 		ca.queue = queues + 6;
@@ -1211,8 +1212,8 @@ void createLane(flyehip_ctx* c)
 	l->c = c;
 	HIPCHECK(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
 	for (auto& e : l->ev) HIPCHECK(hipEventCreate(&e));
-	for (auto& a : l->aux) HIPCHECK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
-	for (auto& e : l->auxEv) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+	HIPCHECK(hipStreamCreateWithFlags(&l->side, hipStreamNonBlocking));
+	for (auto& e : l->sideEv) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 	HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&l->hostSeg), kSegCounters * sizeof(int64_t)));
 	HIPCHECK(l->timer.create());
 	c->lanes.push_back(std::move(l));
@@ -1244,10 +1245,9 @@ void destroyLane(flyehip_ctx* c, Lane& l)
 	l.timer.destroy();
 	for (auto& e : l.ev)
 		if (e) (void)hipEventDestroy(e);
-	for (auto& e : l.auxEv)
+	for (auto& e : l.sideEv)
 		if (e) (void)hipEventDestroy(e);
-	for (auto& a : l.aux)
-		if (a) (void)hipStreamDestroy(a);
+	if (l.side) (void)hipStreamDestroy(l.side);
 	if (l.stream) (void)hipStreamDestroy(l.stream);
 }
 
@@ -1379,7 +1379,7 @@ std::vector<BatchResult> runBatches(flyehip_ctx* c, const PackedSeqs& seqs, cons
 						if (c->lanesRunning > 1)
 						{
 							(void)hipStreamSynchronize(L->stream);
-							for (hipStream_t a : L->aux) (void)hipStreamSynchronize(a);
+							(void)hipStreamSynchronize(L->side);
 							c->gate.leave();
 						}
 					}

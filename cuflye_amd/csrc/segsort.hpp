@@ -104,9 +104,9 @@ struct SegSortBufs
 	int64_t verifyCap;      // verify entries: big ranges and leaf sub-ranges, all disjoint
 	int64_t chunkCap;
 	int64_t* hostCounters;  // pinned, kSegCounters words
-	hipStream_t aux[3];     // default-priority side streams for the heap-sort size classes
-	hipEvent_t auxEv[5];    // (timing disabled) [0..2] size classes done, [3] fork, [4] deferred
-							// exact heap sorts done
+	hipStream_t side;       // default-priority side stream of the deferred exact heap sorts
+	hipEvent_t forkEv;      // (timing disabled) side stream forked
+	hipEvent_t doneEv;      // (timing disabled) deferred exact heap sorts done
 	KTimer* timer;          // kernel-family intervals (may be null)
 };
 
@@ -204,10 +204,10 @@ hipError_t segSortReset(const SegSortBufs& b, hipStream_t s);
 // Runs the rounds and the leaf / heap passes over what the init kernels
 // pushed.  Synchronises the stream once per round (range counts).  With
 // exactPending set, the exact heap sorts of the verify entries that need one
-// (d == 1 after the verification on s) are left running on the side streams:
-// *exactPending tells whether any were launched, and b.auxEv[4] completes
-// with them; the caller makes its stream wait on that event before it reads
-// those ranges of outVal (or the side streams' timing intervals).
+// (d == 1 after the verification on s) are left running on the side stream:
+// *exactPending tells whether any were launched, and b.doneEv completes with
+// them; the caller makes its stream wait on that event before it reads those
+// ranges of outVal (or the side stream's timing intervals).
 hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* exactPending = nullptr);
 
 // Pushes every array [off[i], off[i+1]) (i < nArrays), pruned by b.dup.
