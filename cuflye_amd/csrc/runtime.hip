@@ -241,7 +241,7 @@ struct flyehip_ctx
 	int dpLanes = 8;
 	int queriesPerBatch = 2048;
 	int batchesInFlight = 2;
-	int chainBlocks = 0;
+	int numCUs = 0;
 
 	// read store
 	int32_t nReads = 0;
@@ -561,13 +561,21 @@ HashTable table(flyehip_ctx* c)
 	return t;
 }
 
+// Single-wave blocks of the chain's work-queue kernels: 16 per CU (half the
+// CU's wave slots at their register use) for one lane; with several lanes
+// running, 24 per CU shared among them (at least 4 each), so two lanes' DPs
+// leave a quarter of the slots to the other kernels in flight, above all the
+// deferred heap sorts, whose blocks would otherwise wait for the DPs to end.
 int chainBlocks(flyehip_ctx* c)
 {
-	if (c->chainBlocks > 0) return c->chainBlocks;
-	hipDeviceProp_t prop;
-	HIPCHECK(hipGetDeviceProperties(&prop, c->device));
-	c->chainBlocks = prop.multiProcessorCount * 16;
-	return c->chainBlocks;
+	if (c->numCUs == 0)
+	{
+		hipDeviceProp_t prop;
+		HIPCHECK(hipGetDeviceProperties(&prop, c->device));
+		c->numCUs = prop.multiProcessorCount;
+	}
+	const int lanes = c->lanesRunning;
+	return c->numCUs * (lanes > 1 ? std::max(4, 24 / lanes) : 16);
 }
 
 struct Timer

@@ -264,16 +264,17 @@ __global__ void __launch_bounds__(FH_WAVE) segLeafKernel(SegSortBufs b, int64_t 
 // exsort::siftStepPacked without branches: both children are loaded
 // together from adjacent words (the first index clamped to the range; hx has
 // one spare word after n; a clamped or spare load is not used), keys compared
-// on their 32-bit halves.  Same result: the right child unless it is below
-// the left one, moved up unless it is below the value.
-__device__ inline bool heapStep(uint64_t* hx, int& h, int m, int n, uint64_t v)
+// on the words' bits from sh up.  Same result: the right child unless it is
+// below the left one, moved up unless it is below the value.
+template <typename W>
+__device__ inline bool heapStep(W* hx, int& h, int m, int n, W v, int sh)
 {
 	const int c1 = 2 * h + 1;
 	const int cc = min(c1, n - 1);
-	const uint64_t x1 = hx[cc], x2 = hx[cc + 1];
-	const bool right = c1 + 1 < m && !(static_cast<uint32_t>(x2 >> 32) < static_cast<uint32_t>(x1 >> 32));
-	const uint64_t cv = right ? x2 : x1;
-	const bool up = c1 < m && !(static_cast<uint32_t>(cv >> 32) < static_cast<uint32_t>(v >> 32));
+	const W x1 = hx[cc], x2 = hx[cc + 1];
+	const bool right = c1 + 1 < m && !((x2 >> sh) < (x1 >> sh));
+	const W cv = right ? x2 : x1;
+	const bool up = c1 < m && !((cv >> sh) < (v >> sh));
 	hx[h] = up ? cv : v;
 	h = up ? (right ? c1 + 1 : c1) : h;
 	return up;
@@ -286,108 +287,184 @@ __device__ inline bool heapAncestorOrSelf(int h, int x)
 	return (X >> (__clz(H) - __clz(X))) == H;
 }
 
-// Heap sort, exactly libstdc++'s make_heap + sort_heap, of the ranges of a
-// list (only entries with d == 1 when onlyFlagged), one wave per range staged
-// in LDS as packed (key << 32 | value) words; the schedule of
+// Heap sort, exactly libstdc++'s make_heap + sort_heap, of the n words hx
+// (one wave; keys in the bits from sh up, compared alone); the schedule of
 // exsort::heapSortPipelinedModel: make_heap one heap level at a time with a
 // lane per parent, then sort_heap with up to kSlots pops in flight, one lane
-// each, started two steps apart and advancing one level per step.  Ranges of
-// size in (minN, cap] are taken (dynamic LDS of cap words); larger ones go to
-// segHeapGlobalKernel.  Values go to dstV, keys to dstK when it is set.
-__global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list, int64_t nList, int onlyFlagged,
-															const uint32_t* srcK, const uint32_t* srcV, uint32_t* dstK,
-															uint32_t* dstV, int32_t minN, int32_t cap)
+// each, started two steps apart and advancing one level per step.
+template <typename W>
+__device__ void waveHeapSort(W* hx, int n, int sh)
 {
 	constexpr int kSlots = 16;
-	extern __shared__ uint64_t hx[];
 	const int lane = laneId();
-	for (int64_t i = blockIdx.x; i < nList; i += gridDim.x)
+	if (n < 2) return;
+	// make_heap
+	const int lastParent = (n - 2) / 2;
+	const int top = 31 - __clz(static_cast<unsigned>(lastParent) + 1u);
+	for (int lv = top; lv >= 0; --lv)
 	{
-		const SegRange w = list[i];
-		const int n = w.l - w.f;
-		if (n > cap || n <= minN || (onlyFlagged && w.d != 1)) continue;
-		for (int t = lane; t < n; t += FH_WAVE)
-			hx[t] = (static_cast<uint64_t>(srcK[w.f + t]) << 32) | srcV[w.f + t];
-		waveSync();
-		if (n >= 2)
+		const int lo = (1 << lv) - 1;
+		const int hi = min(lastParent, (2 << lv) - 2);
+		for (int p = hi - lane; p >= lo; p -= FH_WAVE)
 		{
-			// make_heap
-			const int lastParent = (n - 2) / 2;
-			const int top = 31 - __clz(static_cast<unsigned>(lastParent) + 1u);
-			for (int lv = top; lv >= 0; --lv)
+			int hole = p;
+			const W pv = hx[p];
+			while (heapStep(hx, hole, n, n, pv, sh))
 			{
-				const int lo = (1 << lv) - 1;
-				const int hi = min(lastParent, (2 << lv) - 2);
-				for (int p = hi - lane; p >= lo; p -= FH_WAVE)
-				{
-					int hole = p;
-					const uint64_t pv = hx[p];
-					while (heapStep(hx, hole, n, n, pv))
-					{
-					}
-				}
-				waveSync();
 			}
-			// sort_heap
-			bool act = false;
-			int h = 0, m = 0;
-			uint64_t v = 0;
-			int L = n, pops = 0;
-			// LDS operations of one wave complete in program order, so a cycle's
-			// writes reach the next cycle's reads without waiting; the wave
-			// barrier only keeps the compiler from moving them across cycles.
-			// Pops start at most every other cycle, so cycles run in pairs and
-			// only the first of a pair may start one: the start checks (slot
-			// free, no sift above the last element) make the result independent
-			// of when a pop starts, and the second cycle is a plain step.
-			for (;;)
-			{
-				unsigned long long running = __ballot(act);
-				if (L > 1)
-				{
-					const int slot = pops % kSlots;
-					const bool blocked = __ballot(act && heapAncestorOrSelf(h, L - 1)) != 0;
-					if (((running >> slot) & 1ULL) == 0 && !blocked)
-					{
-						if (lane == slot)
-						{
-							v = hx[L - 1];
-							hx[L - 1] = hx[0];
-							h = 0;
-							m = L - 1;
-							act = true;
-						}
-						running |= 1ULL << slot;
-						--L;
-						++pops;
-					}
-				}
-				if (running == 0 && L <= 1) break;
-				if (act) act = heapStep(hx, h, m, n, v);
-				__builtin_amdgcn_wave_barrier();
-				if (act) act = heapStep(hx, h, m, n, v);
-				__builtin_amdgcn_wave_barrier();
-			}
-			waveSync();
-		}
-		for (int t = lane; t < n; t += FH_WAVE)
-		{
-			dstV[w.f + t] = static_cast<uint32_t>(hx[t]);
-			if (dstK) dstK[w.f + t] = static_cast<uint32_t>(hx[t] >> 32);
 		}
 		waveSync();
 	}
+	// sort_heap
+	bool act = false;
+	int h = 0, m = 0;
+	W v = 0;
+	int L = n, pops = 0;
+	// LDS operations of one wave complete in program order, so a cycle's
+	// writes reach the next cycle's reads without waiting; the wave barrier
+	// only keeps the compiler from moving them across cycles.  Pops start at
+	// most every other cycle, so cycles run in pairs and only the first of a
+	// pair may start one: the start checks (slot free, no sift above the last
+	// element) make the result independent of when a pop starts, and the
+	// second cycle is a plain step.
+	for (;;)
+	{
+		unsigned long long running = __ballot(act);
+		if (L > 1)
+		{
+			const int slot = pops % kSlots;
+			const bool blocked = __ballot(act && heapAncestorOrSelf(h, L - 1)) != 0;
+			if (((running >> slot) & 1ULL) == 0 && !blocked)
+			{
+				if (lane == slot)
+				{
+					v = hx[L - 1];
+					hx[L - 1] = hx[0];
+					h = 0;
+					m = L - 1;
+					act = true;
+				}
+				running |= 1ULL << slot;
+				--L;
+				++pops;
+			}
+		}
+		if (running == 0 && L <= 1) break;
+		if (act) act = heapStep(hx, h, m, n, v, sh);
+		__builtin_amdgcn_wave_barrier();
+		if (act) act = heapStep(hx, h, m, n, v, sh);
+		__builtin_amdgcn_wave_barrier();
+	}
+	waveSync();
 }
 
-// The same in global memory, in place on (k, v), for ranges above minN; the
-// values are then copied to outV when it is a separate array.
+// Where a range's heap sort runs.  The LDS kernel stages a range as 32-bit
+// words, (key - min key) << sh | position (sh the bits of n - 1), when the
+// key span fits the bits above sh, else as 64-bit (key << 32 | value) words,
+// in the first of three LDS sizes that holds it (16, 64 and 128 KiB, so
+// small ranges are not held to a few waves per CU by the largest, and the
+// 64 KiB ones fit beside two lanes' DPs); anything larger sorts in global
+// memory, one thread per range (class kHeapClasses).  kHeapNoBig: the device
+// grants no 128 KiB, so that class falls to global memory.
+constexpr int kHeapClasses = 3;
+__host__ __device__ constexpr int heapLdsBytes(int cls) { return cls == 0 ? 16 * 1024 : (cls == 1 ? 64 * 1024 : 128 * 1024); }
+constexpr int kHeapNoBig = 1;
+
+__device__ inline int heapShift(int n) { return n <= 2 ? 1 : 32 - __clz(static_cast<unsigned>(n - 1)); }
+__device__ inline bool heapFits32(uint32_t span, int sh) { return (static_cast<uint64_t>(span) >> (32 - sh)) == 0; }
+__device__ inline int heapClass(int n, bool fits32, int flags)
+{
+	const int top = (flags & kHeapNoBig) ? kHeapClasses - 1 : kHeapClasses;
+	for (int c = 0; c < top; ++c)
+	{
+		// one spare word after the range (heapStep's paired load)
+		if ((fits32 && (n + 1) * 4 <= heapLdsBytes(c)) || (n + 1) * 8 <= heapLdsBytes(c)) return c;
+	}
+	return kHeapClasses;
+}
+
+// Heap sorts of the ranges of a list of class cls (an LDS class; only
+// entries with d == 1 when onlyFlagged), one wave per range (grid: one block
+// per entry, dynamic LDS heapLdsBytes(cls)).  Values go to dstV, keys to dstK
+// when it is set.
+__global__ void __launch_bounds__(FH_WAVE) segHeapLdsKernel(const SegRange* list, int64_t nList, int onlyFlagged,
+															const uint32_t* srcK, const uint32_t* srcV, uint32_t* dstK,
+															uint32_t* dstV, int cls, int flags)
+{
+	extern __shared__ uint64_t hx[];
+	const int lane = laneId();
+	const int64_t i = blockIdx.x;
+	if (i >= nList) return;
+	const SegRange w = list[i];
+	const int n = w.l - w.f;
+	if (n < 1 || (onlyFlagged && w.d != 1)) return;
+	const uint32_t* k = srcK + w.f;
+	uint32_t mn = 0xffffffffu, mx = 0;
+	for (int t = lane; t < n; t += FH_WAVE)
+	{
+		mn = min(mn, k[t]);
+		mx = max(mx, k[t]);
+	}
+	for (int o = 32; o > 0; o >>= 1)
+	{
+		mn = min(mn, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mn), o, FH_WAVE)));
+		mx = max(mx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), o, FH_WAVE)));
+	}
+	const int sh = heapShift(n);
+	const bool fits = heapFits32(mx - mn, sh);
+	if (heapClass(n, fits, flags) != cls) return;
+	if (fits && (n + 1) * 4 <= heapLdsBytes(cls))
+	{
+		uint32_t* hw = reinterpret_cast<uint32_t*>(hx);
+		for (int t = lane; t < n; t += FH_WAVE) hw[t] = ((k[t] - mn) << sh) | static_cast<uint32_t>(t);
+		waveSync();
+		waveHeapSort(hw, n, sh);
+		// the values by position, through LDS: every source value is read
+		// before any is written (dstV may be srcV)
+		const uint32_t mask = (1u << sh) - 1u;
+		for (int t = lane; t < n; t += FH_WAVE)
+		{
+			const uint32_t x = hw[t];
+			if (dstK) dstK[w.f + t] = (x >> sh) + mn;
+			hw[t] = srcV[w.f + static_cast<int>(x & mask)];
+		}
+		waveSync();
+		for (int t = lane; t < n; t += FH_WAVE) dstV[w.f + t] = hw[t];
+	}
+	else
+	{
+		for (int t = lane; t < n; t += FH_WAVE) hx[t] = (static_cast<uint64_t>(k[t]) << 32) | srcV[w.f + t];
+		waveSync();
+		waveHeapSort(hx, n, 32);
+		for (int t = lane; t < n; t += FH_WAVE)
+		{
+			const uint64_t x = hx[t];
+			dstV[w.f + t] = static_cast<uint32_t>(x);
+			if (dstK) dstK[w.f + t] = static_cast<uint32_t>(x >> 32);
+		}
+	}
+}
+
+// The ranges beyond the LDS classes in global memory, in place on (k, v),
+// one thread per range; the values are then copied to outV when it is a
+// separate array.
 __global__ void segHeapGlobalKernel(const SegRange* list, int64_t nList, int onlyFlagged, uint32_t* k, uint32_t* v,
-									uint32_t* outV, int32_t minN)
+									uint32_t* outV, int flags)
 {
 	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (i >= nList) return;
 	const SegRange w = list[i];
-	if (w.l - w.f <= minN || (onlyFlagged && w.d != 1)) return;
+	const int n = w.l - w.f;
+	if (n < 1 || (onlyFlagged && w.d != 1)) return;
+	// 64-bit words of the largest LDS class hold it whatever its keys
+	if ((n + 1) * 8 <= heapLdsBytes(kHeapClasses - ((flags & kHeapNoBig) ? 2 : 1))) return;
+	uint32_t mn = 0xffffffffu, mx = 0;
+	for (int32_t x = w.f; x < w.l; ++x)
+	{
+		mn = min(mn, k[x]);
+		mx = max(mx, k[x]);
+	}
+	if (heapClass(n, heapFits32(mx - mn, heapShift(n)), flags) != kHeapClasses) return;
 	exsort::heapSort(exsort::Span<uint32_t>{k, v}, w.f, w.l);
 	if (outV != v)
 		for (int32_t x = w.f; x < w.l; ++x) outV[x] = v[x];
@@ -469,13 +546,14 @@ hipError_t download(const SegSortBufs& b, hipStream_t s)
 	return hipStreamSynchronize(s);
 }
 
-// Heap sorts of a list, in three size classes staged in LDS (up to 2048,
-// 8192 and 16384 elements: 16, 64 and 128 KiB per wave, so small ranges are
-// not held to one wave per CU by the largest), launched concurrently on the
-// side streams; ranges above 16384 elements sort in place on gk / gv in
-// global memory.  With join the calling stream waits for all of them;
-// otherwise they are joined on the first side stream, where auxEv[4] is
-// recorded after the last of them.
+// Heap sorts of a list (heapClass), one launch per class with a block per
+// entry, largest class first (it holds the longest chains), one after the
+// other; the ranges beyond them sort in place on gk / gv in global memory.  With join they run on the calling stream; otherwise on the
+// side stream, forked from it, where doneEv is recorded after the last of
+// them.  (One side stream per lane: with the lanes' main streams that is as
+// many streams as the runtime's default hardware queues, so no stream shares
+// a queue, whose in-order packets would make one lane's heap sorts wait for
+// another's.)
 hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int onlyFlagged, const uint32_t* srcK,
 					 const uint32_t* srcV, uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s,
 					 int* firstSlot, bool join = true)
@@ -483,46 +561,32 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 	*firstSlot = -1;
 	static const bool bigLds =
 		hipFuncSetAttribute(reinterpret_cast<const void*>(segHeapLdsKernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-							(16384 + 1) * 8) == hipSuccess;
-	const int32_t caps[3] = {2048, 8192, bigLds ? 16384 : 8192};
-	const unsigned grid = static_cast<unsigned>(n < 8192 ? n : 8192);
-	hipError_t e = hipEventRecord(b.auxEv[3], s);
-	if (e != hipSuccess) return e;
-	int32_t lo = 0;
-	int last = 0;
-	for (int t = 0; t < 3; ++t)
+							heapLdsBytes(kHeapClasses - 1)) == hipSuccess;
+	const int flags = bigLds ? 0 : kHeapNoBig;
+	if (n <= 0) return hipSuccess;
+	if (n > 0x7fffffffLL) return hipErrorInvalidValue;
+	hipError_t e = hipSuccess;
+	hipStream_t hs = s;
+	if (!join)
 	{
-		if (caps[t] <= lo) break;
-		if ((e = hipStreamWaitEvent(b.aux[t], b.auxEv[3], 0)) != hipSuccess) return e;
-		const int slot = b.timer ? b.timer->begin(FLYEHIP_K_HEAP, b.aux[t]) : -1;
-		if (t == 0) *firstSlot = slot;
-		// one spare word after the largest range (heapStep's paired load)
-		hipLaunchKernelGGL(segHeapLdsKernel, dim3(grid), dim3(FH_WAVE), static_cast<size_t>(caps[t] + 1) * 8, b.aux[t],
-						   list, n, onlyFlagged, srcK, srcV, dstK, dstV, lo, caps[t]);
-		if (b.timer) b.timer->end(slot, b.aux[t]);
-		if (join)
-		{
-			if ((e = hipEventRecord(b.auxEv[t], b.aux[t])) != hipSuccess) return e;
-			if ((e = hipStreamWaitEvent(s, b.auxEv[t], 0)) != hipSuccess) return e;
-		}
-		lo = caps[t];
-		last = t;
+		if ((e = hipEventRecord(b.forkEv, s)) != hipSuccess) return e;
+		if ((e = hipStreamWaitEvent(b.side, b.forkEv, 0)) != hipSuccess) return e;
+		hs = b.side;
 	}
-	if (join)
-	{
-		hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, s, list, n, onlyFlagged, gk, gv,
-						   dstV, lo);
-		return hipGetLastError();
-	}
-	hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, b.aux[last], list, n, onlyFlagged, gk,
-					   gv, dstV, lo);
+	hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, hs, list, n, onlyFlagged, gk, gv, dstV,
+					   flags);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
-	for (int t = 1; t <= last; ++t)
+	for (int cls = kHeapClasses - (bigLds ? 1 : 2); cls >= 0; --cls)
 	{
-		if ((e = hipEventRecord(b.auxEv[t], b.aux[t])) != hipSuccess) return e;
-		if ((e = hipStreamWaitEvent(b.aux[0], b.auxEv[t], 0)) != hipSuccess) return e;
+		const int slot = b.timer ? b.timer->begin(FLYEHIP_K_HEAP, hs) : -1;
+		if (*firstSlot < 0) *firstSlot = slot;
+		hipLaunchKernelGGL(segHeapLdsKernel, dim3(static_cast<unsigned>(n)), dim3(FH_WAVE),
+						   static_cast<size_t>(heapLdsBytes(cls)), hs, list, n, onlyFlagged, srcK, srcV, dstK, dstV,
+						   cls, flags);
+		if (b.timer) b.timer->end(slot, hs);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}
-	return hipEventRecord(b.auxEv[4], b.aux[0]);
+	return join ? hipSuccess : hipEventRecord(b.doneEv, hs);
 }
 
 // Leaf bytes: key and value read, value written (12 per element).  The leaves
