@@ -202,24 +202,31 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 				}
 			}
 			const bool extSorted = pass && gd.extLen > qd.curLen;
+			// extPos already strictly ascending: distinct keys in order, which
+			// every sort leaves as they are, so the group needs no sort
+			bool ascending = true;
 			if (extSorted)
 			{
 				uint32_t* sk = a.sortKey + hb;
 				uint32_t* sv = a.sortVal + hb;
+				bool down = false;
 #pragma unroll 4
 				for (int i = lane; i < n; i += FH_WAVE)
 				{
-					sk[i] = static_cast<uint32_t>(ext0[i]);
+					const int32_t e = ext0[i];
+					down |= i > 0 && !(ext0[i - 1] < e);
+					sk[i] = static_cast<uint32_t>(e);
 					sv[i] = static_cast<uint32_t>(i);
 				}
+				ascending = __ballot(down) == 0;
 			}
 			if (lane == 0)
 			{
 				a.groupPass[g] = pass ? 1 : 0;
 				a.groupCount[g] = 0;
 			}
-			segPushWave(sb, sb.big[0], lane == 0 && extSorted, static_cast<int32_t>(hb), static_cast<int32_t>(hb + n),
-						exsort::depthLimit(static_cast<uint32_t>(n)), 0);
+			segPushWave(sb, sb.big[0], lane == 0 && extSorted && !ascending, static_cast<int32_t>(hb),
+						static_cast<int32_t>(hb + n), exsort::depthLimit(static_cast<uint32_t>(n)), 0);
 		}
 	}
 }
