@@ -56,9 +56,12 @@ __global__ void tieFlagsKernel(const int64_t* qOff, int32_t nq, const uint64_t* 
 
 // Rank keys (first sorted position of the key in its query) in emission
 // order, for the queries that hold a flagged position (dup: exclusive prefix
-// of the flags); the replay reads no other query's.
+// of the flags); the replay reads no other query's.  With hcur set, equal
+// keys are equal curPos inside a group (no group start between them) and
+// skey is not read.
 __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, const uint32_t* sval,
-							   const int64_t* dup, uint32_t* rkey, int64_t n)
+							   const int64_t* dup, uint32_t* rkey, int64_t n, const int32_t* hcur,
+							   const int32_t* groupFlag)
 {
 	// rank keys are scattered inside the query's range: one XCD per range
 	const int64_t first = static_cast<int64_t>(xcdBlockIndex()) * blockDim.x * kPosPerThread + threadIdx.x;
@@ -71,9 +74,17 @@ __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* 
 		q = stepSeg(qOff, nq, q, i);
 		const int64_t b = qOff[q], e = qOff[q + 1];
 		if (dup[e] == dup[b]) continue;
-		const uint64_t k = skey[i];
 		int64_t f = i;
-		while (f > b && skey[f - 1] == k) --f;
+		if (hcur)
+		{
+			const int32_t c = hcur[i];
+			while (f > b && !groupFlag[f] && hcur[f - 1] == c) --f;
+		}
+		else
+		{
+			const uint64_t k = skey[i];
+			while (f > b && skey[f - 1] == k) --f;
+		}
 		rkey[b + sval[i]] = static_cast<uint32_t>(f - b);
 	}
 }
@@ -98,10 +109,11 @@ inline dim3 gridForPos(int64_t n, int tpb)
 // Equal extIds keep their emission order, which is the stable order.  The
 // join writes every match's emission index, which is its position in the
 // query's emission order, so the values are the positions themselves.  With
-// hcur set the kernel also writes the chain's hits in S order: curPos, the
-// match's extPos (mext in emission order) and the group-start flag (the first
-// slot of an extId; the first claim of a bin sets bit 31 of its histogram
-// word and passes it on in bit 31 of the leader's base).
+// hcur set the kernel writes the chain's hits in S order instead of the
+// keys: curPos, the match's extPos (mext in emission order) and the
+// group-start flag (the first slot of an extId; the first claim of a bin sets
+// bit 31 of its histogram word and passes it on in bit 31 of the leader's
+// base); skey is then written at the group starts only.
 constexpr int kMsortThreads = 1024;
 constexpr int kMsortWaves = kMsortThreads / FH_WAVE;
 constexpr int kMsortRows = 4;
@@ -236,13 +248,19 @@ __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int6
 			const uint32_t lw = lead[r * kMsortThreads + wv * FH_WAVE + ld];
 			const int rank = __popcll(same[r] & lt);
 			const int64_t d = b0 + (lw & ~kClaimed) + rank;
-			skey[d] = key[r];
 			sval[d] = static_cast<uint32_t>(t0 + (wv * kMsortRows + r) * FH_WAVE + lane);
 			if (withHits)
 			{
+				// the hits carry the keys: skey only at the group starts
+				const bool start = (lw & kClaimed) && rank == 0;
 				hits.hcur[d] = static_cast<int32_t>(static_cast<uint32_t>(key[r]));
 				hits.hext[d] = ext[r];
-				hits.groupFlag[d] = (lw & kClaimed) && rank == 0 ? 1 : 0;
+				hits.groupFlag[d] = start ? 1 : 0;
+				if (start) skey[d] = key[r];
+			}
+			else
+			{
+				skey[d] = key[r];
 			}
 		}
 		ldsBarrier();  // lead[] is rewritten by the next tile
@@ -315,7 +333,7 @@ hipError_t launchRankKeys(const MsortArgs& a, const int64_t* dup, int64_t nMatch
 	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
 	const int tpb = 256;
 	hipLaunchKernelGGL(rankKeysKernel, gridForPos(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.sval,
-					   dup, a.rkey, nMatches);
+					   dup, a.rkey, nMatches, a.hits.hcur, a.hits.groupFlag);
 	return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ struct MsortArgs
 	uint32_t* rkey;        // emission-order rank keys (filled here)
 	int32_t* dupFlag;      // per sorted position: a neighbouring position holds the same key
 	const int32_t* order;  // counting sort: queries by descending match count (null: in order)
-	MsortHits hits;        // counting sort only
+	MsortHits hits;        // counting sort (and then the rank keys) only
 };
 
 hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t* bytes);
@@ -43,7 +43,8 @@ hipError_t launchMatchSortStable(const MsortArgs& a, int64_t nMatches, int idBit
 // same key.
 hipError_t launchTieFlags(const MsortArgs& a, int64_t nMatches, hipStream_t s);
 // Rank keys (emission order) of the queries that hold a flagged position;
-// dup is the exclusive prefix of the flags (nMatches + 1 words).
+// dup is the exclusive prefix of the flags (nMatches + 1 words).  Keys from
+// a.hits (curPos and group starts) when a.hits.hcur is set, else from skey.
 hipError_t launchRankKeys(const MsortArgs& a, const int64_t* dup, int64_t nMatches, hipStream_t s);
 
 } // namespace fh

@@ -832,10 +832,11 @@ bool runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches, bool replay = t
 	void* temp = counting ? nullptr : ensure(c, L->bMsortTemp, tempBytes + 256);
 	HIPCHECK(hipEventRecord(L->ev[0], L->stream));
 	// stable order: the counting sort reads the keys twice (histogram, then
-	// to place them) and writes keys and values once (28 bytes per match),
-	// with the hits extPos read and curPos, extPos and group flag written (16
-	// more); the radix fallback makes 2 passes x 24 bytes
-	const double perMatch = counting ? (hits ? 44.0 : 28.0) : 48.0;
+	// to place them) and writes keys and values once (28 bytes per match);
+	// with the hits it reads extPos and writes curPos, extPos, group flag and
+	// value instead of key and value (36); the radix fallback makes 2 passes
+	// x 24 bytes
+	const double perMatch = counting ? (hits ? 36.0 : 28.0) : 48.0;
 	const int tr = L->timer.begin(FLYEHIP_K_RADIX, L->stream, perMatch * static_cast<double>(nMatches));
 	HIPCHECK(launchMatchSortStable(ms, nMatches, bits, nBins, temp, tempBytes, L->stream));
 	L->timer.end(tr, L->stream);
@@ -1103,6 +1104,12 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 		ra.queue = queues + 15;
 		int32_t* groupRel = ensureT<int32_t>(c, L->bGroupRel, static_cast<size_t>(nGroups) + 1);
 		MsortArgs ms = matchSortArgs(L, ja, nMatches);
+		if (hitsReady)
+		{
+			// the counting sort wrote skey at the group starts only
+			ms.hits.hcur = hcur;
+			ms.hits.groupFlag = gflag;
+		}
 		// the tie flags start cleared: the relevance pass sets those of the
 		// tied hits of the groups it marks
 		zero32(L, ms.dupFlag, nMatches);
