@@ -72,14 +72,17 @@ __device__ inline unsigned long long laneMaskLt()
 // smaller part bounds the pending parts: a part pushed while partitioning a
 // range of S elements leaves a current range of at most S / 2, and only
 // ranges above exsort::kThreshold (16) are partitioned, so from kLaneMax <= 63
-// elements at most two parts are pending.  Tile positions and budgets fit
-// 11 + 11 + 10 bits.
+// elements at most two parts are pending.  A pending part packs its tile
+// position (16 bits), its length (6 bits: a part is pushed only while the
+// budget lasts, when the range holds at most kLaneMax elements) and its
+// budget (10 bits).
 template <class K, int kLaneMax>
 __device__ inline void laneIntroLoop(const exsort::Span<K>& s, int first, int last, int depth)
 {
 	static_assert(kLaneMax < 4 * exsort::kThreshold, "two pending parts at most");
+	static_assert(kLaneMax < 64, "part lengths are 6-bit");
 	auto pack = [](int f, int l, int d) -> uint32_t {
-		return static_cast<uint32_t>(f) | (static_cast<uint32_t>(l) << 11) | (static_cast<uint32_t>(d) << 22);
+		return static_cast<uint32_t>(f) | (static_cast<uint32_t>(l - f) << 16) | (static_cast<uint32_t>(d) << 22);
 	};
 	uint32_t st0 = 0, st1 = 0, st2 = 0, st3 = 0;
 	int top = 0;
@@ -111,8 +114,8 @@ __device__ inline void laneIntroLoop(const exsort::Span<K>& s, int first, int la
 			}
 		}
 		if (top == 0) break;
-		f = static_cast<int>(st0 & 0x7ffu);
-		l = static_cast<int>((st0 >> 11) & 0x7ffu);
+		f = static_cast<int>(st0 & 0xffffu);
+		l = f + static_cast<int>((st0 >> 16) & 0x3fu);
 		d = static_cast<int>(st0 >> 22);
 		st0 = st1;
 		st1 = st2;
