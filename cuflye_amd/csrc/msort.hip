@@ -58,10 +58,11 @@ __global__ void tieFlagsKernel(const int64_t* qOff, int32_t nq, const uint64_t* 
 // order, for the queries that hold a flagged position (dup: exclusive prefix
 // of the flags); the replay reads no other query's.  With hcur set, equal
 // keys are equal curPos inside a group (no group start between them) and
-// skey is not read.
+// skey is not read.  With ival set, the emission indices of those queries
+// (the replay's values) are written too.
 __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* skey, const uint32_t* sval,
 							   const int64_t* dup, uint32_t* rkey, int64_t n, const int32_t* hcur,
-							   const int32_t* groupFlag)
+							   const int32_t* groupFlag, uint32_t* ival)
 {
 	// rank keys are scattered inside the query's range: one XCD per range
 	const int64_t first = static_cast<int64_t>(xcdBlockIndex()) * blockDim.x * kPosPerThread + threadIdx.x;
@@ -86,6 +87,8 @@ __global__ void rankKeysKernel(const int64_t* qOff, int32_t nq, const uint64_t* 
 			while (f > b && skey[f - 1] == k) --f;
 		}
 		rkey[b + sval[i]] = static_cast<uint32_t>(f - b);
+		// the emission indices are the identity: written in place, coalesced
+		if (ival) ival[i] = static_cast<uint32_t>(i - b);
 	}
 }
 
@@ -333,7 +336,7 @@ hipError_t launchRankKeys(const MsortArgs& a, const int64_t* dup, int64_t nMatch
 	if (nMatches == 0 || a.nQueries == 0) return hipSuccess;
 	const int tpb = 256;
 	hipLaunchKernelGGL(rankKeysKernel, gridForPos(nMatches, tpb), dim3(tpb), 0, s, a.qOff, a.nQueries, a.skey, a.sval,
-					   dup, a.rkey, nMatches, a.hits.hcur, a.hits.groupFlag);
+					   dup, a.rkey, nMatches, a.hits.hcur, a.hits.groupFlag, a.ival);
 	return hipGetLastError();
 }
 

@@ -29,6 +29,8 @@ struct MsortArgs
 	int32_t* dupFlag;      // per sorted position: a neighbouring position holds the same key
 	const int32_t* order;  // counting sort: queries by descending match count (null: in order)
 	MsortHits hits;        // counting sort (and then the rank keys) only
+	uint32_t* ival = nullptr;  // set: the rank keys also write the emission indices
+							   // (identity, in emission order) of the queries they cover
 };
 
 hipError_t msortTempBytes(int64_t nMatches, int32_t nQueries, int idBits, size_t* bytes);

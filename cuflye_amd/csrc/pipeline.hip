@@ -636,7 +636,7 @@ __global__ void __launch_bounds__(256) joinFillKernel(JoinArgs a)
 				const int64_t o = obL + j - ((selfL >= 0 && j > selfL) ? 1 : 0);
 				a.mkey[o] = (static_cast<uint64_t>(id) << 32) | static_cast<uint32_t>(posL);
 				a.mext[o] = p;
-				a.mval[o] = static_cast<uint32_t>(o - qbL);
+				if (a.mval) a.mval[o] = static_cast<uint32_t>(o - qbL);
 			}
 		}
 	}

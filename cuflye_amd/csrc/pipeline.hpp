@@ -102,7 +102,8 @@ struct JoinArgs
 	const int64_t* filtOff;    // exclusive scan of winFilt (nWin + 1)
 	uint64_t* mkey;            // (extIid << 32) | curPos, emission order
 	int32_t* mext;             // extPos, emission order
-	uint32_t* mval;            // emission index within the query (the stable sort's payload)
+	uint32_t* mval;            // emission index within the query (the stable sort's payload;
+							   // null: the fill skips it)
 	const int64_t* qMatchOff;  // per query match offsets (nQueries + 1)
 	int32_t* filtered;         // filtered positions
 };

@@ -597,7 +597,9 @@ struct BatchOut
 };
 
 // Join for a batch of queries; leaves matches (emission order) in bMKey/bMExt.
-BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& qs, JoinArgs& ja)
+// withVal: the fill writes every match's emission index (mval); without it
+// the caller fills mval where it needs it (the tie replay's queries).
+BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& qs, JoinArgs& ja, bool withVal = true)
 {
 	flyehip_ctx* c = L->c;
 	const int32_t nq = static_cast<int32_t>(qs.size());
@@ -658,9 +660,13 @@ BatchOut runJoin(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& 
 	ja.qMatchOff = qMatchOff;
 	ja.filtered = ensureT<int32_t>(c, L->bFiltered, static_cast<size_t>(bo.filtered) + 1);
 	HIPCHECK(hipEventRecord(L->ev[6], L->stream));
+	// per match: index entry read, key + extPos (+ emission index) written
 	const int tf = L->timer.begin(FLYEHIP_K_JOIN_FILL, L->stream,
-								  seqBytes + 8.0 * static_cast<double>(nWin) + 24.0 * static_cast<double>(bo.matches));
-	HIPCHECK(launchJoinFill(ja, L->stream));
+								  seqBytes + 8.0 * static_cast<double>(nWin) +
+									  (withVal ? 24.0 : 20.0) * static_cast<double>(bo.matches));
+	JoinArgs fa = ja;
+	if (!withVal) fa.mval = nullptr;
+	HIPCHECK(launchJoinFill(fa, L->stream));
 	L->timer.end(tf, L->stream);
 	HIPCHECK(hipEventRecord(L->ev[7], L->stream));
 	HIPCHECK(hipEventSynchronize(L->ev[7]));
@@ -1106,9 +1112,11 @@ BatchOut runChain(Lane* L, const JoinArgs& ja, int64_t nMatches, const std::vect
 		MsortArgs ms = matchSortArgs(L, ja, nMatches);
 		if (hitsReady)
 		{
-			// the counting sort wrote skey at the group starts only
+			// the counting sort wrote skey at the group starts only, and the
+			// join left the emission indices to the rank keys
 			ms.hits.hcur = hcur;
 			ms.hits.groupFlag = gflag;
+			ms.ival = ja.mval;
 		}
 		// the tie flags start cleared: the relevance pass sets those of the
 		// tied hits of the groups it marks
@@ -1312,7 +1320,10 @@ void runBatch(Lane* L, const PackedSeqs& seqs, const std::vector<JoinQuery>& all
 	std::vector<int32_t> lens;
 	for (auto& q : qs) lens.push_back(q.len);
 	JoinArgs ja;
-	BatchOut jo = runJoin(L, seqs, qs, ja);
+	// with the counting sort the emission indices are the positions: the
+	// fill skips them and the tie replay writes its queries'
+	const bool counting = msortCounting(2 * std::max(1, c->nReads));
+	BatchOut jo = runJoin(L, seqs, qs, ja, !counting);
 	sync(L);
 	const bool hitsReady = runMatchSort(L, ja, jo.matches, false, true);
 	BatchOut co = runChain(L, ja, jo.matches, ids, lens, p, nullptr, nullptr, true, hitsReady);
