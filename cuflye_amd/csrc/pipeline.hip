@@ -11,6 +11,10 @@
 //   bucket entries u64 (internal id << 32 | position), bucket = contiguous run
 //   join windows   count i32 + filtered i32 + slot/flip u32 per query window
 //   matches        key u64 (internal extId << 32 | curPos) + extPos i32
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
+
 #include "pipeline.hpp"
 #include "dev_common.hpp"
 
@@ -782,9 +786,31 @@ inline dim3 gridFor(int64_t n, int tpb) { return dim3(static_cast<unsigned>((n +
 
 } // namespace
 
+// rocPRIM's single-pass (decoupled look-back) scan reads the input once and
+// writes the output once; the three-kernel scan above reads it twice.
+struct WidenI32
+{
+	__host__ __device__ int64_t operator()(int32_t x) const { return x; }
+};
+
+size_t lookbackScanBytes(int64_t n)
+{
+	size_t bytes = 0;
+	if (n <= 0) return 0;
+	auto it = rocprim::make_transform_iterator(static_cast<const int32_t*>(nullptr), WidenI32{});
+	if (rocprim::exclusive_scan(nullptr, bytes, it, static_cast<int64_t*>(nullptr), int64_t{0},
+								static_cast<size_t>(n), rocprim::plus<int64_t>()) != hipSuccess)
+		return 0;
+	return bytes;
+}
+
+__global__ void scanTotalKernel(const int32_t* in, int64_t* out, int64_t n) { out[n] = out[n - 1] + in[n - 1]; }
+
 size_t scanTempWords(int64_t n)
 {
-	return static_cast<size_t>((n + kScanTile - 1) / kScanTile + 1);
+	const size_t three = static_cast<size_t>((n + kScanTile - 1) / kScanTile + 1);
+	const size_t look = (lookbackScanBytes(n) + 7) / 8 + 1;
+	return three > look ? three : look;
 }
 
 hipError_t exclusiveScan(const int32_t* in, int64_t* out, int64_t n, int64_t* tmp, hipStream_t s)
@@ -793,6 +819,16 @@ hipError_t exclusiveScan(const int32_t* in, int64_t* out, int64_t n, int64_t* tm
 	if (nb == 0)
 	{
 		return hipMemsetAsync(out, 0, sizeof(int64_t), s);
+	}
+	size_t bytes = lookbackScanBytes(n);
+	if (bytes > 0)
+	{
+		auto it = rocprim::make_transform_iterator(in, WidenI32{});
+		hipError_t e = rocprim::exclusive_scan(static_cast<void*>(tmp), bytes, it, out, int64_t{0},
+											   static_cast<size_t>(n), rocprim::plus<int64_t>(), s);
+		if (e != hipSuccess) return e;
+		hipLaunchKernelGGL(scanTotalKernel, dim3(1), dim3(1), 0, s, in, out, n);
+		return hipGetLastError();
 	}
 	hipLaunchKernelGGL(scanReduceKernel, dim3(static_cast<unsigned>(nb)), dim3(kScanNT), 0, s, in, n, tmp);
 	hipLaunchKernelGGL(scanBlockSumsKernel, dim3(1), dim3(1024), 0, s, tmp, nb);
