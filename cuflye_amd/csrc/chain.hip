@@ -116,6 +116,13 @@ __device__ inline const int32_t* groupHits(const int32_t* sorted, const int32_t*
 	return (extSorted ? sorted : inOrder) + hb;
 }
 
+// Whether a group's hits are read from the extPos-sorted copy: sorted by
+// extPos, unless their extPos already ascend (inPlace, set by the prefilter).
+__device__ inline bool gatheredHits(const GroupDesc& gd, const QueryDesc& qd)
+{
+	return gd.extLen > qd.curLen && !gd.inPlace;
+}
+
 __device__ inline int pullGroup(int32_t* queue)
 {
 	int g = 0;
@@ -224,6 +231,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 			{
 				a.groupPass[g] = pass ? 1 : 0;
 				a.groupCount[g] = 0;
+				a.groups[g].inPlace = extSorted && ascending ? 1 : 0;
 			}
 			segPushWave(sb, sb.big[0], lane == 0 && extSorted && !ascending, static_cast<int32_t>(hb),
 						static_cast<int32_t>(hb + n), exsort::depthLimit(static_cast<uint32_t>(n)), 0);
@@ -449,8 +457,9 @@ __global__ void __launch_bounds__(FH_WAVE) chainGatherKernel(ChainArgs a)
 		const int32_t* ext0 = a.ext + hb;
 		int32_t* hc = a.hcur + hb;
 		int32_t* he = a.hext + hb;
-		// groups in curPos order are read in place (groupHits)
-		if (gd.extLen > qd.curLen)
+		// groups in curPos order, and extPos-sorted groups already in extPos
+		// order, are read in place (groupHits)
+		if (gatheredHits(gd, qd))
 		{
 			const uint32_t* sv = a.sortVal + hb;
 #pragma unroll 4
@@ -647,7 +656,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 			g = gg;
 			n = gd.n;
 			i = 1;
-			extSorted = gd.extLen > a.queries[gd.query].curLen;
+			extSorted = gatheredHits(gd, a.queries[gd.query]);
 			hc = groupHits(a.hcur, a.cur, gd.hitStart, extSorted);
 			he = groupHits(a.hext, a.ext, gd.hitStart, extSorted);
 			sc = a.score + gd.hitStart;
@@ -1014,7 +1023,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainWalkKernel(ChainArgs a, int32_t 
 		// branch, not an exec mask around every step of lane 0's walk
 		const int n = __builtin_amdgcn_readfirstlane(gd.n);
 		const int64_t hb = gd.hitStart;
-		const bool extSorted = gd.extLen > qd.curLen;
+		const bool extSorted = gatheredHits(gd, qd);
 		const int32_t* hc = groupHits(a.hcur, a.cur, hb, extSorted);
 		const int32_t* he = groupHits(a.hext, a.ext, hb, extSorted);
 		const int32_t* sc = a.score + hb;
@@ -1152,7 +1161,7 @@ __global__ void __launch_bounds__(FH_WAVE) chainClaimKernel(ChainArgs a, int32_t
 		const QueryDesc qd = a.queries[gd.query];
 		const int n = __builtin_amdgcn_readfirstlane(gd.n);
 		const int64_t hb = gd.hitStart;
-		const bool extSorted = gd.extLen > qd.curLen;
+		const bool extSorted = gatheredHits(gd, qd);
 		const int32_t* hc = groupHits(a.hcur, a.cur, hb, extSorted);
 		const int32_t* he = groupHits(a.hext, a.ext, hb, extSorted);
 		const int32_t* sc = a.score + hb;

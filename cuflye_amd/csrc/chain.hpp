@@ -28,7 +28,8 @@ struct GroupDesc
 	int32_t n;
 	int32_t query;       // index into the batch's QueryDesc array
 	int32_t extLen;
-	int32_t reserved;
+	int32_t inPlace;     // set by the prefilter: sorted by extPos, but the extPos already
+						 // ascend strictly, so the hits are read in place (no sort, no gather)
 };
 
 // A chain proposal (an OverlapRange before and after the filters).
@@ -46,7 +47,7 @@ struct Proposal
 struct ChainArgs
 {
 	flyehip_overlap_params_t params;
-	const GroupDesc* groups;
+	GroupDesc* groups;         // the prefilter sets inPlace
 	const QueryDesc* queries;
 	const int32_t* order;      // processing order: size-descending
 	int32_t nGroups;

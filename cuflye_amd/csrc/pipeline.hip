@@ -713,7 +713,7 @@ __global__ void buildGroupsKernel(const int64_t* qMatchOff, int32_t nQueries, co
 	d.n = 0;  // set by groupSizeKernel from the next group's start
 	d.query = q;
 	d.extLen = targetLen[eid >> 1];
-	d.reserved = 0;
+	d.inPlace = 0;
 	groups[g] = d;
 }
 
