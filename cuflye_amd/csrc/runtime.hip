@@ -970,8 +970,12 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 		ca.queue = queues + 6;
 		HIPCHECK(launchChainGather(ca, blocks, L->stream));
 		ca.queue = queues + 7;
+		// the deferred groups are the few largest (they hold the ranges that
+		// exhausted the sort's depth budget): a whole wave per group's
+		// lookback finishes them sooner than the batch's tile width (the rows
+		// do not depend on it)
 		const int td2 = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * deferHits);
-		HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, L->stream));
+		HIPCHECK(launchChainDp(ca, std::max(c->dpLanes, 64), blocks, L->stream));
 		L->timer.end(td2, L->stream);
 		ca.passOrder = passOrder;
 		ca.nPass = nPassAll;
