@@ -135,9 +135,10 @@ __device__ inline int pullGroup(int32_t* queue)
 // Prefilters (replay_overlap_chains.py:245-270).  Each wave takes 64
 // consecutive groups of the processing order: a group whose size is below the
 // unique-curPos bound (unique curPos <= n) fails without reading its hits; the
-// others are examined by the whole wave one after another.  Passing groups
-// with extLen > curLen get their extPos sort keys and are pushed to the
-// segmented sort.
+// others are examined by the whole wave one after another (with passKnown,
+// groupPass already holds the result and only passing groups are visited).
+// Passing groups with extLen > curLen get their extPos sort keys and are
+// pushed to the segmented sort.
 __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, SegSortBufs sb)
 {
 	const int lane = laneId();
@@ -155,8 +156,16 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 		if (qi < a.nGroups)
 		{
 			myG = a.order[qi];
-			const int n = a.groups[myG].n;
-			cand = n >= 2 && !(static_cast<float>(n) < minUniq);
+			// with passKnown only the passing groups have work left here
+			if (a.passKnown)
+			{
+				cand = a.groupPass[myG] != 0;
+			}
+			else
+			{
+				const int n = a.groups[myG].n;
+				cand = n >= 2 && !(static_cast<float>(n) < minUniq);
+			}
 			if (!cand)
 			{
 				a.groupPass[myG] = 0;
@@ -164,6 +173,10 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 			}
 		}
 		unsigned long long todo = __ballot(cand);
+		// the extPos sorts of the 64 groups, pushed together after them (one
+		// atomic per list and wave, not per group): lane l holds group l's
+		bool doPush = false;
+		int32_t pushF = 0, pushL = 0;
 		while (todo)
 		{
 			const int l = __ffsll(static_cast<long long>(todo)) - 1;
@@ -233,9 +246,15 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 				a.groupCount[g] = 0;
 				a.groups[g].inPlace = extSorted && ascending ? 1 : 0;
 			}
-			segPushWave(sb, sb.big[0], lane == 0 && extSorted && !ascending, static_cast<int32_t>(hb),
-						static_cast<int32_t>(hb + n), exsort::depthLimit(static_cast<uint32_t>(n)), 0);
+			if (lane == l)
+			{
+				doPush = extSorted && !ascending;
+				pushF = static_cast<int32_t>(hb);
+				pushL = static_cast<int32_t>(hb + n);
+			}
 		}
+		segPushWave(sb, sb.big[0], doPush, pushF, pushL, doPush ? exsort::depthLimit(static_cast<uint32_t>(pushL - pushF)) : 0,
+					0);
 	}
 }
 
