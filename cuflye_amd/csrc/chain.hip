@@ -662,8 +662,15 @@ __global__ void __launch_bounds__(FH_WAVE) chainDpKernel(ChainArgs a)
 		if (g >= 0 && i >= n) g = -1;
 		while (need)
 		{
-			int qi = 0;
-			if (sl == 0) qi = atomicAdd(a.queue, 1);
+			// one atomic per wave for all the subgroups that need a group
+			// (lanes of one atomic instruction to one address are not
+			// combined: each would be an operation of its own)
+			const unsigned long long leaders = __ballot(sl == 0);
+			const int first = __ffsll(static_cast<long long>(leaders)) - 1;
+			int base = 0;
+			if (laneId() == first) base = atomicAdd(a.queue, __popcll(leaders));
+			base = __shfl(base, first, FH_WAVE);
+			int qi = base + __popcll(leaders & laneMaskLt());
 			qi = __shfl(qi, sub * G, FH_WAVE);
 			if (qi >= a.nPass)
 			{

@@ -961,6 +961,16 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	const int td = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * readyHits);
 	HIPCHECK(launchChainDp(ca, c->dpLanes, blocks, L->stream));
 	L->timer.end(td, L->stream);
+	// backtrack .. selection without the score sort for every passing group
+	// whose walk cannot depend on the order of equal scores; the flagged
+	// groups (claimFlag) take the exact score sort and the ordered walk.  The
+	// ready groups' claim runs while the deferred groups' heap sorts finish.
+	int32_t* claimFlag = ensureT<int32_t>(c, L->bClaimFlag, static_cast<size_t>(ng) + 1);
+	ca.queue = queues + 10;
+	// claim: score and backtrack read, 8 bytes per passing hit
+	const int tcl = L->timer.begin(FLYEHIP_K_CLAIM, L->stream, 8.0 * readyHits);
+	HIPCHECK(launchChainClaim(ca, claimFlag, blocks, L->stream));
+	L->timer.end(tcl, L->stream);
 	if (deferOrder)
 	{
 		exactPending = false;
@@ -977,19 +987,14 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 		const int td2 = L->timer.begin(FLYEHIP_K_DP, L->stream, 16.0 * deferHits);
 		HIPCHECK(launchChainDp(ca, std::max(c->dpLanes, 64), blocks, L->stream));
 		L->timer.end(td2, L->stream);
+		ca.queue = queues + 12;
+		const int tcl2 = L->timer.begin(FLYEHIP_K_CLAIM, L->stream, 8.0 * deferHits);
+		HIPCHECK(launchChainClaim(ca, claimFlag, blocks, L->stream));
+		L->timer.end(tcl2, L->stream);
 		ca.passOrder = passOrder;
 		ca.nPass = nPassAll;
 	}
 	HIPCHECK(hipEventRecord(L->ev[4], L->stream));
-	// backtrack .. selection without the score sort for every passing group
-	// whose walk cannot depend on the order of equal scores; the flagged
-	// groups (claimFlag) take the exact score sort and the ordered walk
-	int32_t* claimFlag = ensureT<int32_t>(c, L->bClaimFlag, static_cast<size_t>(ng) + 1);
-	ca.queue = queues + 10;
-	// claim: score and backtrack read, 8 bytes per passing hit
-	const int tcl = L->timer.begin(FLYEHIP_K_CLAIM, L->stream, 8.0 * passHits);
-	HIPCHECK(launchChainClaim(ca, claimFlag, blocks, L->stream));
-	L->timer.end(tcl, L->stream);
 	int32_t* flagOrder = ensureT<int32_t>(c, L->bFlagOrder, static_cast<size_t>(ng) + 1);
 	{
 		ChainArgs fa = ca;

@@ -488,7 +488,7 @@ __global__ void segShadowKernel(SegSortBufs b, int64_t v0, int64_t v1)
 // the exact heap sort (d = 1); with all keys distinct every sort agrees.  With
 // relevantBit set, only a run of equal keys holding two marked values counts
 // (the order of the other elements of a run is immaterial to the caller).
-__global__ void segVerifyKernel(SegSortBufs b, int64_t nVerify)
+__global__ void segVerifyKernel(SegSortBufs b, int64_t nVerify, int heapFlags)
 {
 	const int64_t i = waveIndex();
 	if (i >= nVerify) return;
@@ -517,8 +517,16 @@ __global__ void segVerifyKernel(SegSortBufs b, int64_t nVerify)
 	{
 		b.verify[i].d = any ? 1 : 0;
 		if (any)
+		{
 			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegExactElems]),
 					  static_cast<unsigned long long>(w.l - w.f));
+			// the heap sort's class, as segHeapLdsKernel / segHeapGlobalKernel
+			// decide it from the shadow copy (the same keys, here sorted: the
+			// span is last minus first)
+			const int n = w.l - w.f;
+			const int cls = heapClass(n, heapFits32(b.key[w.l - 1] - b.key[w.f], heapShift(n)), heapFlags);
+			atomicAdd(reinterpret_cast<unsigned long long*>(&b.counters[kSegHeapCls + cls]), 1ULL);
+		}
 	}
 }
 
@@ -554,15 +562,24 @@ hipError_t download(const SegSortBufs& b, hipStream_t s)
 // many streams as the runtime's default hardware queues, so no stream shares
 // a queue, whose in-order packets would make one lane's heap sorts wait for
 // another's.)
-hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int onlyFlagged, const uint32_t* srcK,
-					 const uint32_t* srcV, uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s,
-					 int* firstSlot, bool join = true)
+int heapFlags()
 {
-	*firstSlot = -1;
 	static const bool bigLds =
 		hipFuncSetAttribute(reinterpret_cast<const void*>(segHeapLdsKernel), hipFuncAttributeMaxDynamicSharedMemorySize,
 							heapLdsBytes(kHeapClasses - 1)) == hipSuccess;
-	const int flags = bigLds ? 0 : kHeapNoBig;
+	return bigLds ? 0 : kHeapNoBig;
+}
+
+// clsCount (may be null: launch every class): ranges per class (3 LDS
+// classes, then global memory); a class without ranges is not launched, so
+// its blocks do not wait for LDS the DP holds.
+hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int onlyFlagged, const uint32_t* srcK,
+					 const uint32_t* srcV, uint32_t* dstK, uint32_t* dstV, uint32_t* gk, uint32_t* gv, hipStream_t s,
+					 int* firstSlot, bool join = true, const int64_t* clsCount = nullptr)
+{
+	*firstSlot = -1;
+	const int flags = heapFlags();
+	const bool bigLds = flags == 0;
 	if (n <= 0) return hipSuccess;
 	if (n > 0x7fffffffLL) return hipErrorInvalidValue;
 	hipError_t e = hipSuccess;
@@ -573,11 +590,15 @@ hipError_t heapTiers(const SegSortBufs& b, const SegRange* list, int64_t n, int 
 		if ((e = hipStreamWaitEvent(b.side, b.forkEv, 0)) != hipSuccess) return e;
 		hs = b.side;
 	}
-	hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, hs, list, n, onlyFlagged, gk, gv, dstV,
-					   flags);
-	if ((e = hipGetLastError()) != hipSuccess) return e;
+	if (!clsCount || clsCount[kHeapClasses] > 0)
+	{
+		hipLaunchKernelGGL(segHeapGlobalKernel, dim3(blocksFor(n, 64)), dim3(64), 0, hs, list, n, onlyFlagged, gk, gv,
+						   dstV, flags);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
 	for (int cls = kHeapClasses - (bigLds ? 1 : 2); cls >= 0; --cls)
 	{
+		if (clsCount && clsCount[cls] == 0) continue;
 		const int slot = b.timer ? b.timer->begin(FLYEHIP_K_HEAP, hs) : -1;
 		if (*firstSlot < 0) *firstSlot = slot;
 		hipLaunchKernelGGL(segHeapLdsKernel, dim3(static_cast<unsigned>(n)), dim3(FH_WAVE),
@@ -752,18 +773,17 @@ hipError_t segSortRun(SegSortBufs& b, hipStream_t s, int64_t* tieArrays, bool* e
 	if (nVerify > 0)
 	{
 		const int tpb = FH_WAVE * kWavesPerBlock;
-		hipLaunchKernelGGL(segVerifyKernel, dim3(blocksFor(nVerify, kWavesPerBlock)), dim3(tpb), 0, s, b, nVerify);
+		hipLaunchKernelGGL(segVerifyKernel, dim3(blocksFor(nVerify, kWavesPerBlock)), dim3(tpb), 0, s, b, nVerify,
+						   heapFlags());
+		// the exact element count and the ranges per heap class are known
+		// once the verification has run
+		if ((e = download(b, s)) != hipSuccess) return e;
 		int slot = -1;
 		e = heapTiers(b, b.verify, nVerify, 1, b.shadowKey, b.shadowVal, nullptr, b.outVal, b.shadowKey, b.shadowVal, s,
-					  &slot, exactPending == nullptr);
+					  &slot, exactPending == nullptr, &b.hostCounters[kSegHeapCls]);
 		if (e != hipSuccess) return e;
 		if (exactPending) *exactPending = true;
-		if (b.timer)
-		{
-			// the exact element count is known once the verification has run
-			if ((e = download(b, s)) != hipSuccess) return e;
-			b.timer->addBytes(slot, 12.0 * static_cast<double>(b.hostCounters[kSegExactElems]));
-		}
+		if (b.timer) b.timer->addBytes(slot, 12.0 * static_cast<double>(b.hostCounters[kSegExactElems]));
 		if (std::getenv("FLYEHIP_SEGSORT_TRACE"))
 		{
 			SegRange* hv = new SegRange[static_cast<size_t>(nVerify)];

@@ -68,7 +68,8 @@ enum SegCounter
 	kSegError,        // set when a partition produced an impossible cut or swap
 	kSegNVerify,      // exact-mode ranges whose depth budget ran out
 	kSegExactElems,   // elements of the verify entries that needed the exact heap sort
-	kSegCounters = 12
+	kSegHeapCls,      // those entries per heap-sort class: 3 LDS classes, then global memory
+	kSegCounters = kSegHeapCls + 4
 };
 
 struct SegSortBufs
