@@ -102,8 +102,9 @@ inline dim3 gridForPos(int64_t n, int tpb)
 // bits of the key), one 1,024-thread block per query:
 //   1. a histogram of the extIds in LDS (LDS atomics);
 //   2. its exclusive scan: the first slot of every extId in S;
-//   3. the matches in emission order, 4,096 at a time (every wave 256
-//      consecutive ones as 4 rows of 64): every wave finds, for each lane,
+//   3. the matches in emission order, 6,144 at a time (every wave 384
+//      consecutive ones as 6 rows of 64; more rows spill registers): every
+//      wave finds, for each lane,
 //      the lanes of its row with the same extId (one ballot per id bit), so a
 //      lane knows its rank among them and each group has a leader; the
 //      waves' leaders claim their slots from the histogram in emission order
@@ -119,7 +120,7 @@ inline dim3 gridForPos(int64_t n, int tpb)
 // base); skey is then written at the group starts only.
 constexpr int kMsortThreads = 1024;
 constexpr int kMsortWaves = kMsortThreads / FH_WAVE;
-constexpr int kMsortRows = 4;
+constexpr int kMsortRows = 6;
 
 // Block barrier that orders LDS accesses only: global loads issued before it
 // stay in flight.
@@ -221,7 +222,7 @@ __global__ void __launch_bounds__(kMsortThreads) matchCountSortKernel(const int6
 		int nbin[kMsortRows];
 		load(t0 + kMsortRows * kMsortThreads, nkey, next, nbin);
 		// leaders claim slots in emission order: wave w holds the tile's
-		// elements [w * 256, (w + 1) * 256) as 4 rows of 64, so the waves take
+		// elements [w * 384, (w + 1) * 384) as 6 rows of 64, so the waves take
 		// turns in order and each claims its rows in order (LDS operations of
 		// one wave complete in order)
 		for (int w = 0; w < kMsortWaves; ++w)
