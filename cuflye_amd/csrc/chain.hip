@@ -459,35 +459,46 @@ __global__ void splitPassKernel(const int32_t* order, const int32_t* groupPass, 
 }
 
 // Group-order hit copies for the DP: hits in (extId, curPos) order, or in the
-// extPos order of the sort when extLen > curLen.
-__global__ void __launch_bounds__(FH_WAVE) chainGatherKernel(ChainArgs a)
+// extPos order of the sort when extLen > curLen.  The copies are cut into
+// chunks of kGatherChunk hits, one wave per chunk (a group's copy is a chain
+// of dependent loads per lane, so one wave per large group would leave it
+// alone for milliseconds): gatherChunksKernel counts every passing group's
+// chunks, the caller scans the counts, and chainGatherKernel finds each
+// chunk's group by binary search over the scan.
+constexpr int kGatherChunk = 4096;
+
+__global__ void gatherChunksKernel(ChainArgs a, int32_t* chunks)
+{
+	const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+	if (qi >= a.nPass) return;
+	const GroupDesc gd = a.groups[a.passOrder[qi]];
+	// groups in curPos order, and extPos-sorted groups already in extPos
+	// order, are read in place (groupHits)
+	chunks[qi] = gatheredHits(gd, a.queries[gd.query]) ? (gd.n + kGatherChunk - 1) / kGatherChunk : 0;
+}
+
+__global__ void __launch_bounds__(FH_WAVE) chainGatherKernel(ChainArgs a, const int64_t* chunkOff)
 {
 	const int lane = laneId();
-	for (;;)
+	const int64_t total = chunkOff[a.nPass];
+	for (int64_t c = blockIdx.x; c < total; c += gridDim.x)
 	{
-		const int qi = pullGroup(a.queue);
-		if (qi >= a.nPass) break;
-		const int g = a.passOrder[qi];
-		const GroupDesc gd = a.groups[g];
-		const QueryDesc qd = a.queries[gd.query];
-		const int n = gd.n;
+		const int e = findSeg(chunkOff, a.nPass, c);
+		const GroupDesc gd = a.groups[a.passOrder[e]];
+		const int i0 = static_cast<int>(c - chunkOff[e]) * kGatherChunk;
+		const int i1 = min(gd.n, i0 + kGatherChunk);
 		const int64_t hb = gd.hitStart;
 		const int32_t* cur0 = a.cur + hb;
 		const int32_t* ext0 = a.ext + hb;
 		int32_t* hc = a.hcur + hb;
 		int32_t* he = a.hext + hb;
-		// groups in curPos order, and extPos-sorted groups already in extPos
-		// order, are read in place (groupHits)
-		if (gatheredHits(gd, qd))
-		{
-			const uint32_t* sv = a.sortVal + hb;
+		const uint32_t* sv = a.sortVal + hb;
 #pragma unroll 4
-			for (int i = lane; i < n; i += FH_WAVE)
-			{
-				const uint32_t src = sv[i];
-				hc[i] = cur0[src];
-				he[i] = ext0[src];
-			}
+		for (int i = i0 + lane; i < i1; i += FH_WAVE)
+		{
+			const uint32_t src = sv[i];
+			hc[i] = cur0[src];
+			he[i] = ext0[src];
 		}
 	}
 }
@@ -1521,9 +1532,17 @@ hipError_t launchSplitPass(const ChainArgs& a, const int32_t* deferred, int32_t*
 	return hipGetLastError();
 }
 
-hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream)
+hipError_t launchGatherChunks(const ChainArgs& a, int32_t* chunks, hipStream_t stream)
 {
-	hipLaunchKernelGGL(chainGatherKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a);
+	if (a.nPass <= 0) return hipSuccess;
+	hipLaunchKernelGGL(gatherChunksKernel, dim3((a.nPass + 255) / 256), dim3(256), 0, stream, a, chunks);
+	return hipGetLastError();
+}
+
+hipError_t launchChainGather(const ChainArgs& a, const int64_t* chunkOff, int blocks, hipStream_t stream)
+{
+	if (a.nPass <= 0) return hipSuccess;
+	hipLaunchKernelGGL(chainGatherKernel, dim3(blocks), dim3(FH_WAVE), 0, stream, a, chunkOff);
 	return hipGetLastError();
 }
 

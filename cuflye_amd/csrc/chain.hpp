@@ -122,7 +122,11 @@ hipError_t launchDeferGroups(const SegRange* verify, int64_t nVerify, const Chai
 							 hipStream_t stream);
 hipError_t launchSplitPass(const ChainArgs& a, const int32_t* deferred, int32_t* flagReady, int32_t* flagDefer,
 						   hipStream_t stream);
-hipError_t launchChainGather(const ChainArgs& a, int blocks, hipStream_t stream);
+// group-order hit copies of passOrder[0, nPass): chunk counts per entry
+// (chunks), then, after the caller's exclusive scan of them (chunkOff, nPass
+// + 1 words), the copies
+hipError_t launchGatherChunks(const ChainArgs& a, int32_t* chunks, hipStream_t stream);
+hipError_t launchChainGather(const ChainArgs& a, const int64_t* chunkOff, int blocks, hipStream_t stream);
 hipError_t launchChainDp(const ChainArgs& a, int lanes, int blocks, hipStream_t stream);
 // writes the score-order keys and values of the groups passOrder[0, nPass)
 // into sortKey / sortVal and pushes their ranges to sb

@@ -177,7 +177,7 @@ struct Lane
 	DevBuf bFiltered, bQMatchOff, bQFiltOff, bQDesc, bGroups, bGroupCount, bRowStart, bRows;
 	DevBuf bCScore, bCBt, bCProps, bCOut, bCHCur, bCHExt, bCKey, bCVal, bCPosL, bCPosR;
 	DevBuf bGroupDefer, bFlagReady, bFlagDefer, bPosReady, bPosDefer, bReadyOrder, bDeferOrder;
-	DevBuf bClaimFlag, bFlagOrder, bGroupRel;
+	DevBuf bClaimFlag, bFlagOrder, bGroupRel, bGatherCnt, bGatherOff;
 	DevBuf bGroupPass, bOrder, bPassOrder, bBins, bSKey, bAVal, bRKey, bDupFlag, bDup, bMsortTemp;
 	DevBuf bSegBig0, bSegBig1, bSegLeaf, bSegHeap, bSegCounters, bSegChunkOff, bSegChunkRange, bSegChunkCnt;
 	DevBuf bSegChunkPre, bSegRangeM, bSegRangeLm, bSegTemp, bSegVerify, bSegShadowK, bSegShadowV;
@@ -855,6 +855,18 @@ bool runMatchSort(Lane* L, const JoinArgs& ja, int64_t nMatches, bool replay = t
 	return hits;
 }
 
+// Group-order hit copies of ca.passOrder[0, ca.nPass), one wave per chunk.
+void runGather(Lane* L, const ChainArgs& ca, int64_t* tmp, int blocks)
+{
+	flyehip_ctx* c = L->c;
+	const size_t np = static_cast<size_t>(std::max(ca.nPass, 0)) + 1;
+	int32_t* chunks = ensureT<int32_t>(c, L->bGatherCnt, np);
+	int64_t* chunkOff = ensureT<int64_t>(c, L->bGatherOff, np);
+	HIPCHECK(launchGatherChunks(ca, chunks, L->stream));
+	HIPCHECK(exclusiveScan(chunks, chunkOff, ca.nPass, tmp, L->stream));
+	HIPCHECK(launchChainGather(ca, chunkOff, blocks, L->stream));
+}
+
 // Order, prefilters + extPos sort, DP, score sort + walks over ca.nGroups
 // groups whose hits total nHits; leaves groupCount set.
 void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
@@ -952,8 +964,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	ca.passOrder = readyOrder;
 	ca.nPass = nReady;
 	ca.dpCells = reinterpret_cast<unsigned long long*>(nBig + 8);
-	ca.queue = queues + 1;
-	HIPCHECK(launchChainGather(ca, blocks, L->stream));
+	runGather(L, ca, tmp, blocks);
 	HIPCHECK(hipEventRecord(L->ev[3], L->stream));
 	ca.queue = queues + 2;
 	// DP: hits read (hcur, hext) and score / backtrack written, 16 bytes per
@@ -977,8 +988,7 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 		HIPCHECK(hipStreamWaitEvent(L->stream, L->sideEv[1], 0));
 		ca.passOrder = deferOrder;
 		ca.nPass = nPassAll - nReady;
-		ca.queue = queues + 6;
-		HIPCHECK(launchChainGather(ca, blocks, L->stream));
+		runGather(L, ca, tmp, blocks);
 		ca.queue = queues + 7;
 		// the deferred groups are the few largest (they hold the ranges that
 		// exhausted the sort's depth budget): a whole wave per group's
@@ -1266,7 +1276,7 @@ void releaseBatchBuffers(flyehip_ctx* c, Lane& l)
 					  &l.bSegShadowV, &l.bQueue, &l.bCounters, &l.bQRowCount, &l.bCurLen, &l.bDivOff, &l.bDivCount,
 					  &l.bDivOut, &l.bQueryIds, &l.bCand, &l.bExtWords, &l.bExtWordOff, &l.bExtLen, &l.bAln,
 					  &l.bAlnOff, &l.bRA, &l.bGroupDefer, &l.bFlagReady, &l.bFlagDefer, &l.bPosReady, &l.bPosDefer,
-					  &l.bReadyOrder, &l.bDeferOrder, &l.bClaimFlag, &l.bFlagOrder, &l.bGroupRel, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart, &l.bMinMask, &l.bQOrder})
+					  &l.bReadyOrder, &l.bDeferOrder, &l.bClaimFlag, &l.bFlagOrder, &l.bGroupRel, &l.bGatherCnt, &l.bGatherOff, &l.bRawBases, &l.bRawOff, &l.bPackErr, &l.bGroupStart, &l.bMinMask, &l.bQOrder})
 		release(c, *b);
 }
 
