@@ -501,10 +501,17 @@ __global__ void queryRowCountKernel(const GroupDesc* groups, const int32_t* grou
 	if (c) atomicAdd(&qCount[groups[g].query], c);
 }
 
+// Four words per thread: one 16-byte store where p is 16-byte aligned.
 __global__ void zeroKernel(int32_t* p, int64_t n)
 {
-	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (i < n) p[i] = 0;
+	const int64_t j = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+	if (j >= n) return;
+	if (j + 4 <= n && (reinterpret_cast<uintptr_t>(p) & 15) == 0)
+	{
+		*reinterpret_cast<int4*>(p + j) = make_int4(0, 0, 0, 0);
+		return;
+	}
+	for (int64_t k = j; k < n && k < j + 4; ++k) p[k] = 0;
 }
 
 __global__ void fillU64Kernel(uint64_t* p, int64_t n, uint64_t v)
@@ -525,7 +532,7 @@ dim3 grid(int64_t n, int tpb) { return dim3(static_cast<unsigned>(std::max<int64
 void zero32(Lane* L, int32_t* p, int64_t n)
 {
 	if (n <= 0) return;
-	hipLaunchKernelGGL(zeroKernel, grid(n, 256), dim3(256), 0, L->stream, p, n);
+	hipLaunchKernelGGL(zeroKernel, grid((n + 3) / 4, 256), dim3(256), 0, L->stream, p, n);
 	HIPCHECK(hipGetLastError());
 }
 
