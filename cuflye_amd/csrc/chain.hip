@@ -227,18 +227,22 @@ __global__ void __launch_bounds__(FH_WAVE) chainPrefilterKernel(ChainArgs a, Seg
 			bool ascending = true;
 			if (extSorted)
 			{
+				bool down = false;
+#pragma unroll 4
+				for (int i = lane + 1; i < n; i += FH_WAVE) down |= !(ext0[i - 1] < ext0[i]);
+				ascending = __ballot(down) == 0;
+			}
+			if (extSorted && !ascending)
+			{
+				// sort keys and positions only for the groups the sort takes
 				uint32_t* sk = a.sortKey + hb;
 				uint32_t* sv = a.sortVal + hb;
-				bool down = false;
 #pragma unroll 4
 				for (int i = lane; i < n; i += FH_WAVE)
 				{
-					const int32_t e = ext0[i];
-					down |= i > 0 && !(ext0[i - 1] < e);
-					sk[i] = static_cast<uint32_t>(e);
+					sk[i] = static_cast<uint32_t>(ext0[i]);
 					sv[i] = static_cast<uint32_t>(i);
 				}
-				ascending = __ballot(down) == 0;
 			}
 			if (lane == 0)
 			{
