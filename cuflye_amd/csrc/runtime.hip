@@ -910,7 +910,25 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 	HIPCHECK(segSortReset(sb, L->stream));
 	ca.queue = queues + 0;
 	const int tp = L->timer.begin(FLYEHIP_K_PREFILTER, L->stream, 8.0 * static_cast<double>(nHits));
-	HIPCHECK(launchChainPrefilter(ca, sb, blocks, L->stream));
+	if (ca.passKnown)
+	{
+		// the pass is known (chainRelevanceKernel): the passing groups in
+		// processing order first, then the prefilter visits only them
+		HIPCHECK(launchPassFlags(ca, flag, L->stream));
+		HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
+		HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream));
+		int64_t np = 0;
+		download(L, &np, pos + ng, sizeof(int64_t));
+		sync(L);
+		ChainArgs pa = ca;
+		pa.order = passOrder;
+		pa.nGroups = static_cast<int32_t>(np);
+		HIPCHECK(launchChainPrefilter(pa, sb, blocks, L->stream));
+	}
+	else
+	{
+		HIPCHECK(launchChainPrefilter(ca, sb, blocks, L->stream));
+	}
 	L->timer.end(tp, L->stream);
 	// the extPos sort leaves its exact heap sorts (libstdc++'s heapsort
 	// fallback, a dependent chain per range) running on the side stream: the
@@ -930,9 +948,12 @@ void runChainKernels(Lane* L, ChainArgs& ca, int64_t nHits)
 		}
 	} joinOnExit{L->stream, L->sideEv[1], &exactPending};
 	runSegSort(L, sb, nullptr, &exactPending);
-	HIPCHECK(launchPassFlags(ca, flag, L->stream));
-	HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
-	HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream));
+	if (!ca.passKnown)
+	{
+		HIPCHECK(launchPassFlags(ca, flag, L->stream));
+		HIPCHECK(exclusiveScan(flag, pos, ng, tmp, L->stream));
+		HIPCHECK(launchPassScatter(ca, flag, pos, passOrder, nBig, L->stream));
+	}
 	int32_t* readyOrder = passOrder;
 	int32_t* deferOrder = nullptr;
 	int64_t* posReady = nullptr;
