@@ -314,6 +314,17 @@ __global__ void __launch_bounds__(FH_WAVE) chainRelevanceKernel(ChainArgs a, int
 			const int64_t hb = gd.hitStart;
 			const int32_t* cur0 = a.cur + hb;
 			const int32_t* ext0 = a.ext + hb;
+			// curPos ascends in the group, so its span test needs no scan; a
+			// group that fails it neither passes nor has ties that matter
+			if (cur0[n - 1] - cur0[0] < p.minimum_overlap)
+			{
+				if (lane == 0)
+				{
+					a.groupPass[g] = 0;
+					groupRel[g] = 0;
+				}
+				continue;
+			}
 			int uniq = 0;
 			bool tie = false;
 			int mnE = 0x7fffffff, mxE = -0x7fffffff - 1;
