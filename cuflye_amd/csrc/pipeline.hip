@@ -689,15 +689,40 @@ __global__ void gatherHitsKernel(const int64_t* qMatchOff, int32_t nQueries, con
 // After the tie replay: the hits at flagged (tied, relevant) sorted positions
 // take the extPos of the match the replay put there (curPos is the key's, the
 // same for every member of a tie).
+// The flags of positions [4t, 4t + 4) for thread t: one 16-byte load (flag
+// arrays come from the 256-byte-aligned batch buffers), zero past n.
+__device__ inline int4 flags4(const int32_t* f, int64_t j, int64_t n)
+{
+	if (j + 4 <= n) return *reinterpret_cast<const int4*>(f + j);
+	int4 r = make_int4(0, 0, 0, 0);
+	if (j < n) r.x = f[j];
+	if (j + 1 < n) r.y = f[j + 1];
+	if (j + 2 < n) r.z = f[j + 2];
+	return r;
+}
+
+__device__ inline int flagAt(const int4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// Four positions per thread (the flags are sparse: most threads read their
+// 16 bytes and stop).
 __global__ void fixTiedHitsKernel(const int64_t* qMatchOff, int32_t nQueries, const int32_t* dupFlag,
 								  const uint32_t* val, const int32_t* mext, int32_t* hext, int64_t nMatches)
 {
-	const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (i >= nMatches || !dupFlag[i]) return;
-	const int q = findSegWave(qMatchOff, nQueries, i);
-	hext[i] = mext[qMatchOff[q] + val[i]];
+	const int64_t j = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+	if (j >= nMatches) return;
+	const int4 f = flags4(dupFlag, j, nMatches);
+	for (int k = 0; k < 4; ++k)
+	{
+		if (!flagAt(f, k)) continue;
+		const int64_t i = j + k;
+		const int q = findSegWave(qMatchOff, nQueries, i);
+		hext[i] = mext[qMatchOff[q] + val[i]];
+	}
 }
 
+// One position per thread: group starts are dense (one in about 40
+// positions), and four per thread measured slower (a query search per
+// position round instead of one per wave).
 __global__ void buildGroupsKernel(const int64_t* qMatchOff, int32_t nQueries, const uint64_t* key, const int32_t* groupFlag,
 								  const int64_t* groupOff, const int32_t* targetLen, GroupDesc* groups, int64_t nMatches)
 {
@@ -954,8 +979,8 @@ hipError_t launchFixTiedHits(const int64_t* qMatchOff, int32_t nQueries, const i
 							 const int32_t* mext, int32_t* hext, int64_t nMatches, hipStream_t s)
 {
 	if (nMatches == 0) return hipSuccess;
-	hipLaunchKernelGGL(fixTiedHitsKernel, gridFor(nMatches, 256), dim3(256), 0, s, qMatchOff, nQueries, dupFlag, val,
-					   mext, hext, nMatches);
+	hipLaunchKernelGGL(fixTiedHitsKernel, gridFor((nMatches + 3) / 4, 256), dim3(256), 0, s, qMatchOff, nQueries,
+					   dupFlag, val, mext, hext, nMatches);
 	return hipGetLastError();
 }
 
