@@ -81,7 +81,7 @@ def pmc_valu(kernel: str, cells: float, ms: float, launches: int):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="genome scale of the read set")
     ap.add_argument("--queries", type=int, default=0, help="limit query records (0 = all)")
