@@ -570,10 +570,10 @@ HashTable table(flyehip_ctx* c)
 
 // Single-wave blocks of the chain's work-queue kernels: 16 per CU (half the
 // CU's wave slots at their register use) for one lane; with several lanes
-// running, 28 per CU shared among them (at least 4 each), so two lanes' DPs
-// leave an eighth of the slots (and LDS) to the other kernels in flight,
-// above all the deferred heap sorts, whose blocks would otherwise wait for
-// the DPs to end.
+// running, 24 per CU shared among them (at least 4 each), so two lanes' DPs
+// leave a quarter of the slots and 64 KiB of LDS per CU to the other kernels
+// in flight, above all the deferred heap sorts, whose 64 KiB blocks would
+// otherwise wait for the DPs to end.
 int chainBlocks(flyehip_ctx* c)
 {
 	if (c->numCUs == 0)
@@ -583,7 +583,7 @@ int chainBlocks(flyehip_ctx* c)
 		c->numCUs = prop.multiProcessorCount;
 	}
 	const int lanes = c->lanesRunning;
-	return c->numCUs * (lanes > 1 ? std::max(4, 28 / lanes) : 16);
+	return c->numCUs * (lanes > 1 ? std::max(4, 24 / lanes) : 16);
 }
 
 struct Timer
